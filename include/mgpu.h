@@ -1,0 +1,179 @@
+/*
+ * mgpu.h — C ABI of the MI355X (gfx950) implementation of MG-PICOLA's
+ * particle-mesh COLA step and 2LPT initial-condition FFTs.
+ *
+ * Plain C: opaque context, plain pointers and sizes, int return codes.
+ * One context owns every device buffer of one simulation (particles, meshes,
+ * FFT tables) and one HIP stream; calls on a context are serialised on that
+ * stream and must come from one host thread.  Host pointers passed in are
+ * read/written synchronously before the call returns.
+ *
+ * Each entry point replaces one global-state routine of the reference
+ * (cited below as file:line in HAWinther/MG-PICOLA-PUBLIC).  The reference
+ * reports errors by FatalError -> MPI_Abort (src/auxPM.c:668-673); here every
+ * call returns MGPU_OK (0) or a negative MGPU_E_* code and mgpu_last_error()
+ * describes the failure, so a host wrapper can keep the abort behaviour.
+ *
+ * Layouts:
+ *   particles  SoA float, x[0..np), y[0..np), z[0..np) concatenated ([3][np]);
+ *              particle i is Lagrangian lattice point i = (ix*Ns + iy)*Ns + iz
+ *              (src/main.c:257-309) and is never reordered.
+ *   meshes     returned to the host in the reference's FFTW layout without the
+ *              ghost slab: real [N][N][N], complex [N][N][N/2+1] (re,im).
+ *   per-|n|^2 tables  t[n2], n2 = nx^2+ny^2+nz^2 over integer wavenumbers,
+ *              length mgpu_n2_table_len(N) = 3*(N/2)^2 + 1.
+ */
+#ifndef MGPU_H
+#define MGPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MGPU_OK 0
+#define MGPU_E_INVALID (-1)     /* bad argument or unsupported size */
+#define MGPU_E_NOMEM (-2)       /* device allocation failed */
+#define MGPU_E_HIP (-3)         /* HIP runtime error */
+#define MGPU_E_STATE (-4)       /* call out of order (e.g. forces before density) */
+#define MGPU_E_NODEVICE (-5)    /* no GPU visible */
+
+/* Gravity models: the compile-time MODEL= switch of the reference (Makefile:33-110). */
+enum { MGPU_LCDM = 0, MGPU_FOFR = 1, MGPU_DGP = 2, MGPU_BRANSDICKE = 3, MGPU_MBETA = 4 };
+/* Density-screening smoothing filter (src/mg.h:318-338). */
+enum { MGPU_FILTER_NONE = 0, MGPU_FILTER_GAUSSIAN = 1, MGPU_FILTER_TOPHAT = 2, MGPU_FILTER_SHARPK = 3 };
+/* Mesh ids for mgpu_get_mesh / mgpu_set_mesh. */
+enum {
+  MGPU_MESH_DENSITY_X = 0,  /* delta(x) after PtoMesh's deposit (real) */
+  MGPU_MESH_DENSITY_K = 1,  /* delta(k) after the r2c (complex), P3D */
+  MGPU_MESH_PHI_K = 2,      /* fifth-force potential phi(k) (complex), P3D_mgarray_two */
+  MGPU_MESH_FORCE_X = 3,    /* force meshes after the c2r (real), N11 N12 N13 */
+  MGPU_MESH_FORCE_Y = 4,
+  MGPU_MESH_FORCE_Z = 5,
+  MGPU_MESH_CDELTA1 = 6,    /* stored 1LPT delta(k) (complex), cdelta_cdm */
+  MGPU_MESH_CDELTA2 = 7     /* stored 2LPT -source(k) (complex), cdelta_cdm2 */
+};
+/* Per-particle LPT slots.  Not scale-dependent: slot 0 = P.D (ZA), slot 1 = P.D2
+ * (2LPT).  Scale-dependent: slot 0 = D or ddDddy field, slot 1 = dDdy or deltaD
+ * field, each the sum of the 1LPT and 2LPT fields (src/vars.h:305-312). */
+enum { MGPU_LPT_SLOT0 = 0, MGPU_LPT_SLOT1 = 1 };
+
+typedef struct {
+  int nmesh;               /* Nmesh: 2^a 3^b, 8 <= Nmesh <= 2048 */
+  int nsample;             /* Nsample: particles per dimension */
+  double box;              /* Box [Mpc/h] */
+  double omega;            /* Omega_m */
+  int model;               /* MGPU_LCDM ... */
+  int mg_active;           /* modified_gravity_active */
+  int include_screening;
+  int filter;              /* MGPU_FILTER_* for DGP smoothing */
+  int scale_dependent;     /* SCALEDEPENDENT build */
+  int use_cola;            /* UseCOLA */
+  /* P(k) binning after adjust_pofk_parameters (src/compute_pofk.c:758-805);
+   * kmin/kmax in integer-wavenumber units */
+  int pofk_nbins, pofk_bintype, pofk_subtract_shotnoise;
+  double pofk_kmin, pofk_kmax;
+} mgpu_params;
+
+/* Scalars of the model hooks at a = aexp_global, evaluated by the host
+ * (src/user_defined_functions.h:413-773). */
+typedef struct {
+  double a;
+  double coupling;         /* coupling_function(a) */
+  double mass2;            /* mass2_of_a(a) = m^2(a)/H0^2 */
+  double phi_crit;         /* screening threshold of screening_factor_potential at a */
+  double dgp_fac;          /* 8/9 Omega (rcH0/beta_DGP(a))^2 (screening_factor_density) */
+  double rsmooth;          /* Rsmooth [Mpc/h] */
+  double geff;             /* GeffoverG(a, 0), JBD */
+} mgpu_mg;
+
+/* Binned power spectrum as compute_power_spectrum forms it (src/compute_pofk.c:71-236). */
+typedef struct {
+  double *pofk;            /* [nbins] normalised P(k) [(Mpc/h)^3], shot noise handled */
+  double *kmean;           /* [nbins] mean |k| of the bin [h/Mpc] */
+  double *nmodes;          /* [nbins] mode count (weights 1 or 2) */
+} mgpu_pofk;
+
+size_t mgpu_n2_table_len(int nmesh);
+const char *mgpu_last_error(const void *ctx_or_null);
+int mgpu_device_info(char *name, size_t name_len, size_t *global_mem, int *cu_count, size_t *lds_per_block);
+
+/* ---- context; replaces initialize_ffts/initialize_parts + the MEMORY_MODE
+ *      allocations of GetDisplacements (src/2LPT.c:47-176, src/auxPM.c:46-102) ---- */
+typedef struct mgpu_ctx mgpu_ctx;
+int mgpu_create(const mgpu_params *p, mgpu_ctx **out);
+void mgpu_destroy(mgpu_ctx *ctx);
+int mgpu_sync(mgpu_ctx *ctx);
+
+/* ---- particle state (struct part_data, src/vars.h:293-353) ---- */
+int mgpu_set_particles(mgpu_ctx *ctx, const float *pos, const float *vel);
+int mgpu_get_particles(mgpu_ctx *ctx, float *pos, float *vel);
+int mgpu_set_lpt(mgpu_ctx *ctx, int slot, const float *d);
+int mgpu_get_lpt(mgpu_ctx *ctx, int slot, float *d);
+int mgpu_get_disp(mgpu_ctx *ctx, float *disp);            /* Disp[3] of the last MtoParticles */
+
+/* ---- PM step ---- */
+/* PtoMesh: CIC deposit (deterministic 64-bit fixed point), delta(x) copy for
+ * the MG solver, r2c, and P(k) of delta(k) when pk != NULL.  src/auxPM.c:280-432 */
+int mgpu_pm_density(mgpu_ctx *ctx, mgpu_pofk *pk);
+/* The two halves of mgpu_pm_density: the CIC deposit alone (delta(x) stays in
+ * real layout, readable as MGPU_MESH_DENSITY_X; src/auxPM.c:292-362), and the
+ * r2c + P(k) of the deposited field (src/auxPM.c:365-372). */
+int mgpu_pm_deposit(mgpu_ctx *ctx);
+int mgpu_pm_transform(mgpu_ctx *ctx, mgpu_pofk *pk);
+/* compute_power_spectrum of the current delta(k).  src/compute_pofk.c:71 */
+int mgpu_power_spectrum(mgpu_ctx *ctx, mgpu_pofk *pk);
+/* ComputeFifthForce: f(R)/m-beta potential screening, DGP density screening,
+ * JBD Geff(a).  src/user_defined_functions.h:687-719, src/mg.h:21-260 */
+int mgpu_fifth_force(mgpu_ctx *ctx, const mgpu_mg *mg);
+/* Forces: Green's function -i n/n^2 and 3 c2r.  src/auxPM.c:437-555 */
+int mgpu_forces(mgpu_ctx *ctx);
+/* MtoParticles: CIC gather into Disp, mean over particles.  src/auxPM.c:560-644 */
+int mgpu_mesh_to_particles(mgpu_ctx *ctx, double sumD[3]);
+/* GetDisplacements = PtoMesh + ComputeFifthForce + Forces + MtoParticles.
+ * mg may be NULL when modified gravity is off.  src/auxPM.c:37-103 */
+int mgpu_get_displacements(mgpu_ctx *ctx, const mgpu_mg *mg, mgpu_pofk *pk, double sumD[3]);
+/* Kick: v += dda [-1.5 Omega (Disp - sumD) - UseCOLA lpt/A]; returns the new
+ * mean velocity.  Not scale-dependent lpt = slot0*s1 + slot1*s2 with
+ * s = (ddDddy, ddD2ddy); scale-dependent lpt = slot0.  src/main.c:688-742 */
+int mgpu_kick(mgpu_ctx *ctx, double dda, double A, double s1, double s2, const double sumD[3], double sumV[3]);
+/* Drift: x += (v - sumV) dyyy + UseCOLA lpt, periodic wrap.  Not
+ * scale-dependent lpt = slot0*s1 + slot1*s2 with s = (deltaD, deltaD2);
+ * scale-dependent lpt = slot1.  src/main.c:747-787 */
+int mgpu_drift(mgpu_ctx *ctx, double dyyy, double s1, double s2, const double sumV[3]);
+
+/* ---- initial conditions ---- */
+/* displacement_fields (Gaussian): seed table + per-column RANLUX stream,
+ * delta(k) at z = 0, ZA and 2LPT displacements.  Not scale-dependent: fills
+ * LPT slots 0/1 with ZA and 2LPT lattice displacements (mean removed).
+ * Scale-dependent: keeps delta(k) and -source(k) (cdelta_cdm, cdelta_cdm2).
+ * pk_n2[n2] = PowerSpec(2 pi/Box sqrt(n2)) including any MG rescaling.
+ * src/2LPT.c:185-495, src/2LPT.c:1204-1520 */
+int mgpu_ic_displacement_fields(mgpu_ctx *ctx, unsigned long seed, int sphere_mode, int amplitude_fixed,
+                                int inverted, const double *pk_n2);
+/* Particle init: Vel = 0 (COLA) and Pos = q + D*Di + D2*Di2 (scale-dependent:
+ * q + slot0), periodic wrap.  src/main.c:226-309 */
+int mgpu_init_particles(mgpu_ctx *ctx, double Di, double Di2);
+/* from_cdisp_store_to_ZA + assign_displacment_field_to_particles for both LPT
+ * orders at once: slot = field(cdelta1, g1) + field(cdelta2, g2), each field
+ * i k/k^2 delta(k) g(|k|) c2r'ed, read at the lattice and mean-removed.
+ * g1_n2/g2_n2 already include normfactor (1 and -3/7/N^3).
+ * src/2LPT.c:1539-1735, src/2LPT.c:1758-2005 */
+int mgpu_lpt_fields(mgpu_ctx *ctx, int slot, const double *g1_n2, const double *g2_n2);
+
+/* ---- inspection (tests, output) ---- */
+int mgpu_get_mesh(mgpu_ctx *ctx, int mesh, double *host);
+int mgpu_set_mesh(mgpu_ctx *ctx, int mesh, const double *host);
+
+/* ---- per-phase device time of the last calls, ms (timer categories of src/timer.h:11-29) ---- */
+typedef struct {
+  double ptomesh, fft, pofk, modified_gravity, forces, mtoparticles, kick, drift, lpt_fields, ic;
+} mgpu_timers;
+int mgpu_get_timers(mgpu_ctx *ctx, mgpu_timers *t, int reset);
+int mgpu_enable_timers(mgpu_ctx *ctx, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

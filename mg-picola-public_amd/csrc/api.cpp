@@ -1,0 +1,391 @@
+// C ABI of libmgpu (include/mgpu.h): context management, particle I/O and the
+// entry points that replace the reference's PM/COLA routines.
+#include "mgpu_internal.h"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace {
+thread_local std::string g_err;
+
+int check_ctx(mgpu_ctx *ctx) { return ctx ? MGPU_OK : MGPU_E_INVALID; }
+
+int alloc(mgpu_ctx *ctx, void **p, size_t bytes, const char *what) {
+  if (hipMalloc(p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return mgpu_fail(ctx, MGPU_E_NOMEM, std::string("device allocation failed: ") + what + " (" +
+                                            std::to_string(bytes) + " bytes)");
+  }
+  return MGPU_OK;
+}
+}  // namespace
+
+int mgpu_fail(mgpu_ctx *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->err = msg;
+  g_err = msg;
+  return code;
+}
+
+// Each scope records its own event pair so timers may nest (FFT time is also
+// counted inside PtoMesh and Forces, as in the reference's timer.c).
+PhaseTimer::PhaseTimer(mgpu_ctx *c, double *f) : ctx(c), field(f) {
+  if (!ctx || !ctx->timers_on) return;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    (void)hipGetLastError();
+    e0 = e1 = nullptr;
+    return;
+  }
+  (void)hipEventRecord(e0, ctx->stream);
+}
+PhaseTimer::~PhaseTimer() {
+  if (!e0) return;
+  float ms = 0.f;
+  if (hipEventRecord(e1, ctx->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+      hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+    *field += ms;
+  (void)hipGetLastError();
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+}
+
+extern "C" {
+
+size_t mgpu_n2_table_len(int nmesh) { return (size_t)3 * (nmesh / 2) * (nmesh / 2) + 1; }
+
+const char *mgpu_last_error(const void *ctx) {
+  if (ctx) return ((const mgpu_ctx *)ctx)->err.c_str();
+  return g_err.c_str();
+}
+
+int mgpu_device_info(char *name, size_t name_len, size_t *global_mem, int *cu_count, size_t *lds_per_block) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    (void)hipGetLastError();
+    g_err = "no HIP device visible";
+    return MGPU_E_NODEVICE;
+  }
+  hipDeviceProp_t pr;
+  if (hipGetDeviceProperties(&pr, 0) != hipSuccess) return MGPU_E_HIP;
+  if (name && name_len) {
+    std::string s = std::string(pr.name) + " (" + pr.gcnArchName + ")";
+    std::strncpy(name, s.c_str(), name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  if (global_mem) *global_mem = pr.totalGlobalMem;
+  if (cu_count) *cu_count = pr.multiProcessorCount;
+  if (lds_per_block) *lds_per_block = pr.sharedMemPerBlock;
+  return MGPU_OK;
+}
+
+int mgpu_create(const mgpu_params *p, mgpu_ctx **out) {
+  if (!p || !out) return mgpu_fail(nullptr, MGPU_E_INVALID, "null argument");
+  *out = nullptr;
+  if (!fft_size_supported(p->nmesh))
+    return mgpu_fail(nullptr, MGPU_E_INVALID, "Nmesh must be 2^a or 3*2^a in [8, 2048], got " + std::to_string(p->nmesh));
+  if (p->nsample < 1 || p->nsample > 2048 || p->box <= 0)
+    return mgpu_fail(nullptr, MGPU_E_INVALID, "bad Nsample or Box");
+  if (p->model < MGPU_LCDM || p->model > MGPU_MBETA) return mgpu_fail(nullptr, MGPU_E_INVALID, "unknown model");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    return mgpu_fail(nullptr, MGPU_E_NODEVICE, "no HIP device visible");
+  }
+  mgpu_ctx *ctx = new mgpu_ctx();
+  ctx->p = *p;
+  ctx->N = p->nmesh;
+  ctx->Np = p->nsample;
+  ctx->np = (size_t)p->nsample * p->nsample * p->nsample;
+  int rc;
+  auto fail = [&](int code) {
+    g_err = ctx->err;
+    mgpu_destroy(ctx);
+    return code;
+  };
+  if (hipGetDevice(&ctx->device) != hipSuccess) return fail(mgpu_fail(ctx, MGPU_E_HIP, "hipGetDevice failed"));
+  hipDeviceProp_t pr;
+  if (hipGetDeviceProperties(&pr, ctx->device) == hipSuccess) ctx->lds_max = pr.sharedMemPerBlock;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(mgpu_fail(ctx, MGPU_E_HIP, "stream creation failed"));
+  if ((rc = fft_plan_init(ctx, &ctx->fft, ctx->N))) return fail(rc);
+  ctx->HP = ctx->fft.hp;
+  ctx->slot_elems = (size_t)ctx->N * ctx->N * ctx->HP;
+  const size_t pb = 3 * ctx->np * sizeof(float);
+  if ((rc = alloc(ctx, (void **)&ctx->pos, pb, "positions"))) return fail(rc);
+  if ((rc = alloc(ctx, (void **)&ctx->vel, pb, "velocities"))) return fail(rc);
+  if ((rc = alloc(ctx, (void **)&ctx->disp, pb, "Disp"))) return fail(rc);
+  for (int s = 0; s < 2; s++)
+    if ((rc = alloc(ctx, (void **)&ctx->lpt[s], pb, "LPT slot"))) return fail(rc);
+  const size_t sb = ctx->slot_elems * sizeof(float2);
+  for (int s = 0; s < 3; s++)
+    if ((rc = alloc(ctx, (void **)&ctx->slot[s], sb, "mesh slot"))) return fail(rc);
+  float2 *force = nullptr;
+  if ((rc = alloc(ctx, (void **)&force, 3 * sb, "force meshes"))) return fail(rc);
+  for (int a = 0; a < 3; a++) ctx->slot[3 + a] = force + a * ctx->slot_elems;
+  ctx->acc = (unsigned long long *)force;  // needs 8 N^3 bytes <= 2 slots
+  if (p->scale_dependent)
+    for (int s = 0; s < 2; s++)
+      if ((rc = alloc(ctx, (void **)&ctx->cdelta[s], sb, "stored delta(k)"))) return fail(rc);
+  ctx->partial_len = 3 * (size_t)std::max<size_t>(cdiv(ctx->np, 256), (size_t)ctx->N) + 3 * 1024;
+  if ((rc = alloc(ctx, (void **)&ctx->partial, ctx->partial_len * sizeof(double), "reduction scratch"))) return fail(rc);
+  if ((rc = alloc(ctx, (void **)&ctx->dsum, 8 * sizeof(double), "sums"))) return fail(rc);
+  const int nb = std::max(1, p->pofk_nbins);
+  if ((rc = alloc(ctx, (void **)&ctx->pk, 3 * (size_t)nb * sizeof(double), "P(k) bins"))) return fail(rc);
+  ctx->n2len = mgpu_n2_table_len(ctx->N);
+  for (int s = 0; s < 2; s++)
+    if ((rc = alloc(ctx, (void **)&ctx->n2tab[s], ctx->n2len * sizeof(double), "n2 table"))) return fail(rc);
+  if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess)
+    return fail(mgpu_fail(ctx, MGPU_E_HIP, "event creation failed"));
+  // defined contents everywhere
+  if (hipMemsetAsync(ctx->vel, 0, pb, ctx->stream) != hipSuccess ||
+      hipMemsetAsync(ctx->lpt[0], 0, pb, ctx->stream) != hipSuccess ||
+      hipMemsetAsync(ctx->lpt[1], 0, pb, ctx->stream) != hipSuccess ||
+      hipMemsetAsync(ctx->disp, 0, pb, ctx->stream) != hipSuccess)
+    return fail(mgpu_fail(ctx, MGPU_E_HIP, "memset failed"));
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(mgpu_fail(ctx, MGPU_E_HIP, "sync failed"));
+  *out = ctx;
+  return MGPU_OK;
+}
+
+void mgpu_destroy(mgpu_ctx *ctx) {
+  if (!ctx) return;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  void *ptrs[] = {ctx->pos,      ctx->vel,       ctx->disp,      ctx->lpt[0],    ctx->lpt[1],
+                  ctx->slot[0],  ctx->slot[1],   ctx->slot[2],   ctx->slot[3],   ctx->cdelta[0],
+                  ctx->cdelta[1], ctx->partial,  ctx->dsum,      ctx->pk,        ctx->n2tab[0],
+                  ctx->n2tab[1], ctx->lattice_slot};
+  for (void *q : ptrs)
+    if (q) (void)hipFree(q);
+  fft_plan_free(&ctx->fft);
+  fft_plan_free(&ctx->fft_lattice);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int mgpu_sync(mgpu_ctx *ctx) {
+  if (check_ctx(ctx)) return MGPU_E_INVALID;
+  MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MGPU_OK;
+}
+
+int mgpu_set_particles(mgpu_ctx *ctx, const float *pos, const float *vel) {
+  if (check_ctx(ctx)) return MGPU_E_INVALID;
+  const size_t b = 3 * ctx->np * sizeof(float);
+  if (pos) MGPU_HIP(ctx, hipMemcpyAsync(ctx->pos, pos, b, hipMemcpyHostToDevice, ctx->stream));
+  if (vel) MGPU_HIP(ctx, hipMemcpyAsync(ctx->vel, vel, b, hipMemcpyHostToDevice, ctx->stream));
+  MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MGPU_OK;
+}
+
+int mgpu_get_particles(mgpu_ctx *ctx, float *pos, float *vel) {
+  if (check_ctx(ctx)) return MGPU_E_INVALID;
+  const size_t b = 3 * ctx->np * sizeof(float);
+  if (pos) MGPU_HIP(ctx, hipMemcpyAsync(pos, ctx->pos, b, hipMemcpyDeviceToHost, ctx->stream));
+  if (vel) MGPU_HIP(ctx, hipMemcpyAsync(vel, ctx->vel, b, hipMemcpyDeviceToHost, ctx->stream));
+  MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MGPU_OK;
+}
+
+int mgpu_set_lpt(mgpu_ctx *ctx, int slot, const float *d) {
+  if (check_ctx(ctx) || slot < 0 || slot > 1 || !d) return mgpu_fail(ctx, MGPU_E_INVALID, "bad LPT slot");
+  MGPU_HIP(ctx, hipMemcpyAsync(ctx->lpt[slot], d, 3 * ctx->np * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MGPU_OK;
+}
+
+int mgpu_get_lpt(mgpu_ctx *ctx, int slot, float *d) {
+  if (check_ctx(ctx) || slot < 0 || slot > 1 || !d) return mgpu_fail(ctx, MGPU_E_INVALID, "bad LPT slot");
+  MGPU_HIP(ctx, hipMemcpyAsync(d, ctx->lpt[slot], 3 * ctx->np * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MGPU_OK;
+}
+
+int mgpu_get_disp(mgpu_ctx *ctx, float *d) {
+  if (check_ctx(ctx) || !d) return MGPU_E_INVALID;
+  MGPU_HIP(ctx, hipMemcpyAsync(d, ctx->disp, 3 * ctx->np * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MGPU_OK;
+}
+
+int mgpu_pm_deposit(mgpu_ctx *ctx) {
+  if (check_ctx(ctx)) return MGPU_E_INVALID;
+  int rc = pm_deposit(ctx);
+  if (rc) return rc;
+  ctx->have_deposit = true;
+  ctx->have_density = false;
+  ctx->have_forces = false;
+  ctx->have_phik = false;
+  MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MGPU_OK;
+}
+
+int mgpu_pm_transform(mgpu_ctx *ctx, mgpu_pofk *pk) {
+  if (check_ctx(ctx)) return MGPU_E_INVALID;
+  if (!ctx->have_deposit) return mgpu_fail(ctx, MGPU_E_STATE, "no deposited density: call mgpu_pm_deposit first");
+  int rc;
+  {
+    PhaseTimer pt(ctx, &ctx->t.ptomesh);
+    if ((rc = fft_r2c_3d(ctx, &ctx->fft, ctx->slot[0]))) return rc;
+  }
+  ctx->have_deposit = false;
+  ctx->have_density = true;
+  if (pk && (rc = pm_pofk(ctx, ctx->slot[0], pk))) return rc;
+  return MGPU_OK;
+}
+
+int mgpu_pm_density(mgpu_ctx *ctx, mgpu_pofk *pk) {
+  if (check_ctx(ctx)) return MGPU_E_INVALID;
+  int rc;
+  if ((rc = pm_deposit(ctx))) return rc;
+  ctx->have_deposit = true;
+  ctx->have_forces = false;
+  ctx->have_phik = false;
+  return mgpu_pm_transform(ctx, pk);
+}
+
+int mgpu_power_spectrum(mgpu_ctx *ctx, mgpu_pofk *pk) {
+  if (check_ctx(ctx) || !pk) return MGPU_E_INVALID;
+  if (!ctx->have_density) return mgpu_fail(ctx, MGPU_E_STATE, "no density(k): call mgpu_pm_density first");
+  return pm_pofk(ctx, ctx->slot[0], pk);
+}
+
+int mgpu_fifth_force(mgpu_ctx *ctx, const mgpu_mg *mg) {
+  if (check_ctx(ctx)) return MGPU_E_INVALID;
+  if (!ctx->have_density) return mgpu_fail(ctx, MGPU_E_STATE, "no density(k): call mgpu_pm_density first");
+  return pm_fifth_force(ctx, mg);
+}
+
+int mgpu_forces(mgpu_ctx *ctx) {
+  if (check_ctx(ctx)) return MGPU_E_INVALID;
+  if (!ctx->have_density) return mgpu_fail(ctx, MGPU_E_STATE, "no density(k): call mgpu_pm_density first");
+  int rc = pm_forces(ctx);
+  if (rc) return rc;
+  ctx->have_forces = true;
+  return MGPU_OK;
+}
+
+int mgpu_mesh_to_particles(mgpu_ctx *ctx, double sumD[3]) {
+  if (check_ctx(ctx) || !sumD) return MGPU_E_INVALID;
+  if (!ctx->have_forces) return mgpu_fail(ctx, MGPU_E_STATE, "no force meshes: call mgpu_forces first");
+  int rc = pm_gather(ctx, sumD);
+  if (rc) return rc;
+  ctx->have_disp = true;
+  return MGPU_OK;
+}
+
+int mgpu_get_displacements(mgpu_ctx *ctx, const mgpu_mg *mg, mgpu_pofk *pk, double sumD[3]) {
+  int rc;
+  if ((rc = mgpu_pm_density(ctx, pk))) return rc;
+  if (ctx->p.mg_active && (rc = mgpu_fifth_force(ctx, mg))) return rc;
+  if ((rc = mgpu_forces(ctx))) return rc;
+  return mgpu_mesh_to_particles(ctx, sumD);
+}
+
+int mgpu_kick(mgpu_ctx *ctx, double dda, double A, double s1, double s2, const double sumD[3], double sumV[3]) {
+  if (check_ctx(ctx) || !sumD || !sumV) return MGPU_E_INVALID;
+  if (!ctx->have_disp) return mgpu_fail(ctx, MGPU_E_STATE, "no Disp: call mgpu_mesh_to_particles first");
+  return pm_kick(ctx, dda, A, s1, s2, sumD, sumV);
+}
+
+int mgpu_drift(mgpu_ctx *ctx, double dyyy, double s1, double s2, const double sumV[3]) {
+  if (check_ctx(ctx) || !sumV) return MGPU_E_INVALID;
+  return pm_drift(ctx, dyyy, s1, s2, sumV);
+}
+
+int mgpu_ic_displacement_fields(mgpu_ctx *ctx, unsigned long seed, int sphere_mode, int amplitude_fixed, int inverted,
+                                const double *pk_n2) {
+  if (check_ctx(ctx) || !pk_n2) return MGPU_E_INVALID;
+  return ic_displacement_fields(ctx, seed, sphere_mode, amplitude_fixed, inverted, pk_n2);
+}
+
+int mgpu_init_particles(mgpu_ctx *ctx, double Di, double Di2) {
+  if (check_ctx(ctx)) return MGPU_E_INVALID;
+  int rc = ic_init_particles(ctx, Di, Di2);
+  if (rc) return rc;
+  MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MGPU_OK;
+}
+
+int mgpu_lpt_fields(mgpu_ctx *ctx, int slot, const double *g1_n2, const double *g2_n2) {
+  if (check_ctx(ctx) || slot < 0 || slot > 1 || !g1_n2 || !g2_n2) return MGPU_E_INVALID;
+  return ic_lpt_fields(ctx, slot, g1_n2, g2_n2);
+}
+
+// ---- mesh inspection: convert between the device layout and the FFTW layout (no ghost slab)
+static float2 *mesh_ptr(mgpu_ctx *ctx, int mesh, bool &is_real) {
+  is_real = false;
+  switch (mesh) {
+    case MGPU_MESH_DENSITY_X: is_real = true; return ctx->slot[0];
+    case MGPU_MESH_DENSITY_K: return ctx->slot[0];
+    case MGPU_MESH_PHI_K: return ctx->slot[1];
+    case MGPU_MESH_FORCE_X: is_real = true; return ctx->slot[3];
+    case MGPU_MESH_FORCE_Y: is_real = true; return ctx->slot[4];
+    case MGPU_MESH_FORCE_Z: is_real = true; return ctx->slot[5];
+    case MGPU_MESH_CDELTA1: return ctx->cdelta[0] ? ctx->cdelta[0] : ctx->slot[0];
+    case MGPU_MESH_CDELTA2: return ctx->cdelta[1];
+    default: return nullptr;
+  }
+}
+
+int mgpu_get_mesh(mgpu_ctx *ctx, int mesh, double *host) {
+  if (check_ctx(ctx) || !host) return MGPU_E_INVALID;
+  bool is_real;
+  float2 *src = mesh_ptr(ctx, mesh, is_real);
+  if (!src) return mgpu_fail(ctx, MGPU_E_INVALID, "mesh not available");
+  std::vector<float2> h(ctx->slot_elems);
+  MGPU_HIP(ctx, hipMemcpyAsync(h.data(), src, h.size() * sizeof(float2), hipMemcpyDeviceToHost, ctx->stream));
+  MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const size_t N = ctx->N, HP = ctx->HP, H = N / 2 + 1;
+  const float *hf = (const float *)h.data();
+  for (size_t row = 0; row < N * N; row++) {
+    if (is_real)
+      for (size_t z = 0; z < N; z++) host[row * N + z] = hf[row * 2 * HP + z];
+    else
+      for (size_t k = 0; k < H; k++) {
+        host[2 * (row * H + k)] = h[row * HP + k].x;
+        host[2 * (row * H + k) + 1] = h[row * HP + k].y;
+      }
+  }
+  return MGPU_OK;
+}
+
+int mgpu_set_mesh(mgpu_ctx *ctx, int mesh, const double *host) {
+  if (check_ctx(ctx) || !host) return MGPU_E_INVALID;
+  bool is_real;
+  float2 *dst = mesh_ptr(ctx, mesh, is_real);
+  if (!dst) return mgpu_fail(ctx, MGPU_E_INVALID, "mesh not available");
+  std::vector<float2> h(ctx->slot_elems, make_float2(0.f, 0.f));
+  const size_t N = ctx->N, HP = ctx->HP, H = N / 2 + 1;
+  float *hf = (float *)h.data();
+  for (size_t row = 0; row < N * N; row++) {
+    if (is_real)
+      for (size_t z = 0; z < N; z++) hf[row * 2 * HP + z] = (float)host[row * N + z];
+    else
+      for (size_t k = 0; k < H; k++)
+        h[row * HP + k] = make_float2((float)host[2 * (row * H + k)], (float)host[2 * (row * H + k) + 1]);
+  }
+  MGPU_HIP(ctx, hipMemcpyAsync(dst, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice, ctx->stream));
+  MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (mesh == MGPU_MESH_DENSITY_K) ctx->have_density = true;
+  if (mesh == MGPU_MESH_PHI_K) ctx->have_phik = true;
+  if (mesh >= MGPU_MESH_FORCE_X && mesh <= MGPU_MESH_FORCE_Z) ctx->have_forces = true;
+  return MGPU_OK;
+}
+
+int mgpu_get_timers(mgpu_ctx *ctx, mgpu_timers *t, int reset) {
+  if (check_ctx(ctx) || !t) return MGPU_E_INVALID;
+  *t = ctx->t;
+  if (reset) ctx->t = mgpu_timers{};
+  return MGPU_OK;
+}
+
+int mgpu_enable_timers(mgpu_ctx *ctx, int on) {
+  if (check_ctx(ctx)) return MGPU_E_INVALID;
+  ctx->timers_on = on != 0;
+  return MGPU_OK;
+}
+
+}  // extern "C"

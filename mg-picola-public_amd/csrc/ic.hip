@@ -1,0 +1,371 @@
+// 2LPT initial conditions and scale-dependent LPT fields for gfx950.
+//
+//   k_ic_delta         displacement_fields' Gaussian mode loop (src/2LPT.c:337-495):
+//                      one thread per (i, j) column re-seeds RANLUX from the seed
+//                      table and draws (phase, amplitude) for kz < N/2 in order.
+//   k_lpt_spectrum     i k_a/k^2 delta(k) g(|k|) and the displacement-gradient
+//                      spectra (src/2LPT.c:1224-1277, 1311-1373, 1583-1629),
+//                      2LPT.c wavenumbers (Nyquist = -N/2).
+//   k_lattice_read     trilinear read-out at the Lagrangian lattice
+//                      (src/2LPT.c:1407-1476, 1657-1702) + per-block sums for the
+//                      mean removal (src/2LPT.c:1493-1508, 1713-1729).
+#include "mgpu_internal.h"
+#include "ranlux.h"
+
+#include <cmath>
+#include <vector>
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// one thread per column (i, j) of the N x N grid
+__global__ __launch_bounds__(kBlock) void k_ic_delta(const unsigned *__restrict__ seedtable, int N, int Np, int HP,
+                                                     double box, int sphere_mode, int amp_fixed, int inverted,
+                                                     const double *__restrict__ pk_n2, float2 *__restrict__ cd) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= N * N) return;
+  const int i = col / N, j = col % N;
+  Ranlxd1 r;
+  r.seed(seedtable[col]);
+  const double bnorm = pow(box, -1.5);
+  for (int k = 0; k < N / 2; k++) {
+    const double phase = r.uniform() * 2 * MGPU_PI;
+    double ampl = 0.0;
+    if (!amp_fixed) {
+      do {
+        ampl = r.uniform();
+      } while (ampl == 0);
+    }
+    if (i == N / 2 || j == N / 2 || k == N / 2) continue;
+    if (i == 0 && j == 0 && k == 0) continue;
+    // src/2LPT.c:364-386: k-vector with index < N/2 -> index, else -(N-index); cube or sphere cut
+    // same double expressions as the reference so the cut rounds identically
+    const int ni = i < N / 2 ? i : -(N - i), nj = j < N / 2 ? j : -(N - j), nk = k;
+    const double kx = ni * 2 * MGPU_PI / box, ky = nj * 2 * MGPU_PI / box, kz = nk * 2 * MGPU_PI / box;
+    if (sphere_mode == 1) {
+      const double kmag = sqrt(kx * kx + ky * ky + kz * kz);
+      if (kmag * box / (2 * MGPU_PI) > Np / 2) continue;
+    } else {
+      if (fabs(kx) * box / (2 * MGPU_PI) > Np / 2) continue;
+      if (fabs(ky) * box / (2 * MGPU_PI) > Np / 2) continue;
+      if (fabs(kz) * box / (2 * MGPU_PI) > Np / 2) continue;
+    }
+    double p_of_k = pk_n2[ni * ni + nj * nj + nk * nk];
+    if (!amp_fixed) p_of_k *= -log(ampl);
+    double delta = bnorm * sqrt(p_of_k);
+    if (inverted) delta *= -1.0;
+    const float2 v = make_float2((float)(delta * cos(phase)), (float)(delta * sin(phase)));
+    const float2 vc = make_float2(v.x, -v.y);
+    if (k > 0) {
+      cd[((size_t)i * N + j) * HP + k] = v;
+    } else if (i == 0) {
+      if (j >= N / 2) continue;
+      cd[((size_t)0 * N + j) * HP] = v;
+      cd[((size_t)0 * N + (N - j)) * HP] = vc;
+    } else {
+      if (i >= N / 2) continue;
+      const int jj = (N - j) == N ? 0 : N - j;
+      cd[((size_t)i * N + j) * HP] = v;
+      cd[((size_t)(N - i) * N + jj) * HP] = vc;
+    }
+  }
+}
+
+// 2LPT.c wavenumber component of index i (Nyquist -> -N/2), in h/Mpc
+__device__ __forceinline__ double kc(int i, int N, double kf) { return (i < N / 2 ? i : -(N - i)) * kf; }
+__device__ __forceinline__ int nabs(int i, int N) { return i < N / 2 ? i : N - i; }
+
+enum { SPEC_DISP = 0, SPEC_GRAD = 1, SPEC_DISP2 = 2, SPEC_FIELD = 3, SPEC_NEG = 4 };
+
+// Spectral kernels of the IC / LPT fields.  mode:
+//   SPEC_DISP   out = i k_a/k^2 cd                           (cdisp, src/2LPT.c:417-490)
+//   SPEC_GRAD   out = i k_b (i k_a/k^2 cd) = -k_a k_b/k^2 cd  (cdigrad, src/2LPT.c:1247-1271)
+//   SPEC_DISP2  out = -i k_a/k^2 src                        (cdisp2, src/2LPT.c:1359-1370)
+//   SPEC_FIELD  out = i k_a/k^2 (g1 cd + g2 cd2)            (from_cdisp_store_to_ZA, both orders)
+//   SPEC_NEG    out = -src, 0 at k = 0                      (cdelta_cdm2, src/2LPT.c:1336-1345)
+__global__ __launch_bounds__(kBlock) void k_lpt_spectrum(int mode, const float2 *__restrict__ cd,
+                                                         const float2 *__restrict__ cd2, float2 *__restrict__ out,
+                                                         int N, int HP, double kf, int a, int b,
+                                                         const double *__restrict__ g1,
+                                                         const double *__restrict__ g2) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)N * N * HP) return;
+  const int k = (int)(e % HP);
+  const size_t row = e / HP;
+  const int j = (int)(row % N), i = (int)(row / N);
+  float2 r = make_float2(0.f, 0.f);
+  if (k <= N / 2) {
+    const double kv[3] = {kc(i, N, kf), kc(j, N, kf), kc(k, N, kf)};
+    const double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+    if (k2 > 0.0) {
+      double re = 0, im = 0;
+      if (mode == SPEC_DISP || mode == SPEC_GRAD) {
+        const float2 d = cd[e];
+        // cdisp = (-k_a/k2 * im, k_a/k2 * re)
+        re = -kv[a] / k2 * d.y;
+        im = kv[a] / k2 * d.x;
+        if (mode == SPEC_GRAD) {  // times i k_b
+          const double r2 = -im * kv[b], i2 = re * kv[b];
+          re = r2;
+          im = i2;
+        }
+      } else if (mode == SPEC_DISP2) {
+        const float2 s = cd[e];
+        re = s.y * kv[a] / k2;
+        im = -s.x * kv[a] / k2;
+      } else if (mode == SPEC_FIELD) {
+        const int n2 = nabs(i, N) * nabs(i, N) + nabs(j, N) * nabs(j, N) + nabs(k, N) * nabs(k, N);
+        const float2 d = cd[e];
+        double dr = d.x * g1[n2], di = d.y * g1[n2];
+        if (cd2) {
+          const float2 d2 = cd2[e];
+          dr += d2.x * g2[n2];
+          di += d2.y * g2[n2];
+        }
+        re = -kv[a] / k2 * di;
+        im = kv[a] / k2 * dr;
+      } else {  // SPEC_NEG
+        const float2 s = cd[e];
+        re = -s.x;
+        im = -s.y;
+      }
+      r = make_float2((float)re, (float)im);
+    }
+  }
+  out[e] = r;
+}
+
+// 2LPT source on padded real meshes: src = d00 (d11 + d22) + d11 d22 (first
+// call), src -= d*d (off-diagonal calls).  src/2LPT.c:1282-1290
+__global__ __launch_bounds__(kBlock) void k_source_diag(const float *__restrict__ d00, const float *__restrict__ d11,
+                                                        const float *__restrict__ d22, float *__restrict__ src,
+                                                        size_t n) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const double a = d00[e], b = d11[e], c = d22[e];
+  src[e] = (float)(a * (b + c) + b * c);
+}
+__global__ __launch_bounds__(kBlock) void k_source_off(const float *__restrict__ d, float *__restrict__ src,
+                                                       size_t n) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const double v = d[e];
+  src[e] = (float)((double)src[e] - v * v);
+}
+
+// trilinear lattice read-out, src/2LPT.c:1407-1476.  out[q] = scale * value
+__global__ __launch_bounds__(kBlock) void k_lattice_read(const float *__restrict__ m, int N, int Np, int HP2,
+                                                         double scale, float *__restrict__ out,
+                                                         double *__restrict__ partial) {
+  __shared__ double red[kBlock];
+  const size_t np = (size_t)Np * Np * Np;
+  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double val = 0.0;
+  if (q < np) {
+    const int p = (int)(q % Np), mm = (int)((q / Np) % Np), n = (int)(q / ((size_t)Np * Np));
+    double u = (double)(n * N) / (double)Np, v = (double)(mm * N) / (double)Np, w = (double)(p * N) / (double)Np;
+    int i = (int)u, j = (int)v, k = (int)w;
+    if (i >= N) i = N - 1;
+    if (j >= N) j = N - 1;
+    if (k >= N) k = N - 1;
+    u -= i;
+    v -= j;
+    w -= k;
+    const int ii = i + 1 >= N ? 0 : i + 1, jj = j + 1 >= N ? 0 : j + 1, kk = k + 1 >= N ? 0 : k + 1;
+    auto at = [&](int x, int y, int z) { return (double)m[((size_t)x * N + y) * HP2 + z]; };
+    val = at(i, j, k) * ((1 - u) * (1 - v) * (1 - w)) + at(i, j, kk) * ((1 - u) * (1 - v) * w) +
+          at(i, jj, k) * ((1 - u) * v * (1 - w)) + at(i, jj, kk) * ((1 - u) * v * w) +
+          at(ii, j, k) * (u * (1 - v) * (1 - w)) + at(ii, j, kk) * (u * (1 - v) * w) +
+          at(ii, jj, k) * (u * v * (1 - w)) + at(ii, jj, kk) * (u * v * w);
+    val *= scale;
+    out[q] = (float)val;
+  }
+  red[threadIdx.x] = val;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[(size_t)blockIdx.x * 3] = red[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_sub_mean(float *__restrict__ v, size_t n, const double *__restrict__ sum,
+                                                     double inv_n) {
+  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  v[q] = (float)((double)v[q] - sum[0] * inv_n);
+}
+
+// sum partial[b*3] over blocks in a fixed order -> out[0]
+__global__ __launch_bounds__(1024) void k_reduce1(const double *__restrict__ partial, int nb, double *__restrict__ out) {
+  __shared__ double red[1024];
+  double s = 0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partial[(size_t)b * 3];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// particle init, src/main.c:257-309
+__global__ __launch_bounds__(kBlock) void k_init_particles(float *__restrict__ pos, float *__restrict__ vel,
+                                                           const float *__restrict__ l0,
+                                                           const float *__restrict__ l1, int Np, double box,
+                                                           int sd, double Di, double Di2, int use_cola,
+                                                           double v1, double v2) {
+  const size_t np = (size_t)Np * Np * Np;
+  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  const int c[3] = {(int)(q / ((size_t)Np * Np)), (int)((q / Np) % Np), (int)(q % Np)};
+  const float fb = (float)box;
+  for (int a = 0; a < 3; a++) {
+    const size_t idx = (size_t)a * np + q;
+    double x = c[a] * (box / (double)Np);
+    if (sd) x += (double)l0[idx];
+    else x += (double)l0[idx] * Di + (double)l1[idx] * Di2;
+    float f = (float)x;
+    for (int it = 0; it < 4 && f >= fb; ++it) f -= fb;
+    for (int it = 0; it < 4 && f < 0.f; ++it) f += fb;
+    if (!(f >= 0.f && f < fb)) f = 0.f;
+    pos[idx] = f;
+    if (use_cola) vel[idx] = 0.f;
+    else vel[idx] = sd ? l1[idx] : (float)((double)l0[idx] * v1 + (double)l1[idx] * v2);
+  }
+}
+
+// seed table on the host (a single serial stream), src/2LPT.c:259-271
+void seedtable_host(int N, unsigned long seed, std::vector<unsigned> &t) {
+  Ranlxd1 r;
+  r.seed(seed);
+  t.assign((size_t)N * N, 0u);
+  auto draw = [&]() { return (unsigned)(0x7fffffff * r.uniform()); };
+  for (int i = 0; i < N / 2; i++) {
+    for (int j = 0; j < i; j++) t[i * N + j] = draw();
+    for (int j = 0; j < i + 1; j++) t[j * N + i] = draw();
+    for (int j = 0; j < i; j++) t[(N - 1 - i) * N + j] = draw();
+    for (int j = 0; j < i + 1; j++) t[(N - 1 - j) * N + i] = draw();
+    for (int j = 0; j < i; j++) t[i * N + (N - 1 - j)] = draw();
+    for (int j = 0; j < i + 1; j++) t[j * N + (N - 1 - i)] = draw();
+    for (int j = 0; j < i; j++) t[(N - 1 - i) * N + (N - 1 - j)] = draw();
+    for (int j = 0; j < i + 1; j++) t[(N - 1 - j) * N + (N - 1 - i)] = draw();
+  }
+}
+
+int spectrum(mgpu_ctx *ctx, int mode, const float2 *cd, const float2 *cd2, float2 *out, int a, int b,
+             const double *g1 = nullptr, const double *g2 = nullptr) {
+  const double kf = 2 * MGPU_PI / ctx->p.box;
+  hipLaunchKernelGGL(k_lpt_spectrum, dim3(cdiv(ctx->slot_elems, kBlock)), dim3(kBlock), 0, ctx->stream, mode, cd, cd2,
+                     out, ctx->N, ctx->HP, kf, a, b, g1, g2);
+  MGPU_HIP(ctx, hipGetLastError());
+  return MGPU_OK;
+}
+
+// lattice read-out of a real mesh into dst ([np] floats) with mean removal
+int readout(mgpu_ctx *ctx, const float2 *mesh, double scale, float *dst) {
+  const size_t np = ctx->np;
+  const unsigned nb = cdiv(np, kBlock);
+  hipLaunchKernelGGL(k_lattice_read, dim3(nb), dim3(kBlock), 0, ctx->stream, (const float *)mesh, ctx->N, ctx->Np,
+                     2 * ctx->HP, scale, dst, ctx->partial);
+  MGPU_HIP(ctx, hipGetLastError());
+  hipLaunchKernelGGL(k_reduce1, dim3(1), dim3(1024), 0, ctx->stream, ctx->partial, (int)nb, ctx->dsum);
+  MGPU_HIP(ctx, hipGetLastError());
+  hipLaunchKernelGGL(k_sub_mean, dim3(nb), dim3(kBlock), 0, ctx->stream, dst, np, ctx->dsum, 1.0 / (double)np);
+  MGPU_HIP(ctx, hipGetLastError());
+  return MGPU_OK;
+}
+
+int upload_n2(mgpu_ctx *ctx, int which, const double *h) {
+  MGPU_HIP(ctx, hipMemcpyAsync(ctx->n2tab[which], h, ctx->n2len * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  return MGPU_OK;
+}
+
+}  // namespace
+
+int ic_displacement_fields(mgpu_ctx *ctx, unsigned long seed, int sphere_mode, int amp_fixed, int inverted,
+                           const double *pk_n2) {
+  PhaseTimer pt(ctx, &ctx->t.ic);
+  const int N = ctx->N;
+  int rc;
+  std::vector<unsigned> st;
+  seedtable_host(N, seed, st);
+  unsigned *dseed = nullptr;
+  MGPU_HIP(ctx, hipMalloc(&dseed, st.size() * sizeof(unsigned)));
+  MGPU_HIP(ctx, hipMemcpyAsync(dseed, st.data(), st.size() * sizeof(unsigned), hipMemcpyHostToDevice, ctx->stream));
+  if ((rc = upload_n2(ctx, 0, pk_n2))) return rc;
+  float2 *cd = ctx->p.scale_dependent ? ctx->cdelta[0] : ctx->slot[0];
+  MGPU_HIP(ctx, hipMemsetAsync(cd, 0, ctx->slot_elems * sizeof(float2), ctx->stream));
+  hipLaunchKernelGGL(k_ic_delta, dim3(cdiv((size_t)N * N, kBlock)), dim3(kBlock), 0, ctx->stream, dseed, N, ctx->Np,
+                     ctx->HP, ctx->p.box, sphere_mode, amp_fixed, inverted, ctx->n2tab[0], cd);
+  MGPU_HIP(ctx, hipGetLastError());
+  MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  MGPU_HIP(ctx, hipFree(dseed));
+
+  // displacement gradient -> 2LPT source (real) in slot2
+  const size_t nr = 2 * ctx->slot_elems;
+  const unsigned gr = cdiv(nr, kBlock);
+  static const int ga[3] = {0, 1, 2};
+  for (int g = 0; g < 3; g++) {
+    if ((rc = spectrum(ctx, SPEC_GRAD, cd, nullptr, ctx->slot[3 + g], ga[g], ga[g]))) return rc;
+    if ((rc = fft_c2r_3d(ctx, &ctx->fft, ctx->slot[3 + g]))) return rc;
+  }
+  hipLaunchKernelGGL(k_source_diag, dim3(gr), dim3(kBlock), 0, ctx->stream, (const float *)ctx->slot[3],
+                     (const float *)ctx->slot[4], (const float *)ctx->slot[5], (float *)ctx->slot[2], nr);
+  MGPU_HIP(ctx, hipGetLastError());
+  static const int oa[3] = {0, 0, 1}, ob[3] = {1, 2, 2};
+  for (int g = 0; g < 3; g++) {
+    if ((rc = spectrum(ctx, SPEC_GRAD, cd, nullptr, ctx->slot[3], oa[g], ob[g]))) return rc;
+    if ((rc = fft_c2r_3d(ctx, &ctx->fft, ctx->slot[3]))) return rc;
+    hipLaunchKernelGGL(k_source_off, dim3(gr), dim3(kBlock), 0, ctx->stream, (const float *)ctx->slot[3],
+                       (float *)ctx->slot[2], nr);
+    MGPU_HIP(ctx, hipGetLastError());
+  }
+  if ((rc = fft_r2c_3d(ctx, &ctx->fft, ctx->slot[2]))) return rc;
+
+  if (ctx->p.scale_dependent) {
+    if ((rc = spectrum(ctx, SPEC_NEG, ctx->slot[2], nullptr, ctx->cdelta[1], 0, 0))) return rc;
+    return MGPU_OK;
+  }
+  // ZA and 2LPT displacements at the lattice, mean removed
+  const double inv_n3 = 1.0 / ((double)N * N * N);
+  for (int a = 0; a < 3; a++) {
+    if ((rc = spectrum(ctx, SPEC_DISP, cd, nullptr, ctx->slot[3], a, 0))) return rc;
+    if ((rc = fft_c2r_3d(ctx, &ctx->fft, ctx->slot[3]))) return rc;
+    if ((rc = readout(ctx, ctx->slot[3], 1.0, ctx->lpt[0] + (size_t)a * ctx->np))) return rc;
+    if ((rc = spectrum(ctx, SPEC_DISP2, ctx->slot[2], nullptr, ctx->slot[4], a, 0))) return rc;
+    if ((rc = fft_c2r_3d(ctx, &ctx->fft, ctx->slot[4]))) return rc;
+    if ((rc = readout(ctx, ctx->slot[4], -3. / 7. * inv_n3, ctx->lpt[1] + (size_t)a * ctx->np))) return rc;
+  }
+  MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MGPU_OK;
+}
+
+int ic_init_particles(mgpu_ctx *ctx, double Di, double Di2) {
+  PhaseTimer pt(ctx, &ctx->t.ic);
+  if (ctx->np == 0) return MGPU_OK;
+  hipLaunchKernelGGL(k_init_particles, dim3(cdiv(ctx->np, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->pos, ctx->vel,
+                     ctx->lpt[0], ctx->lpt[1], ctx->Np, ctx->p.box, ctx->p.scale_dependent, Di, Di2,
+                     ctx->p.use_cola, 0.0, 0.0);
+  MGPU_HIP(ctx, hipGetLastError());
+  return MGPU_OK;
+}
+
+int ic_lpt_fields(mgpu_ctx *ctx, int slot, const double *g1_n2, const double *g2_n2) {
+  PhaseTimer pt(ctx, &ctx->t.lpt_fields);
+  if (!ctx->p.scale_dependent) return mgpu_fail(ctx, MGPU_E_STATE, "LPT fields need a scale-dependent context");
+  int rc;
+  if ((rc = upload_n2(ctx, 0, g1_n2))) return rc;
+  if ((rc = upload_n2(ctx, 1, g2_n2))) return rc;
+  for (int a = 0; a < 3; a++) {
+    if ((rc = spectrum(ctx, SPEC_FIELD, ctx->cdelta[0], ctx->cdelta[1], ctx->slot[3], a, 0, ctx->n2tab[0],
+                       ctx->n2tab[1])))
+      return rc;
+    if ((rc = fft_c2r_3d(ctx, &ctx->fft, ctx->slot[3]))) return rc;
+    if ((rc = readout(ctx, ctx->slot[3], 1.0, ctx->lpt[slot] + (size_t)a * ctx->np))) return rc;
+  }
+  MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MGPU_OK;
+}

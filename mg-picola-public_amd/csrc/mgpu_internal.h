@@ -1,0 +1,117 @@
+// Internal definitions shared by the translation units of libmgpu.so.
+//
+// Device mesh layout (one "slot" = N*N*HP complex64):
+//   complex half spectrum  c[(ix*N + iy)*HP + kz],  kz in [0, N/2], HP = N/2+1
+//                          rounded up to a multiple of 16 (128-byte rows);
+//                          entries kz in (N/2, HP) are zero padding.
+//   real mesh (in place)   r[(ix*N + iy)*2*HP + iz], iz in [0, N); the rest of
+//                          each row is padding.
+// This is FFTW's in-place r2c layout with a wider row so that every row starts
+// on a 128-byte boundary and column tiles of 16 complex values are aligned.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/mgpu.h"
+
+#define MGPU_PI 3.14159265358979323846
+#define MGPU_INVERSE_H0_MPCH 2997.92458
+
+struct FFTPlan {
+  int n = 0;       // transform size per dimension
+  int hp = 0;      // padded complex row length
+  int tcols = 16;  // columns per tile in the strided passes
+  float2 *tw = nullptr;  // W_N^m = exp(-2 pi i m / N), m < N
+};
+
+struct mgpu_ctx {
+  mgpu_params p;
+  int N = 0, Np = 0, HP = 0;
+  size_t np = 0;          // particles
+  size_t slot_elems = 0;  // complex elements per mesh slot
+  hipStream_t stream = nullptr;
+  int device = 0;
+  size_t lds_max = 65536;
+
+  // particles, SoA [3][np]
+  float *pos = nullptr, *vel = nullptr, *disp = nullptr;
+  float *lpt[2] = {nullptr, nullptr};
+
+  // mesh slots: 0 delta(k)   1 delta(x) copy / effective density   2 scratch spectrum
+  //             3..5 force components (3..4 double as the 64-bit deposit accumulator)
+  float2 *slot[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  unsigned long long *acc = nullptr;  // aliases slot[3]..slot[4]
+  float2 *cdelta[2] = {nullptr, nullptr};
+  bool have_phik = false;
+
+  // reductions
+  double *partial = nullptr;  // per-block partial sums
+  size_t partial_len = 0;
+  double *dsum = nullptr;     // final sums
+  double *pk = nullptr;       // 3 * nbins
+  double *n2tab[2] = {nullptr, nullptr};  // per-|n|^2 device tables
+  size_t n2len = 0;
+
+  FFTPlan fft;
+  FFTPlan fft_lattice;  // Nsample-sized plan for folded LPT fields (when N % Ns == 0)
+  float2 *lattice_slot = nullptr;
+
+  // state
+  bool have_deposit = false, have_density = false, have_forces = false, have_disp = false;
+
+  // timers
+  bool timers_on = false;
+  mgpu_timers t{};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  std::string err;
+};
+
+// ---------------------------------------------------------------- helpers
+int mgpu_fail(mgpu_ctx *ctx, int code, const std::string &msg);
+#define MGPU_HIP(ctx, call)                                                               \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return mgpu_fail((ctx), MGPU_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// phase timer: scoped accumulate into a field of ctx->t
+struct PhaseTimer {
+  mgpu_ctx *ctx;
+  double *field;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  PhaseTimer(mgpu_ctx *c, double *f);
+  ~PhaseTimer();
+};
+
+// ---------------------------------------------------------------- FFT (fft.hip)
+int fft_plan_init(mgpu_ctx *ctx, FFTPlan *plan, int n);
+void fft_plan_free(FFTPlan *plan);
+bool fft_size_supported(int n);
+// in-place 3D transforms on a slot with the padded layout above
+int fft_r2c_3d(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf);
+int fft_c2r_3d(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf);
+// individual passes (used by fused paths and tests)
+int fft_pass_z(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf, int forward);
+int fft_pass_col(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf, int axis, int forward);
+
+// ---------------------------------------------------------------- PM kernels (pm.hip)
+int pm_deposit(mgpu_ctx *ctx);                         // particles -> acc -> delta(x) in slot0 (+ copy slot1)
+int pm_pofk(mgpu_ctx *ctx, const float2 *dk, mgpu_pofk *out);
+int pm_fifth_force(mgpu_ctx *ctx, const mgpu_mg *mg);
+int pm_forces(mgpu_ctx *ctx);
+int pm_gather(mgpu_ctx *ctx, double sumD[3]);
+int pm_kick(mgpu_ctx *ctx, double dda, double A, double s1, double s2, const double sumD[3], double sumV[3]);
+int pm_drift(mgpu_ctx *ctx, double dyyy, double s1, double s2, const double sumV[3]);
+int pm_reduce3(mgpu_ctx *ctx, int nblocks, double out[3]);
+
+// ---------------------------------------------------------------- IC / LPT (ic.hip)
+int ic_displacement_fields(mgpu_ctx *ctx, unsigned long seed, int sphere_mode, int amp_fixed, int inverted,
+                           const double *pk_n2);
+int ic_init_particles(mgpu_ctx *ctx, double Di, double Di2);
+int ic_lpt_fields(mgpu_ctx *ctx, int slot, const double *g1_n2, const double *g2_n2);
+
+static inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }

@@ -1,0 +1,607 @@
+// Particle-mesh kernels of the COLA step for gfx950: CIC deposit, P(k)
+// binning, the modified-gravity spectral and real-space passes, the Green's
+// function, the CIC gather, Kick and Drift.  Each kernel cites the reference
+// routine (HAWinther/MG-PICOLA-PUBLIC) whose arithmetic it carries out.
+#include "mgpu_internal.h"
+
+#include <cmath>
+#include <vector>
+
+namespace {
+
+constexpr double kFix = 4294967296.0;        // 2^32: fixed-point scale of the deposit
+constexpr double kInvFix = 1.0 / 4294967296.0;
+constexpr int kBlock = 256;
+
+// Cell index and fractional offset of a scaled coordinate X = pos * N / Box,
+// as src/auxPM.c:298-322 forms them.  Positions outside [0, Box) (never
+// produced by Drift's periodic wrap) are wrapped so every index stays in range.
+__device__ __forceinline__ void cell_of(double X, int N, unsigned &I, double &D) {
+  if (!(X >= 0.0 && X < (double)N)) {
+    if (!isfinite(X)) X = 0.0;
+    X -= floor(X / N) * N;
+    if (!(X >= 0.0 && X < (double)N)) X = 0.0;
+  }
+  I = (unsigned)X;
+  D = X - (double)I;
+  if (I >= (unsigned)N) I = 0;
+}
+
+// ------------------------------------------------------------------ deposit
+// PtoMesh CIC scatter (src/auxPM.c:295-343): weights in FP64 from the float
+// positions, WPAR folded into the y weights, accumulated as unsigned 64-bit
+// fixed point (2^-32 resolution) so the sum is independent of the order the
+// atomics land in.  x periodicity replaces the ghost slab of src/auxPM.c:350-356.
+__global__ __launch_bounds__(kBlock) void k_deposit(const float *__restrict__ pos, size_t np, int N, int HP2,
+                                                    double scale, double wpar, unsigned long long *__restrict__ acc) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  unsigned IX, IY, IZ;
+  double DX, DY, DZ;
+  cell_of((double)pos[i] * scale, N, IX, DX);
+  cell_of((double)pos[np + i] * scale, N, IY, DY);
+  cell_of((double)pos[2 * np + i] * scale, N, IZ, DZ);
+  double TX = 1.0 - DX, TY = 1.0 - DY, TZ = 1.0 - DZ;
+  DY *= wpar;
+  TY *= wpar;
+  unsigned IXn = IX + 1 >= (unsigned)N ? 0 : IX + 1;
+  unsigned IYn = IY + 1 >= (unsigned)N ? 0 : IY + 1;
+  unsigned IZn = IZ + 1 >= (unsigned)N ? 0 : IZ + 1;
+  const size_t r00 = ((size_t)IX * N + IY) * N, r01 = ((size_t)IX * N + IYn) * N;
+  const size_t r10 = ((size_t)IXn * N + IY) * N, r11 = ((size_t)IXn * N + IYn) * N;
+#define DEP(idx, w) atomicAdd(acc + (idx), (unsigned long long)__double2ll_rn((w) * kFix))
+  DEP(r00 + IZ, TX * TY * TZ);
+  DEP(r00 + IZn, TX * TY * DZ);
+  DEP(r01 + IZ, TX * DY * TZ);
+  DEP(r01 + IZn, TX * DY * DZ);
+  DEP(r10 + IZ, DX * TY * TZ);
+  DEP(r10 + IZn, DX * TY * DZ);
+  DEP(r11 + IZ, DX * DY * TZ);
+  DEP(r11 + IZn, DX * DY * DZ);
+#undef DEP
+}
+
+// density = -1 + sum of weights (src/auxPM.c:292-293), written as FP32 into
+// the padded real layout of slot0 and, when the MG solver needs delta(x)
+// (CopyDensityArray, src/mg.h:381-386), also into slot1.
+__global__ __launch_bounds__(kBlock) void k_acc_to_density(const unsigned long long *__restrict__ acc, int N, int HP,
+                                                           float *__restrict__ d0, float *__restrict__ d1) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t HP2 = 2 * (size_t)HP;
+  size_t total = (size_t)N * N * HP2;
+  if (e >= total) return;
+  size_t row = e / HP2, iz = e % HP2;
+  float v = 0.f;
+  if (iz < (size_t)N) v = (float)((double)acc[row * N + iz] * kInvFix - 1.0);
+  d0[e] = v;
+  if (d1) d1[e] = v;
+}
+
+// ------------------------------------------------------------------ P(k)
+// compute_power_spectrum (src/compute_pofk.c:71-236): every mode of the half
+// spectrum with weight 1 (kz = 0, N/2) or 2, |n| binned linearly or
+// logarithmically, CIC deconvolution 1/(W_x W_y W_z)^4 and 1/N^6.
+__global__ __launch_bounds__(kBlock) void k_pofk(const float2 *__restrict__ dk, int N, int HP,
+                                                 const double *__restrict__ winv, int nbins, int bintype,
+                                                 double kmin, double kmax, double *__restrict__ out) {
+  extern __shared__ double sh[];  // 3*nbins
+  for (int b = threadIdx.x; b < 3 * nbins; b += blockDim.x) sh[b] = 0.0;
+  __syncthreads();
+  const int H = N / 2 + 1;
+  const size_t total = (size_t)N * N * H;
+  const double lnrange = bintype == 1 ? log(kmax / kmin) : 1.0;
+  for (size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x; m < total; m += (size_t)gridDim.x * blockDim.x) {
+    const int k = (int)(m % H);
+    const size_t rowi = m / H;
+    const int j = (int)(rowi % N), i = (int)(rowi / N);
+    const int d0 = i > N / 2 ? N - i : i, d1 = j > N / 2 ? N - j : j, d2 = k;
+    const double kmag = sqrt((double)(d0 * d0 + d1 * d1 + d2 * d2));
+    int nk;
+    if (bintype == 0) nk = (int)((kmag - kmin) / (kmax - kmin) * nbins + 0.5);
+    else nk = kmag <= 0.0 ? -1 : (int)(log(kmag / kmin) / lnrange * nbins + 0.5);
+    if (nk < 0 || nk >= nbins) continue;
+    const double w = (k == 0 || k == N / 2) ? 1.0 : 2.0;
+    const float2 v = dk[rowi * HP + k];
+    const double p = ((double)v.x * v.x + (double)v.y * v.y) * winv[d0] * winv[d1] * winv[d2];
+    atomicAdd(&sh[nk], w * p);
+    atomicAdd(&sh[nbins + nk], w * kmag);
+    atomicAdd(&sh[2 * nbins + nk], w);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < 3 * nbins; b += blockDim.x)
+    if (sh[b] != 0.0) atomicAdd(&out[b], sh[b]);
+}
+
+// ------------------------------------------------------------------ spectral helpers
+// |n|^2 of a half-spectrum entry (identical under the Forces, mg.h and 2LPT.c
+// index conventions)
+__device__ __forceinline__ int n2_of(int i, int j, int k, int N) {
+  const int a = i > N / 2 ? N - i : i, b = j > N / 2 ? N - j : j;
+  return a * a + b * b + k * k;
+}
+
+// DivideByLaplacian (src/mg.h:21-64): out = delta(k) * nf * (-1/n^2), out(0) = 0
+__global__ __launch_bounds__(kBlock) void k_divide_laplacian(const float2 *__restrict__ in, float2 *__restrict__ out,
+                                                             int N, int HP, double nf) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)N * N * HP) return;
+  const int k = (int)(e % HP);
+  const size_t row = e / HP;
+  const int j = (int)(row % N), i = (int)(row / N);
+  float2 r = make_float2(0.f, 0.f);
+  if (k <= N / 2) {
+    const int n2 = n2_of(i, j, k, N);
+    if (n2 > 0) {
+      const double f = nf * (-1.0 / (double)n2);
+      const float2 v = in[e];
+      r = make_float2((float)(v.x * f), (float)(v.y * f));
+    }
+  }
+  out[e] = r;
+}
+
+// SmoothDensityField (src/mg.h:268-309): out = delta(k) * W(|k| R) / N^3
+__global__ __launch_bounds__(kBlock) void k_smooth(const float2 *__restrict__ in, float2 *__restrict__ out, int N,
+                                                   int HP, int filter, double kfac, double nf) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)N * N * HP) return;
+  const int k = (int)(e % HP);
+  const size_t row = e / HP;
+  const int j = (int)(row % N), i = (int)(row / N);
+  float2 r = make_float2(0.f, 0.f);
+  if (k <= N / 2) {
+    const double kR = sqrt((double)n2_of(i, j, k, N)) * kfac;
+    double W;
+    switch (filter) {
+      case MGPU_FILTER_GAUSSIAN: W = exp(-0.5 * kR * kR); break;
+      case MGPU_FILTER_TOPHAT: W = kR < 1e-5 ? 1.0 : 3.0 / (kR * kR * kR) * (sin(kR) - kR * cos(kR)); break;
+      case MGPU_FILTER_SHARPK: W = kR < 1.0 ? 1.0 : 0.0; break;
+      default: W = 1.0;
+    }
+    const double f = W * nf;
+    const float2 v = in[e];
+    r = make_float2((float)(v.x * f), (float)(v.y * f));
+  }
+  out[e] = r;
+}
+
+// EffDensitykToPhiofk (src/mg.h:74-116): out = coupling * n^2/(n^2 + massterm2) * in, out(0) = 0
+__global__ __launch_bounds__(kBlock) void k_eff_to_phi(float2 *__restrict__ buf, int N, int HP, double coupling,
+                                                       double massterm2) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)N * N * HP) return;
+  const int k = (int)(e % HP);
+  const size_t row = e / HP;
+  const int j = (int)(row % N), i = (int)(row / N);
+  float2 r = make_float2(0.f, 0.f);
+  if (k <= N / 2) {
+    const int n2 = n2_of(i, j, k, N);
+    if (n2 > 0) {
+      const double RK = (double)n2;
+      const double f = coupling * (RK / (RK + massterm2));
+      const float2 v = buf[e];
+      r = make_float2((float)(v.x * f), (float)(v.y * f));
+    }
+  }
+  buf[e] = r;
+}
+
+// out = in * s over the whole slot (DGP without screening, src/mg.h:212-217;
+// JBD Geff(a), src/mg.h:127-137)
+__global__ __launch_bounds__(kBlock) void k_scale(const float2 *__restrict__ in, float2 *__restrict__ out, size_t n,
+                                                  double s) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const float2 v = in[e];
+  out[e] = make_float2((float)(v.x * s), (float)(v.y * s));
+}
+
+// real-space screening passes on padded real meshes:
+//   f(R)/m-beta (src/mg.h:169-171): dx *= screening_factor_potential(a, phi(x))
+//   DGP         (src/mg.h:235-244): dx *= coupling * screening_factor_density(a, delta_R(x))
+__global__ __launch_bounds__(kBlock) void k_screen_potential(float *__restrict__ dx, const float *__restrict__ phi,
+                                                             size_t n, int screening, double phi_crit) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const double p = phi[e];
+  double s = 1.0;
+  if (screening && p < 0.0) {
+    s = fabs(phi_crit / p);
+    if (s > 1.0) s = 1.0;
+  }
+  dx[e] = (float)(dx[e] * s);
+}
+
+__global__ __launch_bounds__(kBlock) void k_screen_density(float *__restrict__ dx, const float *__restrict__ dR,
+                                                           size_t n, int screening, double coupling,
+                                                           double dgp_fac) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double s = 1.0;
+  if (screening) {
+    const double fac = dgp_fac * (1.0 + (double)dR[e]);
+    s = fac < 1e-5 ? 1.0 : 2.0 * (sqrt(1.0 + fac) - 1.0) / fac;
+  }
+  dx[e] = (float)(dx[e] * (coupling * s));
+}
+
+// ------------------------------------------------------------------ Forces
+// src/auxPM.c:450-535: P = delta(k) (+ phi(k)); F_a = i n_a (-1/n^2) P / (N^3 2pi/Box),
+// n_x, n_y in (-N/2, N/2] (Nyquist +N/2), n_z in [0, N/2]; F(0) = 0.
+__global__ __launch_bounds__(kBlock) void k_forces(const float2 *__restrict__ dk, const float2 *__restrict__ phik,
+                                                   float2 *__restrict__ f0, float2 *__restrict__ f1,
+                                                   float2 *__restrict__ f2, int N, int HP, double cnorm) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)N * N * HP) return;
+  const int k = (int)(e % HP);
+  const size_t row = e / HP;
+  const int j = (int)(row % N), i = (int)(row / N);
+  float2 r0 = make_float2(0.f, 0.f), r1 = r0, r2 = r0;
+  if (k <= N / 2) {
+    const int dd0 = i > N / 2 ? i - N : i, dd1 = j > N / 2 ? j - N : j, dd2 = k;
+    const int rk = dd0 * dd0 + dd1 * dd1 + dd2 * dd2;
+    if (rk > 0) {
+      float2 P = dk[e];
+      if (phik) {
+        const float2 q = phik[e];
+        P.x += q.x;
+        P.y += q.y;
+      }
+      const double kk = -1.0 / (double)rk * cnorm;
+      // i * P * kk * dd
+      const double re = -(double)P.y * kk, im = (double)P.x * kk;
+      r0 = make_float2((float)(re * dd0), (float)(im * dd0));
+      r1 = make_float2((float)(re * dd1), (float)(im * dd1));
+      r2 = make_float2((float)(re * dd2), (float)(im * dd2));
+    }
+  }
+  f0[e] = r0;
+  f1[e] = r1;
+  f2[e] = r2;
+}
+
+// ------------------------------------------------------------------ gather
+// MtoParticles (src/auxPM.c:560-644): CIC interpolation of the three force
+// meshes (WPAR = 1), Disp stored as float; per-block sums of the stored values
+// for the mean, reduced later in a fixed order.
+__global__ __launch_bounds__(kBlock) void k_gather(const float *__restrict__ pos, size_t np, int N, int HP2,
+                                                   double scale, const float *__restrict__ g0,
+                                                   const float *__restrict__ g1, const float *__restrict__ g2,
+                                                   float *__restrict__ disp, double *__restrict__ partial) {
+  __shared__ double red[3][kBlock];
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double s[3] = {0, 0, 0};
+  if (i < np) {
+    unsigned IX, IY, IZ;
+    double DX, DY, DZ;
+    cell_of((double)pos[i] * scale, N, IX, DX);
+    cell_of((double)pos[np + i] * scale, N, IY, DY);
+    cell_of((double)pos[2 * np + i] * scale, N, IZ, DZ);
+    const double TX = 1.0 - DX, TY = 1.0 - DY, TZ = 1.0 - DZ;
+    const unsigned IXn = IX + 1 >= (unsigned)N ? 0 : IX + 1;
+    const unsigned IYn = IY + 1 >= (unsigned)N ? 0 : IY + 1;
+    const unsigned IZn = IZ + 1 >= (unsigned)N ? 0 : IZ + 1;
+    const size_t r00 = ((size_t)IX * N + IY) * HP2, r01 = ((size_t)IX * N + IYn) * HP2;
+    const size_t r10 = ((size_t)IXn * N + IY) * HP2, r11 = ((size_t)IXn * N + IYn) * HP2;
+    const double w000 = TX * TY * TZ, w001 = TX * TY * DZ, w010 = TX * DY * TZ, w011 = TX * DY * DZ;
+    const double w100 = DX * TY * TZ, w101 = DX * TY * DZ, w110 = DX * DY * TZ, w111 = DX * DY * DZ;
+    const float *g[3] = {g0, g1, g2};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const float *m = g[a];
+      const double d = m[r00 + IZ] * w000 + m[r00 + IZn] * w001 + m[r01 + IZ] * w010 + m[r01 + IZn] * w011 +
+                       m[r10 + IZ] * w100 + m[r10 + IZn] * w101 + m[r11 + IZ] * w110 + m[r11 + IZn] * w111;
+      const float df = (float)d;
+      disp[(size_t)a * np + i] = df;
+      s[a] = (double)df;
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; a++) red[a][threadIdx.x] = s[a];
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int a = 0; a < 3; a++) red[a][threadIdx.x] += red[a][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int a = 0; a < 3; a++) partial[(size_t)blockIdx.x * 3 + a] = red[a][0];
+}
+
+// fixed-order reduction of per-block partials [nb][3] -> out[3]
+__global__ __launch_bounds__(1024) void k_reduce3(const double *__restrict__ partial, int nb, double *__restrict__ out) {
+  __shared__ double red[3][1024];
+  double s[3] = {0, 0, 0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+    for (int a = 0; a < 3; a++) s[a] += partial[(size_t)b * 3 + a];
+  for (int a = 0; a < 3; a++) red[a][threadIdx.x] = s[a];
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int a = 0; a < 3; a++) red[a][threadIdx.x] += red[a][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int a = 0; a < 3; a++) out[a] = red[a][0];
+}
+
+// ------------------------------------------------------------------ Kick / Drift
+// COLA term (src/main.c:712, 728, 767, 781): D*s1 + D2*s2 in double, or the
+// float sum of the stored scale-dependent fields.
+__device__ __forceinline__ double lpt_term(const float *__restrict__ l0, const float *__restrict__ l1, double s1,
+                                           double s2, int sd, size_t idx) {
+#pragma clang fp contract(off)
+  if (sd == 1) return (double)l0[idx];
+  if (sd == 2) return (double)l1[idx];
+  return (double)l0[idx] * s1 + (double)l1[idx] * s2;
+}
+
+// Kick (src/main.c:688-742)
+__global__ __launch_bounds__(kBlock) void k_kick(float *__restrict__ vel, float *__restrict__ disp,
+                                                 const float *__restrict__ l0, const float *__restrict__ l1,
+                                                 size_t np, double s1, double s2, int sd, double omega, double dda,
+                                                 double A, int use_cola, double sD0, double sD1, double sD2,
+                                                 double *__restrict__ partial) {
+#pragma clang fp contract(off)  // same rounding sequence as the CPU reference
+  __shared__ double red[3][kBlock];
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double s[3] = {0, 0, 0};
+  if (i < np) {
+    const double sD[3] = {sD0, sD1, sD2};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const size_t idx = (size_t)a * np + i;
+      const float dv = (float)((double)disp[idx] - sD[a]);
+      disp[idx] = dv;
+      const double lp = use_cola ? lpt_term(l0, l1, s1, s2, sd, idx) : 0.0;
+      const double force = -1.5 * omega * (double)dv - use_cola * lp / A;
+      const float v = (float)((double)vel[idx] + force * dda);
+      vel[idx] = v;
+      s[a] = (double)v;
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; a++) red[a][threadIdx.x] = s[a];
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int a = 0; a < 3; a++) red[a][threadIdx.x] += red[a][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int a = 0; a < 3; a++) partial[(size_t)blockIdx.x * 3 + a] = red[a][0];
+}
+
+// periodic_wrap on float (src/auxPM.c:649-655) with a bounded loop
+__device__ __forceinline__ float wrapf(float x, float box) {
+  for (int it = 0; it < 4 && x >= box; ++it) x -= box;
+  for (int it = 0; it < 4 && x < 0.f; ++it) x += box;
+  if (!(x >= 0.f && x < box)) {
+    x = isfinite(x) ? fmodf(x, box) : 0.f;
+    if (x < 0.f) x += box;
+    if (!(x >= 0.f && x < box)) x = 0.f;
+  }
+  if (x == box) x = 0.f;
+  return x;
+}
+
+// Drift (src/main.c:747-787)
+__global__ __launch_bounds__(kBlock) void k_drift(float *__restrict__ pos, const float *__restrict__ vel,
+                                                  const float *__restrict__ l0, const float *__restrict__ l1,
+                                                  size_t np, double s1, double s2, int sd, float box, double dyyy,
+                                                  int use_cola, double sV0, double sV1, double sV2) {
+#pragma clang fp contract(off)  // same rounding sequence as the CPU reference
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const double sV[3] = {sV0, sV1, sV2};
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const size_t idx = (size_t)a * np + i;
+    float p = (float)((double)pos[idx] + ((double)vel[idx] - sV[a]) * dyyy);
+    const double lp = use_cola ? lpt_term(l0, l1, s1, s2, sd, idx) : 0.0;
+    pos[idx] = wrapf((float)((double)p + use_cola * lp), box);
+  }
+}
+
+}  // namespace
+
+// =================================================================== host side
+
+int pm_reduce3(mgpu_ctx *ctx, int nblocks, double out[3]) {
+  hipLaunchKernelGGL(k_reduce3, dim3(1), dim3(1024), 0, ctx->stream, ctx->partial, nblocks, ctx->dsum);
+  MGPU_HIP(ctx, hipGetLastError());
+  MGPU_HIP(ctx, hipMemcpyAsync(out, ctx->dsum, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MGPU_OK;
+}
+
+int pm_deposit(mgpu_ctx *ctx) {
+  PhaseTimer pt(ctx, &ctx->t.ptomesh);
+  const int N = ctx->N;
+  const size_t cells = (size_t)N * N * N;
+  MGPU_HIP(ctx, hipMemsetAsync(ctx->acc, 0, cells * sizeof(unsigned long long), ctx->stream));
+  const double scale = (double)N / ctx->p.box;
+  const double wpar = std::pow((double)N / (double)ctx->Np, 3);
+  if (ctx->np) {
+    hipLaunchKernelGGL(k_deposit, dim3(cdiv(ctx->np, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->pos, ctx->np, N,
+                       2 * ctx->HP, scale, wpar, ctx->acc);
+    MGPU_HIP(ctx, hipGetLastError());
+  }
+  const bool need_dx = ctx->p.mg_active && ctx->p.model != MGPU_BRANSDICKE && ctx->p.model != MGPU_LCDM;
+  const size_t total = (size_t)N * N * 2 * ctx->HP;
+  hipLaunchKernelGGL(k_acc_to_density, dim3(cdiv(total, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->acc, N, ctx->HP,
+                     (float *)ctx->slot[0], need_dx ? (float *)ctx->slot[1] : nullptr);
+  MGPU_HIP(ctx, hipGetLastError());
+  return MGPU_OK;
+}
+
+int pm_pofk(mgpu_ctx *ctx, const float2 *dk, mgpu_pofk *out) {
+  PhaseTimer pt(ctx, &ctx->t.pofk);
+  const int N = ctx->N, nbins = ctx->p.pofk_nbins;
+  if (nbins <= 0) return mgpu_fail(ctx, MGPU_E_INVALID, "pofk_nbins must be positive");
+  // per-axis deconvolution 1/sinc(pi d/N)^4 and the 1/N^6 of the FFT normalisation
+  std::vector<double> winv(N / 2 + 1);
+  const double n6 = 1.0 / std::pow((double)N, 6);
+  for (int d = 0; d <= N / 2; d++) {
+    double g = d == 0 ? 1.0 : std::sin((MGPU_PI * d) / (double)N) / ((MGPU_PI * d) / (double)N);
+    winv[d] = 1.0 / std::pow(g, 4.0) * std::cbrt(n6);
+  }
+  double *dwin = ctx->partial;  // scratch: winv in the partial buffer tail
+  if (ctx->partial_len < (size_t)(N / 2 + 1)) return mgpu_fail(ctx, MGPU_E_INVALID, "scratch too small");
+  MGPU_HIP(ctx, hipMemcpyAsync(dwin, winv.data(), winv.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  MGPU_HIP(ctx, hipMemsetAsync(ctx->pk, 0, 3 * nbins * sizeof(double), ctx->stream));
+  const size_t shmem = 3 * (size_t)nbins * sizeof(double);
+  if (shmem > 65536) {
+    MGPU_HIP(ctx, hipFuncSetAttribute((const void *)k_pofk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+  }
+  const size_t modes = (size_t)N * N * (N / 2 + 1);
+  unsigned grid = std::min<size_t>(cdiv(modes, kBlock), 4096);
+  hipLaunchKernelGGL(k_pofk, dim3(grid), dim3(kBlock), shmem, ctx->stream, dk, N, ctx->HP, dwin, nbins,
+                     ctx->p.pofk_bintype, ctx->p.pofk_kmin, ctx->p.pofk_kmax, ctx->pk);
+  MGPU_HIP(ctx, hipGetLastError());
+  std::vector<double> h(3 * nbins);
+  MGPU_HIP(ctx, hipMemcpyAsync(h.data(), ctx->pk, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const double box3 = std::pow(ctx->p.box, 3);
+  const double shot = std::pow(ctx->p.box / (double)ctx->Np, 3);
+  for (int b = 0; b < nbins; b++) {
+    double p = h[b], k = h[nbins + b], n = h[2 * nbins + b];
+    if (n > 0) {
+      p = (p / n) * box3;
+      if (ctx->p.pofk_subtract_shotnoise) p -= shot;
+      k = (k / n) * 2.0 * M_PI / ctx->p.box;
+    }
+    if (out->pofk) out->pofk[b] = p;
+    if (out->kmean) out->kmean[b] = k;
+    if (out->nmodes) out->nmodes[b] = n;
+  }
+  return MGPU_OK;
+}
+
+int pm_fifth_force(mgpu_ctx *ctx, const mgpu_mg *mg) {
+  PhaseTimer pt(ctx, &ctx->t.modified_gravity);
+  ctx->have_phik = false;
+  if (!ctx->p.mg_active || ctx->p.model == MGPU_LCDM) return MGPU_OK;
+  if (!mg) return mgpu_fail(ctx, MGPU_E_INVALID, "modified gravity active but no mgpu_mg coefficients given");
+  const int N = ctx->N, HP = ctx->HP;
+  const size_t nc = ctx->slot_elems, nr = 2 * nc;
+  const unsigned gc = cdiv(nc, kBlock), gr = cdiv(nr, kBlock);
+  float2 *dk = ctx->slot[0], *dx = ctx->slot[1], *one = ctx->slot[2];
+  int rc;
+  switch (ctx->p.model) {
+    case MGPU_BRANSDICKE:
+      hipLaunchKernelGGL(k_scale, dim3(gc), dim3(kBlock), 0, ctx->stream, dk, dk, nc, mg->geff);
+      MGPU_HIP(ctx, hipGetLastError());
+      return MGPU_OK;
+    case MGPU_FOFR:
+    case MGPU_MBETA: {
+      const double massterm2 = mg->a * mg->a * mg->mass2 / std::pow(2.0 * MGPU_PI * MGPU_INVERSE_H0_MPCH / ctx->p.box, 2);
+      if (ctx->p.include_screening) {
+        double nf = 1.0 / std::pow((double)N, 3);
+        nf *= 1.5 * ctx->p.omega / mg->a * std::pow(ctx->p.box / MGPU_INVERSE_H0_MPCH / (2.0 * MGPU_PI), 2);
+        hipLaunchKernelGGL(k_divide_laplacian, dim3(gc), dim3(kBlock), 0, ctx->stream, dk, one, N, HP, nf);
+        MGPU_HIP(ctx, hipGetLastError());
+        if ((rc = fft_c2r_3d(ctx, &ctx->fft, one))) return rc;
+        hipLaunchKernelGGL(k_screen_potential, dim3(gr), dim3(kBlock), 0, ctx->stream, (float *)dx,
+                           (const float *)one, nr, 1, mg->phi_crit);
+        MGPU_HIP(ctx, hipGetLastError());
+        if ((rc = fft_r2c_3d(ctx, &ctx->fft, dx))) return rc;
+      } else {
+        // the reference re-transforms its delta(x) copy (src/mg.h:153-156), which reproduces delta(k)
+        MGPU_HIP(ctx, hipMemcpyAsync(dx, dk, nc * sizeof(float2), hipMemcpyDeviceToDevice, ctx->stream));
+      }
+      hipLaunchKernelGGL(k_eff_to_phi, dim3(gc), dim3(kBlock), 0, ctx->stream, dx, N, HP, mg->coupling, massterm2);
+      MGPU_HIP(ctx, hipGetLastError());
+      ctx->have_phik = true;
+      return MGPU_OK;
+    }
+    case MGPU_DGP: {
+      if (!ctx->p.include_screening) {
+        hipLaunchKernelGGL(k_scale, dim3(gc), dim3(kBlock), 0, ctx->stream, dk, dx, nc, mg->coupling);
+        MGPU_HIP(ctx, hipGetLastError());
+      } else {
+        const double nf = 1.0 / std::pow((double)N, 3);
+        hipLaunchKernelGGL(k_smooth, dim3(gc), dim3(kBlock), 0, ctx->stream, dk, one, N, HP, ctx->p.filter,
+                           2.0 * MGPU_PI / ctx->p.box * mg->rsmooth, nf);
+        MGPU_HIP(ctx, hipGetLastError());
+        if ((rc = fft_c2r_3d(ctx, &ctx->fft, one))) return rc;
+        hipLaunchKernelGGL(k_screen_density, dim3(gr), dim3(kBlock), 0, ctx->stream, (float *)dx, (const float *)one,
+                           nr, 1, mg->coupling, mg->dgp_fac);
+        MGPU_HIP(ctx, hipGetLastError());
+        if ((rc = fft_r2c_3d(ctx, &ctx->fft, dx))) return rc;
+      }
+      ctx->have_phik = true;
+      return MGPU_OK;
+    }
+    default:
+      return mgpu_fail(ctx, MGPU_E_INVALID, "unknown model");
+  }
+}
+
+int pm_forces(mgpu_ctx *ctx) {
+  PhaseTimer pt(ctx, &ctx->t.forces);
+  const int N = ctx->N;
+  const double scale = 2. * M_PI / ctx->p.box;
+  const double cnorm = 1.0 / (std::pow((double)N, 3) * scale);
+  hipLaunchKernelGGL(k_forces, dim3(cdiv(ctx->slot_elems, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->slot[0],
+                     ctx->have_phik ? ctx->slot[1] : nullptr, ctx->slot[3], ctx->slot[4], ctx->slot[5], N, ctx->HP,
+                     cnorm);
+  MGPU_HIP(ctx, hipGetLastError());
+  int rc;
+  for (int a = 0; a < 3; a++)
+    if ((rc = fft_c2r_3d(ctx, &ctx->fft, ctx->slot[3 + a]))) return rc;
+  return MGPU_OK;
+}
+
+int pm_gather(mgpu_ctx *ctx, double sumD[3]) {
+  PhaseTimer pt(ctx, &ctx->t.mtoparticles);
+  if (ctx->np == 0) {
+    sumD[0] = sumD[1] = sumD[2] = 0;
+    return MGPU_OK;
+  }
+  const int N = ctx->N;
+  const unsigned nb = cdiv(ctx->np, kBlock);
+  hipLaunchKernelGGL(k_gather, dim3(nb), dim3(kBlock), 0, ctx->stream, ctx->pos, ctx->np, N, 2 * ctx->HP,
+                     (double)N / ctx->p.box, (const float *)ctx->slot[3], (const float *)ctx->slot[4],
+                     (const float *)ctx->slot[5], ctx->disp, ctx->partial);
+  MGPU_HIP(ctx, hipGetLastError());
+  double s[3];
+  int rc = pm_reduce3(ctx, (int)nb, s);
+  if (rc) return rc;
+  for (int a = 0; a < 3; a++) sumD[a] = s[a] / (double)ctx->np;
+  return MGPU_OK;
+}
+
+int pm_kick(mgpu_ctx *ctx, double dda, double A, double s1, double s2, const double sumD[3], double sumV[3]) {
+  PhaseTimer pt(ctx, &ctx->t.kick);
+  if (ctx->np == 0) {
+    sumV[0] = sumV[1] = sumV[2] = 0;
+    return MGPU_OK;
+  }
+  const unsigned nb = cdiv(ctx->np, kBlock);
+  const int sd = ctx->p.scale_dependent ? 1 : 0;
+  hipLaunchKernelGGL(k_kick, dim3(nb), dim3(kBlock), 0, ctx->stream, ctx->vel, ctx->disp, ctx->lpt[0], ctx->lpt[1],
+                     ctx->np, s1, s2, sd, ctx->p.omega, dda, A, ctx->p.use_cola, sumD[0], sumD[1], sumD[2],
+                     ctx->partial);
+  MGPU_HIP(ctx, hipGetLastError());
+  double s[3];
+  int rc = pm_reduce3(ctx, (int)nb, s);
+  if (rc) return rc;
+  for (int a = 0; a < 3; a++) sumV[a] = s[a] / (double)ctx->np;
+  return MGPU_OK;
+}
+
+int pm_drift(mgpu_ctx *ctx, double dyyy, double s1, double s2, const double sumV[3]) {
+  PhaseTimer pt(ctx, &ctx->t.drift);
+  if (ctx->np == 0) return MGPU_OK;
+  const int sd = ctx->p.scale_dependent ? 2 : 0;
+  hipLaunchKernelGGL(k_drift, dim3(cdiv(ctx->np, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->pos, ctx->vel,
+                     ctx->lpt[0], ctx->lpt[1], ctx->np, s1, s2, sd, (float)ctx->p.box, dyyy, ctx->p.use_cola,
+                     sumV[0], sumV[1], sumV[2]);
+  MGPU_HIP(ctx, hipGetLastError());
+  return MGPU_OK;
+}

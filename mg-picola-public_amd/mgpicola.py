@@ -1,0 +1,285 @@
+"""Python binding of libmgpu (include/mgpu.h) and libmgpicola (host/mgpicola.h).
+
+Thin ctypes layer used by the tests, the benchmark and the smoke check.  The
+compute path is the HIP library; there is no Python or CPU fallback: if
+libmgpu.so is missing, or no GPU is visible, the calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBMGPU = os.path.join(HERE, "libmgpu.so")
+LIBHOST = os.path.join(HERE, "libmgpicola.so")
+
+LCDM, FOFR, DGP, BRANSDICKE, MBETA = 0, 1, 2, 3, 4
+FILTER_NONE, FILTER_GAUSSIAN, FILTER_TOPHAT, FILTER_SHARPK = 0, 1, 2, 3
+MESH_DENSITY_X, MESH_DENSITY_K, MESH_PHI_K, MESH_FORCE_X, MESH_FORCE_Y, MESH_FORCE_Z, MESH_CDELTA1, MESH_CDELTA2 = range(8)
+
+# every symbol include/mgpu.h declares
+MGPU_SYMBOLS = [
+    "mgpu_n2_table_len", "mgpu_last_error", "mgpu_device_info", "mgpu_create", "mgpu_destroy", "mgpu_sync",
+    "mgpu_set_particles", "mgpu_get_particles", "mgpu_set_lpt", "mgpu_get_lpt", "mgpu_get_disp",
+    "mgpu_pm_density", "mgpu_pm_deposit", "mgpu_pm_transform", "mgpu_power_spectrum", "mgpu_fifth_force", "mgpu_forces", "mgpu_mesh_to_particles",
+    "mgpu_get_displacements", "mgpu_kick", "mgpu_drift", "mgpu_ic_displacement_fields", "mgpu_init_particles",
+    "mgpu_lpt_fields", "mgpu_get_mesh", "mgpu_set_mesh", "mgpu_get_timers", "mgpu_enable_timers",
+]
+
+
+class Params(C.Structure):
+    _fields_ = [("nmesh", C.c_int), ("nsample", C.c_int), ("box", C.c_double), ("omega", C.c_double),
+                ("model", C.c_int), ("mg_active", C.c_int), ("include_screening", C.c_int), ("filter", C.c_int),
+                ("scale_dependent", C.c_int), ("use_cola", C.c_int), ("pofk_nbins", C.c_int),
+                ("pofk_bintype", C.c_int), ("pofk_subtract_shotnoise", C.c_int), ("pofk_kmin", C.c_double),
+                ("pofk_kmax", C.c_double)]
+
+
+class MG(C.Structure):
+    _fields_ = [("a", C.c_double), ("coupling", C.c_double), ("mass2", C.c_double), ("phi_crit", C.c_double),
+                ("dgp_fac", C.c_double), ("rsmooth", C.c_double), ("geff", C.c_double)]
+
+
+class Pofk(C.Structure):
+    _fields_ = [("pofk", C.POINTER(C.c_double)), ("kmean", C.POINTER(C.c_double)),
+                ("nmodes", C.POINTER(C.c_double))]
+
+
+class Timers(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("ptomesh", "fft", "pofk", "modified_gravity", "forces", "mtoparticles",
+                                          "kick", "drift", "lpt_fields", "ic")]
+
+
+class MGPUError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libmgpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBMGPU):
+            raise MGPUError(f"{LIBMGPU} not built: run __graft_entry__.build() or make -C mg-picola-public_amd")
+        L = C.CDLL(LIBMGPU)
+        vp = C.c_void_p
+        L.mgpu_n2_table_len.restype = C.c_size_t
+        L.mgpu_n2_table_len.argtypes = [C.c_int]
+        L.mgpu_last_error.restype = C.c_char_p
+        L.mgpu_last_error.argtypes = [vp]
+        L.mgpu_device_info.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_size_t)]
+        L.mgpu_create.argtypes = [C.POINTER(Params), C.POINTER(vp)]
+        L.mgpu_destroy.argtypes = [vp]
+        L.mgpu_destroy.restype = None
+        for n in ("mgpu_sync", "mgpu_forces"):
+            getattr(L, n).argtypes = [vp]
+        L.mgpu_set_particles.argtypes = [vp, vp, vp]
+        L.mgpu_get_particles.argtypes = [vp, vp, vp]
+        L.mgpu_set_lpt.argtypes = [vp, C.c_int, vp]
+        L.mgpu_get_lpt.argtypes = [vp, C.c_int, vp]
+        L.mgpu_get_disp.argtypes = [vp, vp]
+        L.mgpu_pm_density.argtypes = [vp, C.POINTER(Pofk)]
+        L.mgpu_pm_deposit.argtypes = [vp]
+        L.mgpu_pm_transform.argtypes = [vp, C.POINTER(Pofk)]
+        L.mgpu_power_spectrum.argtypes = [vp, C.POINTER(Pofk)]
+        L.mgpu_fifth_force.argtypes = [vp, C.POINTER(MG)]
+        L.mgpu_mesh_to_particles.argtypes = [vp, C.POINTER(C.c_double)]
+        L.mgpu_get_displacements.argtypes = [vp, C.POINTER(MG), C.POINTER(Pofk), C.POINTER(C.c_double)]
+        L.mgpu_kick.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double),
+                                C.POINTER(C.c_double)]
+        L.mgpu_drift.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
+        L.mgpu_ic_displacement_fields.argtypes = [vp, C.c_ulong, C.c_int, C.c_int, C.c_int, vp]
+        L.mgpu_init_particles.argtypes = [vp, C.c_double, C.c_double]
+        L.mgpu_lpt_fields.argtypes = [vp, C.c_int, vp, vp]
+        L.mgpu_get_mesh.argtypes = [vp, C.c_int, vp]
+        L.mgpu_set_mesh.argtypes = [vp, C.c_int, vp]
+        L.mgpu_get_timers.argtypes = [vp, C.POINTER(Timers), C.c_int]
+        L.mgpu_enable_timers.argtypes = [vp, C.c_int]
+        _lib = L
+    return _lib
+
+
+def n2_table_len(n):
+    return 3 * (n // 2) ** 2 + 1
+
+
+def device_info():
+    name = C.create_string_buffer(256)
+    mem, cus, lds = C.c_size_t(), C.c_int(), C.c_size_t()
+    rc = lib().mgpu_device_info(name, 256, C.byref(mem), C.byref(cus), C.byref(lds))
+    if rc:
+        raise MGPUError(lib().mgpu_last_error(None).decode())
+    return {"name": name.value.decode(), "global_mem": mem.value, "cu_count": cus.value, "lds_per_block": lds.value}
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def _f32(a, shape):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    assert a.size == int(np.prod(shape)), (a.shape, shape)
+    return a
+
+
+class MGPU:
+    """One simulation context on the GPU (owns particles, meshes, FFT tables)."""
+
+    def __init__(self, nmesh, nsample, box, omega, model=LCDM, mg_active=0, include_screening=0,
+                 filter=FILTER_NONE, scale_dependent=0, use_cola=1, pofk_nbins=0, pofk_bintype=0,
+                 pofk_subtract_shotnoise=1, pofk_kmin=0.0, pofk_kmax=0.0):
+        if pofk_nbins <= 0:
+            pofk_nbins = nmesh
+        if pofk_kmax <= pofk_kmin:
+            pofk_kmin, pofk_kmax = (0.0 if pofk_bintype == 0 else 1.0), float(nmesh)
+        self.p = Params(nmesh, nsample, box, omega, model, mg_active, include_screening, filter, scale_dependent,
+                        use_cola, pofk_nbins, pofk_bintype, pofk_subtract_shotnoise, pofk_kmin, pofk_kmax)
+        self.n, self.ns, self.np = nmesh, nsample, nsample ** 3
+        self.ctx = C.c_void_p()
+        rc = lib().mgpu_create(C.byref(self.p), C.byref(self.ctx))
+        if rc:
+            raise MGPUError(f"mgpu_create failed ({rc}): {lib().mgpu_last_error(None).decode()}")
+
+    def close(self):
+        if self.ctx:
+            lib().mgpu_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc, what):
+        if rc:
+            raise MGPUError(f"{what} failed ({rc}): {lib().mgpu_last_error(self.ctx).decode()}")
+
+    # particles
+    def set_particles(self, pos, vel=None):
+        pos = _f32(pos, (3, self.np))
+        vel = None if vel is None else _f32(vel, (3, self.np))
+        self._ck(lib().mgpu_set_particles(self.ctx, _ptr(pos), _ptr(vel)), "set_particles")
+
+    def get_particles(self):
+        pos = np.empty((3, self.np), np.float32)
+        vel = np.empty((3, self.np), np.float32)
+        self._ck(lib().mgpu_get_particles(self.ctx, _ptr(pos), _ptr(vel)), "get_particles")
+        return pos, vel
+
+    def set_lpt(self, slot, d):
+        d = _f32(d, (3, self.np))
+        self._ck(lib().mgpu_set_lpt(self.ctx, slot, _ptr(d)), "set_lpt")
+
+    def get_lpt(self, slot):
+        d = np.empty((3, self.np), np.float32)
+        self._ck(lib().mgpu_get_lpt(self.ctx, slot, _ptr(d)), "get_lpt")
+        return d
+
+    def get_disp(self):
+        d = np.empty((3, self.np), np.float32)
+        self._ck(lib().mgpu_get_disp(self.ctx, _ptr(d)), "get_disp")
+        return d
+
+    def _pofk_struct(self):
+        nb = self.p.pofk_nbins
+        bufs = [np.zeros(nb), np.zeros(nb), np.zeros(nb)]
+        s = Pofk(*[b.ctypes.data_as(C.POINTER(C.c_double)) for b in bufs])
+        return s, bufs
+
+    # PM step
+    def pm_density(self, pofk=False):
+        if pofk:
+            s, bufs = self._pofk_struct()
+            self._ck(lib().mgpu_pm_density(self.ctx, C.byref(s)), "pm_density")
+            return tuple(bufs)
+        self._ck(lib().mgpu_pm_density(self.ctx, None), "pm_density")
+        return None
+
+    def pm_deposit(self):
+        self._ck(lib().mgpu_pm_deposit(self.ctx), "pm_deposit")
+
+    def pm_transform(self, pofk=False):
+        if pofk:
+            s, bufs = self._pofk_struct()
+            self._ck(lib().mgpu_pm_transform(self.ctx, C.byref(s)), "pm_transform")
+            return tuple(bufs)
+        self._ck(lib().mgpu_pm_transform(self.ctx, None), "pm_transform")
+        return None
+
+    def power_spectrum(self):
+        s, bufs = self._pofk_struct()
+        self._ck(lib().mgpu_power_spectrum(self.ctx, C.byref(s)), "power_spectrum")
+        return tuple(bufs)
+
+    def fifth_force(self, mg):
+        self._ck(lib().mgpu_fifth_force(self.ctx, C.byref(mg)), "fifth_force")
+
+    def forces(self):
+        self._ck(lib().mgpu_forces(self.ctx), "forces")
+
+    def mesh_to_particles(self):
+        s = (C.c_double * 3)()
+        self._ck(lib().mgpu_mesh_to_particles(self.ctx, s), "mesh_to_particles")
+        return np.array(list(s))
+
+    def get_displacements(self, mg=None, pofk=False):
+        s = (C.c_double * 3)()
+        ps, bufs = self._pofk_struct() if pofk else (None, None)
+        self._ck(lib().mgpu_get_displacements(self.ctx, None if mg is None else C.byref(mg),
+                                              None if ps is None else C.byref(ps), s), "get_displacements")
+        return np.array(list(s)), (tuple(bufs) if pofk else None)
+
+    def kick(self, dda, A, s1, s2, sumD):
+        sV = (C.c_double * 3)()
+        self._ck(lib().mgpu_kick(self.ctx, dda, A, s1, s2, (C.c_double * 3)(*sumD), sV), "kick")
+        return np.array(list(sV))
+
+    def drift(self, dyyy, s1, s2, sumV):
+        self._ck(lib().mgpu_drift(self.ctx, dyyy, s1, s2, (C.c_double * 3)(*sumV)), "drift")
+
+    # initial conditions
+    def ic_displacement_fields(self, seed, pk_n2, sphere_mode=0, amplitude_fixed=0, inverted=0):
+        pk = np.ascontiguousarray(pk_n2, np.float64)
+        assert pk.size == n2_table_len(self.n)
+        self._ck(lib().mgpu_ic_displacement_fields(self.ctx, seed, sphere_mode, amplitude_fixed, inverted,
+                                                   _ptr(pk)), "ic_displacement_fields")
+
+    def init_particles(self, Di, Di2):
+        self._ck(lib().mgpu_init_particles(self.ctx, Di, Di2), "init_particles")
+
+    def lpt_fields(self, slot, g1_n2, g2_n2):
+        g1 = np.ascontiguousarray(g1_n2, np.float64)
+        g2 = np.ascontiguousarray(g2_n2, np.float64)
+        self._ck(lib().mgpu_lpt_fields(self.ctx, slot, _ptr(g1), _ptr(g2)), "lpt_fields")
+
+    # inspection
+    def get_mesh(self, which):
+        n = self.n
+        real = which in (MESH_DENSITY_X, MESH_FORCE_X, MESH_FORCE_Y, MESH_FORCE_Z)
+        out = np.empty(n * n * n if real else 2 * n * n * (n // 2 + 1), np.float64)
+        self._ck(lib().mgpu_get_mesh(self.ctx, which, _ptr(out)), "get_mesh")
+        if real:
+            return out.reshape(n, n, n)
+        return out.view(np.complex128).reshape(n, n, n // 2 + 1)
+
+    def set_mesh(self, which, arr):
+        a = np.ascontiguousarray(arr)
+        if np.iscomplexobj(a):
+            a = a.astype(np.complex128).view(np.float64)
+        a = np.ascontiguousarray(a, np.float64).ravel()
+        self._ck(lib().mgpu_set_mesh(self.ctx, which, _ptr(a)), "set_mesh")
+
+    def enable_timers(self, on=True):
+        self._ck(lib().mgpu_enable_timers(self.ctx, 1 if on else 0), "enable_timers")
+
+    def timers(self, reset=False):
+        t = Timers()
+        self._ck(lib().mgpu_get_timers(self.ctx, C.byref(t), 1 if reset else 0), "get_timers")
+        return {n: getattr(t, n) for n, _ in Timers._fields_}
+
+    def sync(self):
+        self._ck(lib().mgpu_sync(self.ctx), "sync")
