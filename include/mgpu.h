@@ -173,6 +173,19 @@ typedef struct {
 int mgpu_get_timers(mgpu_ctx *ctx, mgpu_timers *t, int reset);
 int mgpu_enable_timers(mgpu_ctx *ctx, int on);
 
+/* ---- per-kernel HIP-event timing (no counterpart in the reference) ----
+ * When enabled, every launch of the listed kernel classes is bracketed by HIP
+ * events on the context's stream; mgpu_kernel_timing synchronises and reports
+ * launches, summed device time and summed algorithmic bytes (compulsory HBM
+ * reads + writes) since the last reset. */
+enum {
+  MGPU_K_DEPOSIT = 0, MGPU_K_DENSITY, MGPU_K_FFT_Z, MGPU_K_FFT_Y, MGPU_K_FFT_X, MGPU_K_POFK, MGPU_K_SPECTRAL,
+  MGPU_K_REAL, MGPU_K_GATHER, MGPU_K_KICK, MGPU_K_DRIFT, MGPU_K_LPT, MGPU_K_READOUT, MGPU_K_COUNT
+};
+const char *mgpu_kernel_name(int kernel);
+int mgpu_enable_kernel_timing(mgpu_ctx *ctx, int on);
+int mgpu_kernel_timing(mgpu_ctx *ctx, int kernel, int *launches, double *ms, double *bytes, int reset);
+
 #ifdef __cplusplus
 }
 #endif

@@ -50,7 +50,65 @@ PhaseTimer::~PhaseTimer() {
   (void)hipEventDestroy(e1);
 }
 
+KScope::KScope(mgpu_ctx *c, int cat, double bytes) : ctx(c) {
+  if (!ctx || !ctx->ktimers_on) return;
+  r.cat = cat;
+  r.bytes = bytes;
+  if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) {
+    (void)hipGetLastError();
+    return;
+  }
+  on = hipEventRecord(r.a, ctx->stream) == hipSuccess;
+}
+KScope::~KScope() {
+  if (!on) return;
+  if (hipEventRecord(r.b, ctx->stream) == hipSuccess) ctx->krec.push_back(r);
+}
+
+static int flush_ktimers(mgpu_ctx *ctx) {
+  for (auto &r : ctx->krec) {
+    float ms = 0.f;
+    if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      ctx->kcount[r.cat] += 1;
+      ctx->kms[r.cat] += ms;
+      ctx->kbytes[r.cat] += r.bytes;
+    }
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  (void)hipGetLastError();
+  ctx->krec.clear();
+  return MGPU_OK;
+}
+
 extern "C" {
+
+const char *mgpu_kernel_name(int k) {
+  static const char *names[MGPU_K_COUNT] = {"deposit", "density", "fft_z",   "fft_y",  "fft_x",   "pofk",   "spectral",
+                                            "real",    "gather",  "kick",    "drift",  "lpt",     "readout"};
+  return (k >= 0 && k < MGPU_K_COUNT) ? names[k] : "?";
+}
+
+int mgpu_enable_kernel_timing(mgpu_ctx *ctx, int on) {
+  if (!ctx) return MGPU_E_INVALID;
+  if (!on) flush_ktimers(ctx);
+  ctx->ktimers_on = on != 0;
+  return MGPU_OK;
+}
+
+int mgpu_kernel_timing(mgpu_ctx *ctx, int k, int *launches, double *ms, double *bytes, int reset) {
+  if (!ctx || k < 0 || k >= MGPU_K_COUNT) return MGPU_E_INVALID;
+  flush_ktimers(ctx);
+  if (launches) *launches = ctx->kcount[k];
+  if (ms) *ms = ctx->kms[k];
+  if (bytes) *bytes = ctx->kbytes[k];
+  if (reset) {
+    ctx->kcount[k] = 0;
+    ctx->kms[k] = 0;
+    ctx->kbytes[k] = 0;
+  }
+  return MGPU_OK;
+}
 
 size_t mgpu_n2_table_len(int nmesh) { return (size_t)3 * (nmesh / 2) * (nmesh / 2) + 1; }
 
@@ -151,6 +209,7 @@ int mgpu_create(const mgpu_params *p, mgpu_ctx **out) {
 void mgpu_destroy(mgpu_ctx *ctx) {
   if (!ctx) return;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  flush_ktimers(ctx);
   void *ptrs[] = {ctx->pos,      ctx->vel,       ctx->disp,      ctx->lpt[0],    ctx->lpt[1],
                   ctx->slot[0],  ctx->slot[1],   ctx->slot[2],   ctx->slot[3],   ctx->cdelta[0],
                   ctx->cdelta[1], ctx->partial,  ctx->dsum,      ctx->pk,        ctx->n2tab[0],

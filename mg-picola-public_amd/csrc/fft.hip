@@ -346,6 +346,9 @@ void fft_plan_free(FFTPlan *plan) {
 }
 
 int fft_pass_z(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf, int forward) {
+  // compulsory traffic: N^2 rows of N reals one way and N/2+1 complex the other
+  const double rows = (double)plan->n * plan->n;
+  KScope ks(ctx, MGPU_K_FFT_Z, rows * (plan->n * 4.0 + (plan->n / 2 + 1) * 8.0));
   switch (plan->n) {
 #define X(S) \
   case S:    \
@@ -362,6 +365,8 @@ int fft_pass_col(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf, int axis, int 
   // axis 1 (y): element stride HP, one tile row per x; axis 0 (x): stride N*HP, one per y
   const size_t ls = axis == 1 ? HP : N * HP;
   const size_t os = axis == 1 ? N * HP : HP;
+  // compulsory traffic: read + write of the N*N*(N/2+1) half spectrum
+  KScope ks(ctx, axis == 1 ? MGPU_K_FFT_Y : MGPU_K_FFT_X, 2.0 * 8.0 * (double)N * N * (N / 2 + 1));
   switch (plan->n) {
 #define X(S) \
   case S:    \

@@ -258,8 +258,12 @@ void seedtable_host(int N, unsigned long seed, std::vector<unsigned> &t) {
 int spectrum(mgpu_ctx *ctx, int mode, const float2 *cd, const float2 *cd2, float2 *out, int a, int b,
              const double *g1 = nullptr, const double *g2 = nullptr) {
   const double kf = 2 * MGPU_PI / ctx->p.box;
-  hipLaunchKernelGGL(k_lpt_spectrum, dim3(cdiv(ctx->slot_elems, kBlock)), dim3(kBlock), 0, ctx->stream, mode, cd, cd2,
-                     out, ctx->N, ctx->HP, kf, a, b, g1, g2);
+  const double modes = (double)ctx->N * ctx->N * (ctx->N / 2 + 1);
+  {
+    KScope ks(ctx, MGPU_K_LPT, 8.0 * modes * (cd2 ? 3 : 2));
+    hipLaunchKernelGGL(k_lpt_spectrum, dim3(cdiv(ctx->slot_elems, kBlock)), dim3(kBlock), 0, ctx->stream, mode, cd,
+                       cd2, out, ctx->N, ctx->HP, kf, a, b, g1, g2);
+  }
   MGPU_HIP(ctx, hipGetLastError());
   return MGPU_OK;
 }
@@ -268,8 +272,12 @@ int spectrum(mgpu_ctx *ctx, int mode, const float2 *cd, const float2 *cd2, float
 int readout(mgpu_ctx *ctx, const float2 *mesh, double scale, float *dst) {
   const size_t np = ctx->np;
   const unsigned nb = cdiv(np, kBlock);
-  hipLaunchKernelGGL(k_lattice_read, dim3(nb), dim3(kBlock), 0, ctx->stream, (const float *)mesh, ctx->N, ctx->Np,
-                     2 * ctx->HP, scale, dst, ctx->partial);
+  {
+    // compulsory: one mesh sample read and one float written per lattice point
+    KScope ks(ctx, MGPU_K_READOUT, 8.0 * np);
+    hipLaunchKernelGGL(k_lattice_read, dim3(nb), dim3(kBlock), 0, ctx->stream, (const float *)mesh, ctx->N, ctx->Np,
+                       2 * ctx->HP, scale, dst, ctx->partial);
+  }
   MGPU_HIP(ctx, hipGetLastError());
   hipLaunchKernelGGL(k_reduce1, dim3(1), dim3(1024), 0, ctx->stream, ctx->partial, (int)nb, ctx->dsum);
   MGPU_HIP(ctx, hipGetLastError());

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string>
+#include <vector>
 
 #include "../../include/mgpu.h"
 
@@ -66,7 +67,27 @@ struct mgpu_ctx {
   mgpu_timers t{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
+  // per-kernel event timing
+  bool ktimers_on = false;
+  struct KRec {
+    int cat;
+    hipEvent_t a, b;
+    double bytes;
+  };
+  std::vector<KRec> krec;
+  int kcount[MGPU_K_COUNT] = {};
+  double kms[MGPU_K_COUNT] = {}, kbytes[MGPU_K_COUNT] = {};
+
   std::string err;
+};
+
+// brackets one kernel launch with events when per-kernel timing is on
+struct KScope {
+  mgpu_ctx *ctx;
+  mgpu_ctx::KRec r{};
+  bool on = false;
+  KScope(mgpu_ctx *c, int cat, double bytes);
+  ~KScope();
 };
 
 // ---------------------------------------------------------------- helpers

@@ -427,14 +427,19 @@ int pm_deposit(mgpu_ctx *ctx) {
   const double scale = (double)N / ctx->p.box;
   const double wpar = std::pow((double)N / (double)ctx->Np, 3);
   if (ctx->np) {
+    // compulsory: positions read once, the accumulator mesh written once
+    KScope ks(ctx, MGPU_K_DEPOSIT, 12.0 * ctx->np + 8.0 * cells);
     hipLaunchKernelGGL(k_deposit, dim3(cdiv(ctx->np, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->pos, ctx->np, N,
                        2 * ctx->HP, scale, wpar, ctx->acc);
     MGPU_HIP(ctx, hipGetLastError());
   }
   const bool need_dx = ctx->p.mg_active && ctx->p.model != MGPU_BRANSDICKE && ctx->p.model != MGPU_LCDM;
   const size_t total = (size_t)N * N * 2 * ctx->HP;
-  hipLaunchKernelGGL(k_acc_to_density, dim3(cdiv(total, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->acc, N, ctx->HP,
-                     (float *)ctx->slot[0], need_dx ? (float *)ctx->slot[1] : nullptr);
+  {
+    KScope ks(ctx, MGPU_K_DENSITY, 8.0 * cells + (need_dx ? 8.0 : 4.0) * cells);
+    hipLaunchKernelGGL(k_acc_to_density, dim3(cdiv(total, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->acc, N,
+                       ctx->HP, (float *)ctx->slot[0], need_dx ? (float *)ctx->slot[1] : nullptr);
+  }
   MGPU_HIP(ctx, hipGetLastError());
   return MGPU_OK;
 }
@@ -460,8 +465,11 @@ int pm_pofk(mgpu_ctx *ctx, const float2 *dk, mgpu_pofk *out) {
   }
   const size_t modes = (size_t)N * N * (N / 2 + 1);
   unsigned grid = std::min<size_t>(cdiv(modes, kBlock), 4096);
-  hipLaunchKernelGGL(k_pofk, dim3(grid), dim3(kBlock), shmem, ctx->stream, dk, N, ctx->HP, dwin, nbins,
-                     ctx->p.pofk_bintype, ctx->p.pofk_kmin, ctx->p.pofk_kmax, ctx->pk);
+  {
+    KScope ks(ctx, MGPU_K_POFK, 8.0 * modes);
+    hipLaunchKernelGGL(k_pofk, dim3(grid), dim3(kBlock), shmem, ctx->stream, dk, N, ctx->HP, dwin, nbins,
+                       ctx->p.pofk_bintype, ctx->p.pofk_kmin, ctx->p.pofk_kmax, ctx->pk);
+  }
   MGPU_HIP(ctx, hipGetLastError());
   std::vector<double> h(3 * nbins);
   MGPU_HIP(ctx, hipMemcpyAsync(h.data(), ctx->pk, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -492,45 +500,66 @@ int pm_fifth_force(mgpu_ctx *ctx, const mgpu_mg *mg) {
   const unsigned gc = cdiv(nc, kBlock), gr = cdiv(nr, kBlock);
   float2 *dk = ctx->slot[0], *dx = ctx->slot[1], *one = ctx->slot[2];
   int rc;
+  // compulsory bytes of one spectral (read + write C) and one real-space pass (2 reads + 1 write R)
+  const double modes = (double)N * N * (N / 2 + 1), cells = (double)N * N * N;
+  const double cbytes = 16.0 * modes, rbytes = 12.0 * cells;
   switch (ctx->p.model) {
-    case MGPU_BRANSDICKE:
+    case MGPU_BRANSDICKE: {
+      KScope ks(ctx, MGPU_K_SPECTRAL, cbytes);
       hipLaunchKernelGGL(k_scale, dim3(gc), dim3(kBlock), 0, ctx->stream, dk, dk, nc, mg->geff);
       MGPU_HIP(ctx, hipGetLastError());
       return MGPU_OK;
+    }
     case MGPU_FOFR:
     case MGPU_MBETA: {
       const double massterm2 = mg->a * mg->a * mg->mass2 / std::pow(2.0 * MGPU_PI * MGPU_INVERSE_H0_MPCH / ctx->p.box, 2);
       if (ctx->p.include_screening) {
         double nf = 1.0 / std::pow((double)N, 3);
         nf *= 1.5 * ctx->p.omega / mg->a * std::pow(ctx->p.box / MGPU_INVERSE_H0_MPCH / (2.0 * MGPU_PI), 2);
-        hipLaunchKernelGGL(k_divide_laplacian, dim3(gc), dim3(kBlock), 0, ctx->stream, dk, one, N, HP, nf);
+        {
+          KScope ks(ctx, MGPU_K_SPECTRAL, cbytes);
+          hipLaunchKernelGGL(k_divide_laplacian, dim3(gc), dim3(kBlock), 0, ctx->stream, dk, one, N, HP, nf);
+        }
         MGPU_HIP(ctx, hipGetLastError());
         if ((rc = fft_c2r_3d(ctx, &ctx->fft, one))) return rc;
-        hipLaunchKernelGGL(k_screen_potential, dim3(gr), dim3(kBlock), 0, ctx->stream, (float *)dx,
-                           (const float *)one, nr, 1, mg->phi_crit);
+        {
+          KScope ks(ctx, MGPU_K_REAL, rbytes);
+          hipLaunchKernelGGL(k_screen_potential, dim3(gr), dim3(kBlock), 0, ctx->stream, (float *)dx,
+                             (const float *)one, nr, 1, mg->phi_crit);
+        }
         MGPU_HIP(ctx, hipGetLastError());
         if ((rc = fft_r2c_3d(ctx, &ctx->fft, dx))) return rc;
       } else {
         // the reference re-transforms its delta(x) copy (src/mg.h:153-156), which reproduces delta(k)
         MGPU_HIP(ctx, hipMemcpyAsync(dx, dk, nc * sizeof(float2), hipMemcpyDeviceToDevice, ctx->stream));
       }
-      hipLaunchKernelGGL(k_eff_to_phi, dim3(gc), dim3(kBlock), 0, ctx->stream, dx, N, HP, mg->coupling, massterm2);
+      {
+        KScope ks(ctx, MGPU_K_SPECTRAL, cbytes);
+        hipLaunchKernelGGL(k_eff_to_phi, dim3(gc), dim3(kBlock), 0, ctx->stream, dx, N, HP, mg->coupling, massterm2);
+      }
       MGPU_HIP(ctx, hipGetLastError());
       ctx->have_phik = true;
       return MGPU_OK;
     }
     case MGPU_DGP: {
       if (!ctx->p.include_screening) {
+        KScope ks(ctx, MGPU_K_SPECTRAL, cbytes);
         hipLaunchKernelGGL(k_scale, dim3(gc), dim3(kBlock), 0, ctx->stream, dk, dx, nc, mg->coupling);
         MGPU_HIP(ctx, hipGetLastError());
       } else {
         const double nf = 1.0 / std::pow((double)N, 3);
-        hipLaunchKernelGGL(k_smooth, dim3(gc), dim3(kBlock), 0, ctx->stream, dk, one, N, HP, ctx->p.filter,
-                           2.0 * MGPU_PI / ctx->p.box * mg->rsmooth, nf);
+        {
+          KScope ks(ctx, MGPU_K_SPECTRAL, cbytes);
+          hipLaunchKernelGGL(k_smooth, dim3(gc), dim3(kBlock), 0, ctx->stream, dk, one, N, HP, ctx->p.filter,
+                             2.0 * MGPU_PI / ctx->p.box * mg->rsmooth, nf);
+        }
         MGPU_HIP(ctx, hipGetLastError());
         if ((rc = fft_c2r_3d(ctx, &ctx->fft, one))) return rc;
-        hipLaunchKernelGGL(k_screen_density, dim3(gr), dim3(kBlock), 0, ctx->stream, (float *)dx, (const float *)one,
-                           nr, 1, mg->coupling, mg->dgp_fac);
+        {
+          KScope ks(ctx, MGPU_K_REAL, rbytes);
+          hipLaunchKernelGGL(k_screen_density, dim3(gr), dim3(kBlock), 0, ctx->stream, (float *)dx,
+                             (const float *)one, nr, 1, mg->coupling, mg->dgp_fac);
+        }
         MGPU_HIP(ctx, hipGetLastError());
         if ((rc = fft_r2c_3d(ctx, &ctx->fft, dx))) return rc;
       }
@@ -547,9 +576,13 @@ int pm_forces(mgpu_ctx *ctx) {
   const int N = ctx->N;
   const double scale = 2. * M_PI / ctx->p.box;
   const double cnorm = 1.0 / (std::pow((double)N, 3) * scale);
-  hipLaunchKernelGGL(k_forces, dim3(cdiv(ctx->slot_elems, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->slot[0],
-                     ctx->have_phik ? ctx->slot[1] : nullptr, ctx->slot[3], ctx->slot[4], ctx->slot[5], N, ctx->HP,
-                     cnorm);
+  {
+    const double modes = (double)N * N * (N / 2 + 1);
+    KScope ks(ctx, MGPU_K_SPECTRAL, 8.0 * modes * ((ctx->have_phik ? 2 : 1) + 3));
+    hipLaunchKernelGGL(k_forces, dim3(cdiv(ctx->slot_elems, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->slot[0],
+                       ctx->have_phik ? ctx->slot[1] : nullptr, ctx->slot[3], ctx->slot[4], ctx->slot[5], N, ctx->HP,
+                       cnorm);
+  }
   MGPU_HIP(ctx, hipGetLastError());
   int rc;
   for (int a = 0; a < 3; a++)
@@ -565,9 +598,13 @@ int pm_gather(mgpu_ctx *ctx, double sumD[3]) {
   }
   const int N = ctx->N;
   const unsigned nb = cdiv(ctx->np, kBlock);
-  hipLaunchKernelGGL(k_gather, dim3(nb), dim3(kBlock), 0, ctx->stream, ctx->pos, ctx->np, N, 2 * ctx->HP,
-                     (double)N / ctx->p.box, (const float *)ctx->slot[3], (const float *)ctx->slot[4],
-                     (const float *)ctx->slot[5], ctx->disp, ctx->partial);
+  {
+    // compulsory: positions and Disp (12 B each per particle) and the three force meshes once
+    KScope ks(ctx, MGPU_K_GATHER, 24.0 * ctx->np + 12.0 * (double)N * N * N);
+    hipLaunchKernelGGL(k_gather, dim3(nb), dim3(kBlock), 0, ctx->stream, ctx->pos, ctx->np, N, 2 * ctx->HP,
+                       (double)N / ctx->p.box, (const float *)ctx->slot[3], (const float *)ctx->slot[4],
+                       (const float *)ctx->slot[5], ctx->disp, ctx->partial);
+  }
   MGPU_HIP(ctx, hipGetLastError());
   double s[3];
   int rc = pm_reduce3(ctx, (int)nb, s);
@@ -584,9 +621,13 @@ int pm_kick(mgpu_ctx *ctx, double dda, double A, double s1, double s2, const dou
   }
   const unsigned nb = cdiv(ctx->np, kBlock);
   const int sd = ctx->p.scale_dependent ? 1 : 0;
-  hipLaunchKernelGGL(k_kick, dim3(nb), dim3(kBlock), 0, ctx->stream, ctx->vel, ctx->disp, ctx->lpt[0], ctx->lpt[1],
-                     ctx->np, s1, s2, sd, ctx->p.omega, dda, A, ctx->p.use_cola, sumD[0], sumD[1], sumD[2],
-                     ctx->partial);
+  {
+    // vel, Disp read + written, one (scale-dependent) or two LPT vectors read
+    KScope ks(ctx, MGPU_K_KICK, (48.0 + (sd ? 12.0 : 24.0)) * ctx->np);
+    hipLaunchKernelGGL(k_kick, dim3(nb), dim3(kBlock), 0, ctx->stream, ctx->vel, ctx->disp, ctx->lpt[0], ctx->lpt[1],
+                       ctx->np, s1, s2, sd, ctx->p.omega, dda, A, ctx->p.use_cola, sumD[0], sumD[1], sumD[2],
+                       ctx->partial);
+  }
   MGPU_HIP(ctx, hipGetLastError());
   double s[3];
   int rc = pm_reduce3(ctx, (int)nb, s);
@@ -599,9 +640,13 @@ int pm_drift(mgpu_ctx *ctx, double dyyy, double s1, double s2, const double sumV
   PhaseTimer pt(ctx, &ctx->t.drift);
   if (ctx->np == 0) return MGPU_OK;
   const int sd = ctx->p.scale_dependent ? 2 : 0;
-  hipLaunchKernelGGL(k_drift, dim3(cdiv(ctx->np, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->pos, ctx->vel,
-                     ctx->lpt[0], ctx->lpt[1], ctx->np, s1, s2, sd, (float)ctx->p.box, dyyy, ctx->p.use_cola,
-                     sumV[0], sumV[1], sumV[2]);
+  {
+    // pos read + written, vel read, one or two LPT vectors read
+    KScope ks(ctx, MGPU_K_DRIFT, (36.0 + (sd ? 12.0 : 24.0)) * ctx->np);
+    hipLaunchKernelGGL(k_drift, dim3(cdiv(ctx->np, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->pos, ctx->vel,
+                       ctx->lpt[0], ctx->lpt[1], ctx->np, s1, s2, sd, (float)ctx->p.box, dyyy, ctx->p.use_cola,
+                       sumV[0], sumV[1], sumV[2]);
+  }
   MGPU_HIP(ctx, hipGetLastError());
   return MGPU_OK;
 }

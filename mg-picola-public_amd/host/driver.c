@@ -1,0 +1,356 @@
+/* The COLA simulation driver: initial conditions, the kick-drift-kick time
+ * loop over the output list and the outputs, restating src/main.c:33-787 with
+ * every mesh/particle operation on the GPU through libmgpu. */
+#include "internal.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+#include <time.h>
+
+struct mgp_sim {
+  mgp_params *p;
+  mgp_cosmo *c;
+  mgpu_ctx *g;
+  int verbose;
+  int stepDistr, StdDA;
+  double sumD[3], sumV[3];
+  /* P(k) binning after adjust_pofk_parameters (integer-k units) */
+  int nbins, bintype, subtract;
+  double kmin, kmax;
+  double *pk, *pkk, *pkn;
+  double *g1, *g2;   /* per-|n|^2 LPT growth tables */
+  char *err;
+  size_t errlen;
+  char errbuf[4096];
+  /* time-loop state (src/main.c:386-611) */
+  double A, AI, AF, AFF, Di, Di2, da;
+  int i, ts, nsteps, NoutputStart, finished;
+  double loop_time;
+  long steps_done;
+};
+typedef struct mgp_sim sim;
+
+static double now_s(void) {
+  struct timespec t;
+  clock_gettime(CLOCK_MONOTONIC, &t);
+  return t.tv_sec + 1e-9 * t.tv_nsec;
+}
+
+#define GPU(call)                                                                      \
+  do {                                                                                 \
+    int rc_ = (call);                                                                  \
+    if (rc_) {                                                                         \
+      snprintf(s->err, s->errlen, "%s failed (%d): %s", #call, rc_, mgpu_last_error(s->g)); \
+      return -1;                                                                       \
+    }                                                                                  \
+  } while (0)
+
+/* adjust_pofk_parameters (src/compute_pofk.c:758-805) on integer-k units */
+static void adjust_pofk(const mgp_params *p, int *nbins, int *bintype, int *sub, double *kmin, double *kmax) {
+  if (!(*bintype == 0 || *bintype == 1)) *bintype = 0;
+  if (!(*sub == 0 || *sub == 1)) *sub = 1;
+  if (*nbins <= 0) *nbins = p->Nmesh;
+  if (*kmax <= *kmin) {
+    if (*bintype == 0) *kmin = 0.0;
+    if (*bintype == 1) *kmin = 1.0;
+    *kmax = (double)p->Nmesh;
+  }
+  if (*kmin < 0.0) *kmin = *bintype == 0 ? 0.0 : 1.0;
+  if (*bintype == 1 && *kmin == 0.0) *kmin = 1.0;
+  if (*kmax > sqrt(3.0) * (double)p->Nmesh) *kmax = (double)p->Nmesh;
+}
+
+static int lpt_fields(sim *s, int slot, int field, double A, double AFF) {
+  mgp_lpt_tables(s->c, field, A, AFF, s->g1, s->g2);
+  GPU(mgpu_lpt_fields(s->g, slot, s->g1, s->g2));
+  return 0;
+}
+
+/* Kick, src/main.c:688-742 */
+static int kick(sim *s, double AI, double AF, double A) {
+  double dda;
+  if (s->StdDA == 0) dda = mgp_Sphi(s->c, AI, AF, A);
+  else {
+    extern double mgp_SphiStd(const mgp_cosmo *, double, double);
+    dda = mgp_SphiStd(s->c, AI, AF);
+  }
+  double s1 = 0, s2 = 0;
+  if (!s->p->scale_dependent) {
+    s1 = mgp_growth(s->c, MGP_G_ddDddy, A);
+    s2 = mgp_growth(s->c, MGP_G_ddD2ddy, A);
+  }
+  GPU(mgpu_kick(s->g, dda, A, s1, s2, s->sumD, s->sumV));
+  return 0;
+}
+
+/* Drift, src/main.c:747-787 */
+static int drift(sim *s, double A, double AFF, double AF, double Di, double Di2) {
+  double dyyy;
+  if (s->StdDA == 0) dyyy = mgp_Sq(s->c, A, AFF, AF);
+  else {
+    extern double mgp_SqStd(const mgp_cosmo *, double, double);
+    dyyy = mgp_SqStd(s->c, A, AFF);
+  }
+  double s1 = 0, s2 = 0;
+  if (!s->p->scale_dependent) {
+    s1 = mgp_growth(s->c, MGP_G_D, AFF) - Di;
+    s2 = mgp_growth(s->c, MGP_G_D2, AFF) - Di2;
+  }
+  GPU(mgpu_drift(s->g, dyyy, s1, s2, s->sumV));
+  return 0;
+}
+
+/* Output, src/main.c:792-1062 */
+static int output(sim *s, double A, double AF, double AFF, double z_label) {
+  const mgp_params *p = s->p;
+  const size_t np = (size_t)p->Nsample * p->Nsample * p->Nsample;
+  if (p->scale_dependent) {
+    if (lpt_fields(s, 1, MGP_FIELD_dDdy, A, AFF)) return -1;
+  }
+  float *pos = malloc(sizeof(float) * 3 * np), *vel = malloc(sizeof(float) * 3 * np);
+  float *l0 = malloc(sizeof(float) * 3 * np), *l1 = malloc(sizeof(float) * 3 * np);
+  int rc = mgpu_get_particles(s->g, pos, vel) || mgpu_get_lpt(s->g, 0, l0) || mgpu_get_lpt(s->g, 1, l1);
+  if (!rc) {
+    const double UnitTime_in_s = p->UnitLength_in_cm / p->UnitVelocity_in_cm_per_s;
+    const double Hubble = MGP_HUBBLE * UnitTime_in_s;
+    const double fac = Hubble / pow(A, 1.5), velfac = p->UnitVelocity_in_cm_per_s / 1.0e5;
+    const double dDdy = mgp_growth(s->c, MGP_G_dDdy, A), dD2dy = mgp_growth(s->c, MGP_G_dD2dy, A);
+    for (int k = 0; k < 3; k++)
+      for (size_t n = 0; n < np; n++) {
+        size_t i = k * np + n;
+        double lptv = p->scale_dependent ? (double)l1[i] : (double)l0[i] * dDdy + (double)l1[i] * dD2dy;
+        vel[i] = (float)(velfac * fac * (vel[i] - s->sumV[k] + lptv * p->UseCOLA));
+      }
+    if (p->gadget_style) rc = mgp_write_gadget(p, A, z_label, pos, vel, np);
+    if (!rc) rc = mgp_write_info(p, A, np);
+  }
+  free(pos);
+  free(vel);
+  free(l0);
+  free(l1);
+  if (rc) {
+    snprintf(s->err, s->errlen, "writing output at z = %f failed", z_label);
+    return -1;
+  }
+  if (p->scale_dependent && lpt_fields(s, 1, MGP_FIELD_deltaD, A, AFF)) return -1;
+  return 0;
+}
+
+static int get_displacements(sim *s, double A) {
+  const mgp_params *p = s->p;
+  mgpu_mg mg;
+  mgp_mg_coeffs(s->c, A, &mg);
+  const int want_pk = p->compute_pofk && p->pofk_compute_every_step;
+  mgpu_pofk pk = {s->pk, s->pkk, s->pkn};
+  GPU(mgpu_get_displacements(s->g, p->modified_gravity_active ? &mg : NULL, want_pk ? &pk : NULL, s->sumD));
+  if (want_pk && s->pk &&
+      mgp_write_pofk(p, A, "CDM", s->pk, s->pkk, s->pkn, s->nbins, s->bintype, s->kmin, s->kmax) && s->verbose)
+    fprintf(stderr, "warning: could not write P(k) file in %s\n", p->OutputDir);
+  return 0;
+}
+
+
+/* Set-up: growth tables and the GPU context (src/main.c:68-162). */
+mgp_sim *mgp_sim_create(mgp_params *p, int verbose, char *err, size_t errlen) {
+  sim *s = calloc(1, sizeof(sim));
+  s->p = p;
+  s->verbose = verbose;
+  s->err = s->errbuf;
+  s->errlen = sizeof s->errbuf;
+  mkdir(p->OutputDir, 0755);
+  if (p->UseCOLA) {
+    s->stepDistr = 0;
+    s->StdDA = 0;
+  } else {
+    s->stepDistr = 1;
+    s->StdDA = 2;
+  }
+  const double t_start = now_s();
+  s->c = mgp_cosmo_create(p, s->err, s->errlen);
+  if (!s->c) goto fail;
+  if (s->verbose) printf("[mgpicola] growth tables ready (%.2f s)\n", now_s() - t_start);
+  mgpu_params gp;
+  memset(&gp, 0, sizeof gp);
+  gp.nmesh = p->Nmesh;
+  gp.nsample = p->Nsample;
+  gp.box = p->Box;
+  gp.omega = p->Omega;
+  switch (p->model) {
+    case MGP_MODEL_FOFR:
+    case MGP_MODEL_FOFR_LCDM: gp.model = MGPU_FOFR; break;
+    case MGP_MODEL_DGP: gp.model = MGPU_DGP; break;
+    case MGP_MODEL_BRANSDICKE: gp.model = MGPU_BRANSDICKE; break;
+    default: gp.model = MGPU_MBETA;
+  }
+  gp.mg_active = p->modified_gravity_active;
+  gp.include_screening = p->include_screening;
+  gp.filter = p->model == MGP_MODEL_DGP ? MGPU_FILTER_GAUSSIAN : MGPU_FILTER_NONE;
+  gp.scale_dependent = p->scale_dependent;
+  gp.use_cola = p->UseCOLA;
+  s->nbins = p->pofk_nbins;
+  s->bintype = p->pofk_bintype;
+  s->subtract = p->pofk_subtract_shotnoise;
+  s->kmin = p->pofk_kmin * p->Box / (2.0 * M_PI);
+  s->kmax = p->pofk_kmax * p->Box / (2.0 * M_PI);
+  adjust_pofk(p, &s->nbins, &s->bintype, &s->subtract, &s->kmin, &s->kmax);
+  gp.pofk_nbins = s->nbins;
+  gp.pofk_bintype = s->bintype;
+  gp.pofk_subtract_shotnoise = s->subtract;
+  gp.pofk_kmin = s->kmin;
+  gp.pofk_kmax = s->kmax;
+  if (mgpu_create(&gp, &s->g)) {
+    snprintf(s->err, s->errlen, "mgpu_create failed: %s", mgpu_last_error(NULL));
+    goto fail;
+  }
+  s->pk = calloc(s->nbins, sizeof(double));
+  s->pkk = calloc(s->nbins, sizeof(double));
+  s->pkn = calloc(s->nbins, sizeof(double));
+  const size_t n2len = mgpu_n2_table_len(p->Nmesh);
+  s->g1 = malloc(sizeof(double) * n2len);
+  s->g2 = malloc(sizeof(double) * n2len);
+  return s;
+fail:
+  snprintf(err, errlen, "%s", s->err);
+  mgp_sim_free(s);
+  return NULL;
+}
+
+void mgp_sim_free(mgp_sim *s) {
+  if (!s) return;
+  if (s->g) mgpu_destroy(s->g);
+  mgp_cosmo_free(s->c);
+  free(s->pk);
+  free(s->pkk);
+  free(s->pkn);
+  free(s->g1);
+  free(s->g2);
+  free(s);
+}
+
+mgpu_ctx *mgp_sim_gpu(mgp_sim *s) { return s->g; }
+mgp_cosmo *mgp_sim_cosmo(mgp_sim *s) { return s->c; }
+const char *mgp_sim_error(const mgp_sim *s) { return s->errbuf; }
+double mgp_sim_a(const mgp_sim *s) { return s->A; }
+
+/* Initial conditions and particle set-up (src/main.c:164-384). */
+int mgp_sim_ic(mgp_sim *s) {
+  const mgp_params *p = s->p;
+  const size_t n2len = mgpu_n2_table_len(p->Nmesh);
+  s->A = 1.0 / (1.0 + p->Init_Redshift);
+  s->Di = mgp_growth(s->c, MGP_G_D, s->A);
+  s->Di2 = mgp_growth(s->c, MGP_G_D2, s->A);
+  s->AI = s->AF = s->AFF = s->A;
+  double *pk = malloc(sizeof(double) * n2len);
+  mgp_ic_pk_table(s->c, pk);
+  int rc = mgpu_ic_displacement_fields(s->g, (unsigned long)p->Seed, p->SphereMode,
+                                       p->amplitude_fixed_initial_condition, p->inverted_initial_condition, pk);
+  free(pk);
+  if (rc) {
+    snprintf(s->err, s->errlen, "IC generation failed: %s", mgpu_last_error(s->g));
+    return -1;
+  }
+  if (p->scale_dependent) {
+    if (lpt_fields(s, 0, MGP_FIELD_D, s->A, s->AFF)) return -1;
+    if (lpt_fields(s, 1, MGP_FIELD_dDdy, s->A, s->AFF)) return -1;
+  }
+  GPU(mgpu_init_particles(s->g, s->Di, s->Di2));
+  s->sumV[0] = s->sumV[1] = s->sumV[2] = 0.0;
+  s->NoutputStart = 0;
+  if ((p->Init_Redshift - p->out_z[0]) / p->Init_Redshift <= 1.0E-6 || p->Init_Redshift <= 1.0e-6) {
+    if (output(s, s->A, s->AF, s->AFF, p->out_z[0])) return -1;
+    if (p->Noutputs == 1) {
+      s->finished = 1;
+      return 0;
+    }
+    s->NoutputStart++;
+  }
+  s->i = s->NoutputStart;
+  s->ts = 0;
+  s->nsteps = -1;
+  return 0;
+}
+
+/* One iteration of the time loop body (src/main.c:405-610).  *counted is 1
+ * when the iteration is one of the timed steps (not the final sync half-kick). */
+int mgp_sim_advance(mgp_sim *s, int *counted) {
+  const mgp_params *p = s->p;
+  *counted = 0;
+  if (s->finished) return 0;
+  if (s->nsteps < 0) { /* entering output interval i */
+    s->da = 0.0;
+    if (s->i == p->Noutputs) s->nsteps = 1;
+    else {
+      s->nsteps = p->out_nsteps[s->i];
+      double ao = 1.0 / (1.0 + p->out_z[s->i]);
+      if (s->stepDistr == 0) s->da = (ao - s->A) / (double)s->nsteps;
+      else s->da = (log(ao) - log(s->A)) / (double)s->nsteps;
+    }
+    s->ts = 0;
+  }
+  const double t0 = now_s();
+  const double da = s->da;
+  double A = s->A;
+  if (s->ts == 0 && s->i != s->NoutputStart) s->AF = A;
+  else s->AF = s->stepDistr == 0 ? A + da * 0.5 : A * exp(da * 0.5);
+  s->AFF = s->stepDistr == 0 ? A + da : A * exp(da);
+  if (s->verbose) {
+    if (s->i != p->Noutputs) printf("[mgpicola] step a = %f -> %f\n", A, s->AFF);
+    else printf("[mgpicola] final half timestep\n");
+  }
+  if (get_displacements(s, A)) return -1;
+  if (p->scale_dependent) {
+    if (lpt_fields(s, 1, MGP_FIELD_deltaD, A, s->AFF)) return -1;
+    if (lpt_fields(s, 0, MGP_FIELD_ddDddy, A, s->AFF)) return -1;
+  }
+  if (kick(s, s->AI, s->AF, A)) return -1;
+  if (s->ts == 0 && s->i != s->NoutputStart) {
+    if (output(s, A, s->AF, s->AFF, p->out_z[s->i - 1])) return -1;
+    if (s->i == p->Noutputs) {
+      s->finished = 1;
+      return 0;
+    }
+    s->AI = A;
+    s->AF = s->stepDistr == 0 ? A + da * 0.5 : A * exp(da * 0.5);
+    s->sumD[0] = s->sumD[1] = s->sumD[2] = 0.0;
+    if (kick(s, s->AI, s->AF, A)) return -1;
+  }
+  if (drift(s, A, s->AFF, s->AF, s->Di, s->Di2)) return -1;
+  s->A = s->AFF;
+  s->AI = s->AF;
+  s->Di = mgp_growth(s->c, MGP_G_D, s->A);
+  s->Di2 = mgp_growth(s->c, MGP_G_D2, s->A);
+  GPU(mgpu_sync(s->g));
+  s->loop_time += now_s() - t0;
+  s->steps_done++;
+  *counted = 1;
+  if (++s->ts == s->nsteps) {
+    s->i++;
+    s->nsteps = -1;
+    if (s->i > p->Noutputs) s->finished = 1;
+  }
+  return 0;
+}
+
+int mgp_sim_finished(const mgp_sim *s) { return s->finished; }
+
+int mgp_run(mgp_params *p, int verbose, mgp_run_stats *st, char *err, size_t errlen) {
+  err[0] = 0;
+  sim *s = mgp_sim_create(p, verbose, err, errlen);
+  if (!s) return -1;
+  int rc = mgp_sim_ic(s);
+  while (!rc && !s->finished) {
+    int counted;
+    rc = mgp_sim_advance(s, &counted);
+  }
+  if (rc) snprintf(err, errlen, "%s", s->errbuf);
+  if (st) {
+    st->step_seconds = s->loop_time;
+    st->steps = s->steps_done;
+    st->final_A = s->A;
+  }
+  mgp_sim_free(s);
+  return rc;
+}

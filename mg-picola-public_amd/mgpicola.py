@@ -24,6 +24,7 @@ MGPU_SYMBOLS = [
     "mgpu_pm_density", "mgpu_pm_deposit", "mgpu_pm_transform", "mgpu_power_spectrum", "mgpu_fifth_force", "mgpu_forces", "mgpu_mesh_to_particles",
     "mgpu_get_displacements", "mgpu_kick", "mgpu_drift", "mgpu_ic_displacement_fields", "mgpu_init_particles",
     "mgpu_lpt_fields", "mgpu_get_mesh", "mgpu_set_mesh", "mgpu_get_timers", "mgpu_enable_timers",
+    "mgpu_kernel_name", "mgpu_enable_kernel_timing", "mgpu_kernel_timing",
 ]
 
 
@@ -283,3 +284,264 @@ class MGPU:
 
     def sync(self):
         self._ck(lib().mgpu_sync(self.ctx), "sync")
+
+
+# ---------------------------------------------------------------------------
+# host library (libmgpicola.so): paramfile interface, cosmology, the run
+# ---------------------------------------------------------------------------
+MODELS = {"FOFR": 0, "FOFR_LCDM": 1, "DGP": 2, "BRANSDICKE": 3, "MBETA": 4}
+G_D, G_dDdy, G_ddDddy, G_D2, G_dD2dy, G_ddD2ddy = range(6)
+FIELD_D, FIELD_dDdy, FIELD_ddDddy, FIELD_deltaD = range(4)
+
+
+class HostParams(C.Structure):
+    _fields_ = (
+        [(n, C.c_int) for n in ("model", "scale_dependent", "compute_pofk", "gadget_style",
+                                "modified_gravity_active", "use_lcdm_growth_factors", "input_pofk_is_for_lcdm",
+                                "input_sigma8_is_for_lcdm", "inverted_initial_condition",
+                                "amplitude_fixed_initial_condition", "include_screening",
+                                "pofk_compute_every_step", "pofk_compute_rsd_pofk", "pofk_nbins", "pofk_bintype",
+                                "pofk_subtract_shotnoise")]
+        + [(n, C.c_double) for n in ("pofk_kmin", "pofk_kmax", "fofr0", "nfofr", "range_symm", "assb_symm",
+                                     "beta_symm", "Rsmooth", "rcH0_DGP", "wBD", "Omegah2", "Omegarh2", "Omegavh2")]
+        + [(n, C.c_char * 500) for n in ("OutputDir", "FileBase", "OutputRedshiftFile")]
+        + [("NumFilesWrittenInParallel", C.c_int), ("UseCOLA", C.c_int), ("Buffer", C.c_double),
+           ("Nmesh", C.c_int), ("Nsample", C.c_int), ("Box", C.c_double), ("Init_Redshift", C.c_double),
+           ("Seed", C.c_int), ("SphereMode", C.c_int), ("WhichSpectrum", C.c_int), ("WhichTransfer", C.c_int),
+           ("FileWithInputSpectrum", C.c_char * 500), ("FileWithInputTransfer", C.c_char * 500)]
+        + [(n, C.c_double) for n in ("Omega", "OmegaBaryon", "HubbleParam", "Sigma8", "PrimordialIndex",
+                                     "UnitVelocity_in_cm_per_s", "UnitLength_in_cm", "UnitMass_in_g",
+                                     "InputSpectrum_UnitLength_in_cm")]
+        + [("Noutputs", C.c_int), ("out_z", C.c_double * 512), ("out_nsteps", C.c_int * 512)]
+    )
+
+
+class RunStats(C.Structure):
+    _fields_ = [("step_seconds", C.c_double), ("steps", C.c_long), ("final_A", C.c_double)]
+
+
+_hlib = None
+
+
+def hlib():
+    """Load libmgpicola.so (the C host; it links libmgpu.so)."""
+    global _hlib
+    if _hlib is None:
+        if not os.path.exists(LIBHOST):
+            raise MGPUError(f"{LIBHOST} not built: run __graft_entry__.build()")
+        L = C.CDLL(LIBHOST)
+        vp, d = C.c_void_p, C.c_double
+        P = C.POINTER(HostParams)
+        L.mgp_params_defaults.argtypes = [P, C.c_int]
+        L.mgp_params_defaults.restype = None
+        L.mgp_read_paramfile.argtypes = [C.c_char_p, P, C.c_char_p, C.c_size_t]
+        L.mgp_read_outputs.argtypes = [P, C.c_char_p, C.c_size_t]
+        L.mgp_param_tag_count.argtypes = [P]
+        L.mgp_param_tag_name.argtypes = [P, C.c_int]
+        L.mgp_param_tag_name.restype = C.c_char_p
+        L.mgp_cosmo_create.argtypes = [P, C.c_char_p, C.c_size_t]
+        L.mgp_cosmo_create.restype = vp
+        L.mgp_cosmo_free.argtypes = [vp]
+        L.mgp_cosmo_free.restype = None
+        for n in ("mgp_hubble", "mgp_dhubbleda", "mgp_coupling", "mgp_mass2", "mgp_beta", "mgp_beta_dgp",
+                  "mgp_phi_crit", "mgp_power_spec", "mgp_mg_sigma8_enhancement"):
+            getattr(L, n).argtypes = [vp, d]
+            getattr(L, n).restype = d
+        for n in ("mgp_growth", "mgp_growth_lcdm", "mgp_growth_lcdmfit"):
+            getattr(L, n).argtypes = [vp, C.c_int, d]
+            getattr(L, n).restype = d
+        L.mgp_growth_sd.argtypes = [vp, C.c_int, d, d]
+        L.mgp_growth_sd.restype = d
+        for n in ("mgp_Sq", "mgp_Sphi"):
+            getattr(L, n).argtypes = [vp, d, d, d]
+            getattr(L, n).restype = d
+        for n in ("mgp_GeffoverG", "mgp_screening_potential", "mgp_screening_density", "mgp_mg_pofk_ratio"):
+            getattr(L, n).argtypes = [vp, d, d]
+            getattr(L, n).restype = d
+        L.mgp_sigma8_norm.argtypes = [vp]
+        L.mgp_sigma8_norm.restype = d
+        L.mgp_mg_coeffs.argtypes = [vp, d, C.POINTER(MG)]
+        L.mgp_mg_coeffs.restype = None
+        L.mgp_ic_pk_table.argtypes = [vp, vp]
+        L.mgp_ic_pk_table.restype = None
+        L.mgp_lpt_tables.argtypes = [vp, C.c_int, d, d, vp, vp]
+        L.mgp_lpt_tables.restype = None
+        L.mgp_run.argtypes = [P, C.c_int, C.POINTER(RunStats), C.c_char_p, C.c_size_t]
+        _hlib = L
+    return _hlib
+
+
+def write_paramfile(path, values, model="FOFR", compute_pofk=True):
+    """Write a paramfile holding exactly the tags the configuration expects."""
+    hp = HostParams()
+    hlib().mgp_params_defaults(C.byref(hp), MODELS[model])
+    hp.compute_pofk = 1 if compute_pofk else 0
+    names = [hlib().mgp_param_tag_name(C.byref(hp), i).decode() for i in range(hlib().mgp_param_tag_count(C.byref(hp)))]
+    missing = [n for n in names if n not in values]
+    if missing:
+        raise ValueError(f"paramfile values missing: {missing}")
+    with open(path, "w") as f:
+        f.write("% generated paramfile\n")
+        for n in names:
+            f.write(f"{n:<36s} {values[n]}\n")
+    return names
+
+
+def read_params(paramfile, model="FOFR", compute_pofk=True):
+    hp = HostParams()
+    hlib().mgp_params_defaults(C.byref(hp), MODELS[model])
+    hp.compute_pofk = 1 if compute_pofk else 0
+    err = C.create_string_buffer(4096)
+    if hlib().mgp_read_paramfile(paramfile.encode(), C.byref(hp), err, 4096):
+        raise ValueError(err.value.decode())
+    if hlib().mgp_read_outputs(C.byref(hp), err, 4096):
+        raise ValueError(err.value.decode())
+    return hp
+
+
+class Cosmo:
+    """Growth factors, time integrals, hooks and P(k) of the host (libmgpicola)."""
+
+    def __init__(self, hp):
+        self.hp = hp
+        err = C.create_string_buffer(4096)
+        self.h = hlib().mgp_cosmo_create(C.byref(hp), err, 4096)
+        if not self.h:
+            raise MGPUError(err.value.decode())
+
+    def close(self):
+        if self.h:
+            hlib().mgp_cosmo_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __getattr__(self, name):
+        f = getattr(hlib(), "mgp_" + name)
+        return lambda *a: f(self.h, *a)
+
+    def mg(self, a):
+        m = MG()
+        hlib().mgp_mg_coeffs(self.h, a, C.byref(m))
+        return m
+
+    def ic_pk_table(self):
+        out = np.zeros(n2_table_len(self.hp.Nmesh))
+        hlib().mgp_ic_pk_table(self.h, out.ctypes.data)
+        return out
+
+    def lpt_tables(self, field, A, AFF):
+        g1 = np.zeros(n2_table_len(self.hp.Nmesh))
+        g2 = np.zeros_like(g1)
+        hlib().mgp_lpt_tables(self.h, field, A, AFF, g1.ctypes.data, g2.ctypes.data)
+        return g1, g2
+
+
+def run(hp, verbose=False):
+    st = RunStats()
+    err = C.create_string_buffer(4096)
+    if hlib().mgp_run(C.byref(hp), 1 if verbose else 0, C.byref(st), err, 4096):
+        raise MGPUError(err.value.decode())
+    return {"step_seconds": st.step_seconds, "steps": st.steps, "final_A": st.final_A}
+
+
+KERNELS = ["deposit", "density", "fft_z", "fft_y", "fft_x", "pofk", "spectral", "real", "gather", "kick", "drift",
+           "lpt", "readout"]
+
+
+def _kt_setup():
+    L = lib()
+    if not getattr(L, "_kt", False):
+        L.mgpu_enable_kernel_timing.argtypes = [C.c_void_p, C.c_int]
+        L.mgpu_kernel_timing.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double), C.c_int]
+        L.mgpu_kernel_name.restype = C.c_char_p
+        L.mgpu_kernel_name.argtypes = [C.c_int]
+        L._kt = True
+    return L
+
+
+def enable_kernel_timing(g, on=True):
+    g._ck(_kt_setup().mgpu_enable_kernel_timing(g.ctx, 1 if on else 0), "enable_kernel_timing")
+
+
+def kernel_timing(g, reset=True):
+    """{name: (launches, total_ms, total_algorithmic_bytes)} since the last reset."""
+    L = _kt_setup()
+    out = {}
+    for k, name in enumerate(KERNELS):
+        n, ms, b = C.c_int(), C.c_double(), C.c_double()
+        g._ck(L.mgpu_kernel_timing(g.ctx, k, C.byref(n), C.byref(ms), C.byref(b), 1 if reset else 0), "kernel_timing")
+        out[name] = (n.value, ms.value, b.value)
+    return out
+
+
+class Sim:
+    """The C host's simulation (libmgpicola mgp_sim_*): IC, then one time-loop iteration per advance()."""
+
+    def __init__(self, hp, verbose=False):
+        L = hlib()
+        if not getattr(L, "_sim", False):
+            L.mgp_sim_create.argtypes = [C.POINTER(HostParams), C.c_int, C.c_char_p, C.c_size_t]
+            L.mgp_sim_create.restype = C.c_void_p
+            for n in ("mgp_sim_ic", "mgp_sim_finished"):
+                getattr(L, n).argtypes = [C.c_void_p]
+            L.mgp_sim_advance.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+            L.mgp_sim_free.argtypes = [C.c_void_p]
+            L.mgp_sim_free.restype = None
+            L.mgp_sim_gpu.argtypes = [C.c_void_p]
+            L.mgp_sim_gpu.restype = C.c_void_p
+            L.mgp_sim_cosmo.argtypes = [C.c_void_p]
+            L.mgp_sim_cosmo.restype = C.c_void_p
+            L.mgp_sim_error.argtypes = [C.c_void_p]
+            L.mgp_sim_error.restype = C.c_char_p
+            L.mgp_sim_a.argtypes = [C.c_void_p]
+            L.mgp_sim_a.restype = C.c_double
+            L._sim = True
+        self.hp = hp
+        err = C.create_string_buffer(4096)
+        self.h = L.mgp_sim_create(C.byref(hp), 1 if verbose else 0, err, 4096)
+        if not self.h:
+            raise MGPUError(err.value.decode())
+        # an MGPU view sharing the context (not owning it)
+        g = MGPU.__new__(MGPU)
+        g.p = None
+        g.n, g.ns, g.np = hp.Nmesh, hp.Nsample, hp.Nsample ** 3
+        g.ctx = C.c_void_p(L.mgp_sim_gpu(self.h))
+        g.close = lambda: None
+        self.gpu = g
+
+    def _ck(self, rc, what):
+        if rc:
+            raise MGPUError(f"{what} failed: {hlib().mgp_sim_error(self.h).decode()}")
+
+    def ic(self):
+        self._ck(hlib().mgp_sim_ic(self.h), "mgp_sim_ic")
+
+    def advance(self):
+        counted = C.c_int()
+        self._ck(hlib().mgp_sim_advance(self.h, C.byref(counted)), "mgp_sim_advance")
+        return bool(counted.value)
+
+    @property
+    def finished(self):
+        return bool(hlib().mgp_sim_finished(self.h))
+
+    @property
+    def a(self):
+        return hlib().mgp_sim_a(self.h)
+
+    def close(self):
+        if self.h:
+            hlib().mgp_sim_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
