@@ -162,6 +162,21 @@ int mgpu_init_particles(mgpu_ctx *ctx, double Di, double Di2);
  * src/2LPT.c:1539-1735, src/2LPT.c:1758-2005 */
 int mgpu_lpt_fields(mgpu_ctx *ctx, int slot, const double *g1_n2, const double *g2_n2);
 
+/* Growth factor of one LPT order as a function of ln k, as the restriction of
+ * the reference's bicubic (ln a, ln k) spline to the current a: cubic Hermite
+ * pieces on nk uniform ln k nodes, g(k) = norm * num(ln k) / den(ln k), ln k
+ * clamped to the node range (Lookup_GSL_2D_Spline, src/wrappers.c:129-137;
+ * growth_X_scaledependent, src/cosmo.c:987-1010). */
+typedef struct {
+  double lnk0, dlnk;
+  int nk;
+  const double *num, *dnum;  /* numerator value and d/dln k at the nodes */
+  const double *den, *dden;  /* denominator (growth at a = 1) value and derivative */
+  double norm;               /* normfactor: 1 (1LPT) or -3/7/N^3 (2LPT) */
+} mgpu_kslice;
+/* mgpu_lpt_fields with the growth evaluated on the GPU from the two slices. */
+int mgpu_lpt_fields_slices(mgpu_ctx *ctx, int slot, const mgpu_kslice *o1, const mgpu_kslice *o2);
+
 /* ---- inspection (tests, output) ---- */
 int mgpu_get_mesh(mgpu_ctx *ctx, int mesh, double *host);
 int mgpu_set_mesh(mgpu_ctx *ctx, int mesh, const double *host);

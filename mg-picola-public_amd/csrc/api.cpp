@@ -182,9 +182,16 @@ int mgpu_create(const mgpu_params *p, mgpu_ctx **out) {
   if ((rc = alloc(ctx, (void **)&force, 3 * sb, "force meshes"))) return fail(rc);
   for (int a = 0; a < 3; a++) ctx->slot[3 + a] = force + a * ctx->slot_elems;
   ctx->acc = (unsigned long long *)force;  // needs 8 N^3 bytes <= 2 slots
-  if (p->scale_dependent)
+  if (p->scale_dependent) {
     for (int s = 0; s < 2; s++)
       if ((rc = alloc(ctx, (void **)&ctx->cdelta[s], sb, "stored delta(k)"))) return fail(rc);
+    // LPT fields by the exact lattice fold when the mesh is a multiple of the lattice
+    if (ctx->N % ctx->Np == 0 && fft_size_supported(ctx->Np)) {
+      if ((rc = fft_plan_init(ctx, &ctx->fft_lattice, ctx->Np))) return fail(rc);
+      const size_t nl = (size_t)ctx->Np * ctx->Np * ctx->fft_lattice.hp;
+      if ((rc = alloc(ctx, (void **)&ctx->lattice_slot, 3 * nl * sizeof(float2), "lattice spectra"))) return fail(rc);
+    }
+  }
   ctx->partial_len = 3 * (size_t)std::max<size_t>(cdiv(ctx->np, 256), (size_t)ctx->N) + 3 * 1024;
   if ((rc = alloc(ctx, (void **)&ctx->partial, ctx->partial_len * sizeof(double), "reduction scratch"))) return fail(rc);
   if ((rc = alloc(ctx, (void **)&ctx->dsum, 8 * sizeof(double), "sums"))) return fail(rc);
@@ -213,7 +220,7 @@ void mgpu_destroy(mgpu_ctx *ctx) {
   void *ptrs[] = {ctx->pos,      ctx->vel,       ctx->disp,      ctx->lpt[0],    ctx->lpt[1],
                   ctx->slot[0],  ctx->slot[1],   ctx->slot[2],   ctx->slot[3],   ctx->cdelta[0],
                   ctx->cdelta[1], ctx->partial,  ctx->dsum,      ctx->pk,        ctx->n2tab[0],
-                  ctx->n2tab[1], ctx->lattice_slot};
+                  ctx->n2tab[1], ctx->lattice_slot, ctx->slice_buf};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   fft_plan_free(&ctx->fft);
@@ -371,6 +378,13 @@ int mgpu_init_particles(mgpu_ctx *ctx, double Di, double Di2) {
 int mgpu_lpt_fields(mgpu_ctx *ctx, int slot, const double *g1_n2, const double *g2_n2) {
   if (check_ctx(ctx) || slot < 0 || slot > 1 || !g1_n2 || !g2_n2) return MGPU_E_INVALID;
   return ic_lpt_fields(ctx, slot, g1_n2, g2_n2);
+}
+
+int mgpu_lpt_fields_slices(mgpu_ctx *ctx, int slot, const mgpu_kslice *o1, const mgpu_kslice *o2) {
+  if (check_ctx(ctx) || slot < 0 || slot > 1 || !o1 || !o2 || !o1->num || !o1->dnum || !o1->den || !o1->dden ||
+      !o2->num || !o2->dnum || !o2->den || !o2->dden)
+    return mgpu_fail(ctx, MGPU_E_INVALID, "bad arguments to mgpu_lpt_fields_slices");
+  return ic_lpt_fields_slices(ctx, slot, o1, o2);
 }
 
 // ---- mesh inspection: convert between the device layout and the FFTW layout (no ghost slab)

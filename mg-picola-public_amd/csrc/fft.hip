@@ -219,10 +219,13 @@ __global__ __launch_bounds__(RowCfg<N>::NT) void k_fft_z_r2c(float2 *__restrict_
   }
 }
 
-// c2r along z: X[0..N/2] (Im of ends dropped) -> real row, zero padding.
+// c2r along z: X[0..N/2] (Im of ends dropped) -> real row.  The row is written
+// to out with a row stride of orow float2 (out = buf, orow = HP in place, the
+// rest of the row zero padding; or a compact [N^2][N] real array, orow = N/2).
 template <int N>
 __global__ __launch_bounds__(RowCfg<N>::NT) void k_fft_z_c2r(float2 *__restrict__ buf, int HP, size_t nrows,
-                                                              const float2 *__restrict__ tw) {
+                                                              const float2 *__restrict__ tw, float2 *__restrict__ out,
+                                                              int orow) {
   using C = RowCfg<N>;
   extern __shared__ float2 lds[];
   const int tid = threadIdx.x;
@@ -253,11 +256,11 @@ __global__ __launch_bounds__(RowCfg<N>::NT) void k_fft_z_c2r(float2 *__restrict_
   }
   __syncthreads();
   stockham<C::M, C::RB, C::NT, +1, C::LS, 1, false, 1>(lds, tw, 2, tid);
-  for (int e = tid; e < C::RB * HP; e += C::NT) {
-    const int r = e / HP, k = e % HP;
+  for (int e = tid; e < C::RB * orow; e += C::NT) {
+    const int r = e / orow, k = e % orow;
     const size_t row = row0 + r;
     if (row >= nrows) continue;
-    buf[row * HP + k] = k < C::M ? lds[r * C::LS + k] : make_float2(0.f, 0.f);
+    out[row * orow + k] = k < C::M ? lds[r * C::LS + k] : make_float2(0.f, 0.f);
   }
 }
 
@@ -277,7 +280,7 @@ int launch_col(mgpu_ctx *ctx, float2 *buf, size_t line_stride, size_t outer_stri
 }
 
 template <int N>
-int launch_z(mgpu_ctx *ctx, float2 *buf, int HP, const float2 *tw, int forward) {
+int launch_z(mgpu_ctx *ctx, float2 *buf, int HP, const float2 *tw, int forward, float2 *out = nullptr) {
   using C = RowCfg<N>;
   size_t nrows = (size_t)N * N;
   size_t lds = (size_t)C::RB * C::LS * sizeof(float2);
@@ -289,7 +292,8 @@ int launch_z(mgpu_ctx *ctx, float2 *buf, int HP, const float2 *tw, int forward) 
   } else {
     auto kern = k_fft_z_c2r<N>;
     if (lds > 65536) MGPU_HIP(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(kern, grid, dim3(C::NT), lds, ctx->stream, buf, HP, nrows, tw);
+    hipLaunchKernelGGL(kern, grid, dim3(C::NT), lds, ctx->stream, buf, HP, nrows, tw, out ? out : buf,
+                       out ? N / 2 : HP);
   }
   MGPU_HIP(ctx, hipGetLastError());
   return MGPU_OK;
@@ -345,14 +349,14 @@ void fft_plan_free(FFTPlan *plan) {
   plan->tw = nullptr;
 }
 
-int fft_pass_z(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf, int forward) {
+int fft_pass_z(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf, int forward, float *out) {
   // compulsory traffic: N^2 rows of N reals one way and N/2+1 complex the other
   const double rows = (double)plan->n * plan->n;
   KScope ks(ctx, MGPU_K_FFT_Z, rows * (plan->n * 4.0 + (plan->n / 2 + 1) * 8.0));
   switch (plan->n) {
 #define X(S) \
   case S:    \
-    return launch_z<S>(ctx, buf, plan->hp, plan->tw, forward);
+    return launch_z<S>(ctx, buf, plan->hp, plan->tw, forward, (float2 *)out);
     MGPU_FFT_SIZES(X)
 #undef X
     default:
@@ -381,15 +385,15 @@ int fft_pass_col(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf, int axis, int 
 int fft_r2c_3d(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf) {
   PhaseTimer pt(ctx, &ctx->t.fft);
   int rc;
-  if ((rc = fft_pass_z(ctx, plan, buf, 1))) return rc;
+  if ((rc = fft_pass_z(ctx, plan, buf, 1, nullptr))) return rc;
   if ((rc = fft_pass_col(ctx, plan, buf, 1, 1))) return rc;
   return fft_pass_col(ctx, plan, buf, 0, 1);
 }
 
-int fft_c2r_3d(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf) {
+int fft_c2r_3d(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf, float *out) {
   PhaseTimer pt(ctx, &ctx->t.fft);
   int rc;
   if ((rc = fft_pass_col(ctx, plan, buf, 0, 0))) return rc;
   if ((rc = fft_pass_col(ctx, plan, buf, 1, 0))) return rc;
-  return fft_pass_z(ctx, plan, buf, 0);
+  return fft_pass_z(ctx, plan, buf, 0, out);
 }

@@ -237,6 +237,127 @@ __global__ __launch_bounds__(kBlock) void k_init_particles(float *__restrict__ p
   }
 }
 
+// ---------------------------------------------------------------- lattice fold
+// LPT field components X_a = i k_a/k^2 (g1 cd1 + g2 cd2) at half-spectrum index
+// (i, j, l), l in [0, N/2], 2LPT.c wavenumbers (src/2LPT.c:1583-1629).
+__device__ __forceinline__ void lpt_x(int i, int j, int l, int N, int HP, int Np, double kf,
+                                      const float2 *__restrict__ cd1, const float2 *__restrict__ cd2,
+                                      const double *__restrict__ g1, const double *__restrict__ g2, double X[6]) {
+  const double kv[3] = {kc(i, N, kf), kc(j, N, kf), kc(l, N, kf)};
+  const double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+#pragma unroll
+  for (int q = 0; q < 6; q++) X[q] = 0.0;
+  if (k2 == 0.0) return;
+  const int a = nabs(i, N), b = nabs(j, N), c = nabs(l, N);
+  const int n2 = a * a + b * b + c * c;
+  const size_t e = ((size_t)i * N + j) * HP + l;
+  double dr = 0.0, di = 0.0;
+  // the 1LPT field is zero outside the initial-condition cube |n_a| <= Nsample/2
+  if (a <= Np / 2 && b <= Np / 2 && c <= Np / 2) {
+    const float2 d = cd1[e];
+    dr = d.x * g1[n2];
+    di = d.y * g1[n2];
+  }
+  if (cd2) {
+    const float2 d = cd2[e];
+    dr += d.x * g2[n2];
+    di += d.y * g2[n2];
+  }
+  const double inv = 1.0 / k2;
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    X[2 * q] = -kv[q] * inv * di;
+    X[2 * q + 1] = kv[q] * inv * dr;
+  }
+}
+
+// Sampling the c2r of a field at every m-th mesh point (the Lagrangian lattice
+// when N = m * Nsample, where the read-out of src/2LPT.c:1657-1702 has weight 1
+// on one node) equals an Nsample^3 c2r of the aliased spectrum
+//   F_f(k') = sum_{k = k' (mod Nsample)} F(k),
+// F the Hermitian extension of the half spectrum with the kz = 0, N/2 planes
+// symmetrised as the c2r's real-part rule implies.  One thread per output mode;
+// the folded DC mode is zeroed, which removes the lattice mean exactly
+// (src/2LPT.c:1719-1729).
+__global__ __launch_bounds__(kBlock) void k_lpt_fold(const float2 *__restrict__ cd1, const float2 *__restrict__ cd2,
+                                                     const double *__restrict__ g1, const double *__restrict__ g2,
+                                                     int N, int HP, int Np, int HPn, int m, double kf,
+                                                     float2 *__restrict__ o0, float2 *__restrict__ o1,
+                                                     float2 *__restrict__ o2) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)Np * Np * HPn) return;
+  const int lp = (int)(e % HPn);
+  const size_t row = e / HPn;
+  const int jp = (int)(row % Np), ip = (int)(row / Np);
+  double acc[6] = {0, 0, 0, 0, 0, 0}, X[6], Y[6];
+  if (lp <= Np / 2 && !(ip == 0 && jp == 0 && lp == 0)) {
+    for (int a = 0; a < m; a++) {
+      const int i = ip + a * Np, mi = i == 0 ? 0 : N - i;
+      for (int b = 0; b < m; b++) {
+        const int j = jp + b * Np, mj = j == 0 ? 0 : N - j;
+        for (int c = 0; c < m; c++) {
+          const int l = lp + c * Np;
+          if (l > 0 && l < N / 2) {
+            lpt_x(i, j, l, N, HP, Np, kf, cd1, cd2, g1, g2, X);
+#pragma unroll
+            for (int q = 0; q < 6; q++) acc[q] += X[q];
+          } else if (l > N / 2) {
+            lpt_x(mi, mj, N - l, N, HP, Np, kf, cd1, cd2, g1, g2, X);
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+              acc[2 * q] += X[2 * q];
+              acc[2 * q + 1] -= X[2 * q + 1];
+            }
+          } else {  // kz = 0 or N/2 plane: (X(k) + conj X(-k)) / 2
+            lpt_x(i, j, l, N, HP, Np, kf, cd1, cd2, g1, g2, X);
+            lpt_x(mi, mj, l, N, HP, Np, kf, cd1, cd2, g1, g2, Y);
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+              acc[2 * q] += 0.5 * (X[2 * q] + Y[2 * q]);
+              acc[2 * q + 1] += 0.5 * (X[2 * q + 1] - Y[2 * q + 1]);
+            }
+          }
+        }
+      }
+    }
+  }
+  o0[e] = make_float2((float)acc[0], (float)acc[1]);
+  o1[e] = make_float2((float)acc[2], (float)acc[3]);
+  o2[e] = make_float2((float)acc[4], (float)acc[5]);
+}
+
+// g_o(|n|^2) = norm_o num_o(ln k) / den_o(ln k) from cubic Hermite slices on a
+// uniform ln k grid; t = [num, dnum, den, dden] (4*ny doubles)
+struct KSl {
+  double y0, dy, norm;
+  int ny;
+};
+__device__ __forceinline__ double herm(const double *__restrict__ v, const double *__restrict__ dv, const KSl &s,
+                                       double y) {
+  const double ymax = s.y0 + s.dy * (s.ny - 1);
+  if (y < s.y0) y = s.y0;
+  if (y > ymax) y = ymax;
+  int j = (int)floor((y - s.y0) / s.dy);
+  if (j > s.ny - 2) j = s.ny - 2;
+  if (j < 0) j = 0;
+  const double h = s.dy, t = (y - (s.y0 + j * s.dy)) / h, t2 = t * t, t3 = t2 * t;
+  return (2 * t3 - 3 * t2 + 1) * v[j] + (t3 - 2 * t2 + t) * h * dv[j] + (-2 * t3 + 3 * t2) * v[j + 1] +
+         (t3 - t2) * h * dv[j + 1];
+}
+__global__ __launch_bounds__(kBlock) void k_growth_n2(size_t n2len, double kf, KSl s1, const double *__restrict__ t1,
+                                                      KSl s2, const double *__restrict__ t2, double *__restrict__ g1,
+                                                      double *__restrict__ g2) {
+  const size_t n2 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n2 >= n2len) return;
+  if (n2 == 0) {
+    g1[0] = g2[0] = 0.0;
+    return;
+  }
+  const double lk = log(kf * sqrt((double)n2));
+  g1[n2] = s1.norm * (herm(t1, t1 + s1.ny, s1, lk) / herm(t1 + 2 * s1.ny, t1 + 3 * s1.ny, s1, lk));
+  g2[n2] = s2.norm * (herm(t2, t2 + s2.ny, s2, lk) / herm(t2 + 2 * s2.ny, t2 + 3 * s2.ny, s2, lk));
+}
+
 // seed table on the host (a single serial stream), src/2LPT.c:259-271
 void seedtable_host(int N, unsigned long seed, std::vector<unsigned> &t) {
   Ranlxd1 r;
@@ -361,12 +482,27 @@ int ic_init_particles(mgpu_ctx *ctx, double Di, double Di2) {
   return MGPU_OK;
 }
 
-int ic_lpt_fields(mgpu_ctx *ctx, int slot, const double *g1_n2, const double *g2_n2) {
-  PhaseTimer pt(ctx, &ctx->t.lpt_fields);
-  if (!ctx->p.scale_dependent) return mgpu_fail(ctx, MGPU_E_STATE, "LPT fields need a scale-dependent context");
+// LPT fields from the device growth tables n2tab[0], n2tab[1]
+static int lpt_fields_from_tables(mgpu_ctx *ctx, int slot) {
   int rc;
-  if ((rc = upload_n2(ctx, 0, g1_n2))) return rc;
-  if ((rc = upload_n2(ctx, 1, g2_n2))) return rc;
+  if (ctx->lattice_slot) {
+    // exact lattice fold: N = m * Nsample
+    const int Np = ctx->Np, HPn = ctx->fft_lattice.hp, m = ctx->N / Np;
+    const size_t nl = (size_t)Np * Np * HPn;
+    float2 *o[3] = {ctx->lattice_slot, ctx->lattice_slot + nl, ctx->lattice_slot + 2 * nl};
+    {
+      // compulsory: both stored spectra read once (the 1LPT one inside the IC cube only), 3 folded outputs written
+      const double modes = (double)ctx->N * ctx->N * (ctx->N / 2 + 1);
+      KScope ks(ctx, MGPU_K_LPT, 8.0 * modes * (1.0 + 1.0 / (double)(m * m * m)) + 3.0 * 8.0 * nl);
+      hipLaunchKernelGGL(k_lpt_fold, dim3(cdiv(nl, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->cdelta[0],
+                         ctx->cdelta[1], ctx->n2tab[0], ctx->n2tab[1], ctx->N, ctx->HP, Np, HPn, m,
+                         2 * MGPU_PI / ctx->p.box, o[0], o[1], o[2]);
+    }
+    MGPU_HIP(ctx, hipGetLastError());
+    for (int a = 0; a < 3; a++)
+      if ((rc = fft_c2r_3d(ctx, &ctx->fft_lattice, o[a], ctx->lpt[slot] + (size_t)a * ctx->np))) return rc;
+    return MGPU_OK;
+  }
   for (int a = 0; a < 3; a++) {
     if ((rc = spectrum(ctx, SPEC_FIELD, ctx->cdelta[0], ctx->cdelta[1], ctx->slot[3], a, 0, ctx->n2tab[0],
                        ctx->n2tab[1])))
@@ -374,6 +510,50 @@ int ic_lpt_fields(mgpu_ctx *ctx, int slot, const double *g1_n2, const double *g2
     if ((rc = fft_c2r_3d(ctx, &ctx->fft, ctx->slot[3]))) return rc;
     if ((rc = readout(ctx, ctx->slot[3], 1.0, ctx->lpt[slot] + (size_t)a * ctx->np))) return rc;
   }
+  return MGPU_OK;
+}
+
+int ic_lpt_fields_slices(mgpu_ctx *ctx, int slot, const mgpu_kslice *o1, const mgpu_kslice *o2) {
+  PhaseTimer pt(ctx, &ctx->t.lpt_fields);
+  if (!ctx->p.scale_dependent) return mgpu_fail(ctx, MGPU_E_STATE, "LPT fields need a scale-dependent context");
+  if (o1->nk < 2 || o2->nk < 2 || o1->nk > 8192 || o2->nk > 8192 || o1->dlnk <= 0 || o2->dlnk <= 0)
+    return mgpu_fail(ctx, MGPU_E_INVALID, "bad growth slice");
+  std::vector<double> h(4 * (size_t)(o1->nk + o2->nk));
+  double *p = h.data();
+  for (const mgpu_kslice *s : {o1, o2}) {
+    for (const double *src : {s->num, s->dnum, s->den, s->dden}) {
+      std::copy(src, src + s->nk, p);
+      p += s->nk;
+    }
+  }
+  if (!ctx->slice_buf || ctx->slice_len < h.size()) {
+    if (ctx->slice_buf) MGPU_HIP(ctx, hipFree(ctx->slice_buf));
+    ctx->slice_buf = nullptr;
+    MGPU_HIP(ctx, hipMalloc(&ctx->slice_buf, h.size() * sizeof(double)));
+    ctx->slice_len = h.size();
+  }
+  MGPU_HIP(ctx, hipMemcpyAsync(ctx->slice_buf, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
+                               ctx->stream));
+  KSl k1{o1->lnk0, o1->dlnk, o1->norm, o1->nk}, k2{o2->lnk0, o2->dlnk, o2->norm, o2->nk};
+  hipLaunchKernelGGL(k_growth_n2, dim3(cdiv(ctx->n2len, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->n2len,
+                     2 * MGPU_PI / ctx->p.box, k1, (const double *)ctx->slice_buf, k2,
+                     (const double *)ctx->slice_buf + 4 * o1->nk, ctx->n2tab[0], ctx->n2tab[1]);
+  MGPU_HIP(ctx, hipGetLastError());
+  int rc = lpt_fields_from_tables(ctx, slot);
+  if (rc) return rc;
+  // the host buffer h must outlive the async copy
   MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return MGPU_OK;
 }
+
+int ic_lpt_fields(mgpu_ctx *ctx, int slot, const double *g1_n2, const double *g2_n2) {
+  PhaseTimer pt(ctx, &ctx->t.lpt_fields);
+  if (!ctx->p.scale_dependent) return mgpu_fail(ctx, MGPU_E_STATE, "LPT fields need a scale-dependent context");
+  int rc;
+  if ((rc = upload_n2(ctx, 0, g1_n2))) return rc;
+  if ((rc = upload_n2(ctx, 1, g2_n2))) return rc;
+  if ((rc = lpt_fields_from_tables(ctx, slot))) return rc;
+  MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MGPU_OK;
+}
+

@@ -57,7 +57,9 @@ struct mgpu_ctx {
 
   FFTPlan fft;
   FFTPlan fft_lattice;  // Nsample-sized plan for folded LPT fields (when N % Ns == 0)
-  float2 *lattice_slot = nullptr;
+  float2 *lattice_slot = nullptr;  // 3 Nsample^3 half spectra
+  double *slice_buf = nullptr;     // growth slices uploaded by mgpu_lpt_fields_slices
+  size_t slice_len = 0;
 
   // state
   bool have_deposit = false, have_density = false, have_forces = false, have_disp = false;
@@ -114,9 +116,10 @@ void fft_plan_free(FFTPlan *plan);
 bool fft_size_supported(int n);
 // in-place 3D transforms on a slot with the padded layout above
 int fft_r2c_3d(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf);
-int fft_c2r_3d(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf);
+// c2r; with out != nullptr the real result goes to a compact [N][N][N] array
+int fft_c2r_3d(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf, float *out = nullptr);
 // individual passes (used by fused paths and tests)
-int fft_pass_z(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf, int forward);
+int fft_pass_z(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf, int forward, float *out = nullptr);
 int fft_pass_col(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf, int axis, int forward);
 
 // ---------------------------------------------------------------- PM kernels (pm.hip)
@@ -134,5 +137,6 @@ int ic_displacement_fields(mgpu_ctx *ctx, unsigned long seed, int sphere_mode, i
                            const double *pk_n2);
 int ic_init_particles(mgpu_ctx *ctx, double Di, double Di2);
 int ic_lpt_fields(mgpu_ctx *ctx, int slot, const double *g1_n2, const double *g2_n2);
+int ic_lpt_fields_slices(mgpu_ctx *ctx, int slot, const mgpu_kslice *o1, const mgpu_kslice *o2);
 
 static inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }

@@ -306,6 +306,43 @@ void mgp_lpt_tables(const mgp_cosmo *c, int field, double A, double AFF, double 
   free(buf);
 }
 
+/* The same growth as cubic Hermite slices in ln k for evaluation on the GPU
+ * (mgpu_lpt_fields_slices): buf holds 8 * ny doubles. */
+int mgp_lpt_kslices(const mgp_cosmo *c, int field, double A, double AFF, mgpu_kslice *o1, mgpu_kslice *o2,
+                    double *buf) {
+  if (!c->have_sd) return -1;
+  const int ny = c->sd[0].ny;
+  const int num1 = field == MGP_FIELD_D || field == MGP_FIELD_deltaD ? MGP_G_D
+                   : field == MGP_FIELD_dDdy                        ? MGP_G_dDdy
+                                                                    : MGP_G_ddDddy;
+  const int num2 = num1 + 3;
+  double *n1v = buf, *n1d = buf + ny, *d1v = buf + 2 * ny, *d1d = buf + 3 * ny;
+  double *n2v = buf + 4 * ny, *n2d = buf + 5 * ny, *d2v = buf + 6 * ny, *d2d = buf + 7 * ny;
+  mgp_spline2d_slice(&c->sd[num1], log(A), n1v, n1d);
+  mgp_spline2d_slice(&c->sd[num2], log(A), n2v, n2d);
+  if (field == MGP_FIELD_deltaD) {
+    /* D(AFF) - D(A): the slices are linear in the tabulated values */
+    double *tv = malloc(sizeof(double) * 2 * ny), *td = tv + ny;
+    mgp_spline2d_slice(&c->sd[num1], log(AFF), tv, td);
+    for (int j = 0; j < ny; j++) {
+      n1v[j] = tv[j] - n1v[j];
+      n1d[j] = td[j] - n1d[j];
+    }
+    mgp_spline2d_slice(&c->sd[num2], log(AFF), tv, td);
+    for (int j = 0; j < ny; j++) {
+      n2v[j] = tv[j] - n2v[j];
+      n2d[j] = td[j] - n2d[j];
+    }
+    free(tv);
+  }
+  mgp_spline2d_slice(&c->sd[MGP_G_D], 0.0, d1v, d1d);
+  mgp_spline2d_slice(&c->sd[MGP_G_D2], 0.0, d2v, d2d);
+  const int N = c->p->Nmesh;
+  *o1 = (mgpu_kslice){c->sd_lnkmin, c->sd_dlnk, ny, n1v, n1d, d1v, d1d, 1.0};
+  *o2 = (mgpu_kslice){c->sd_lnkmin, c->sd_dlnk, ny, n2v, n2d, d2v, d2d, -3.0 / 7.0 / ((double)N * N * N)};
+  return 0;
+}
+
 /* ------------------------------------------------------------------ object */
 mgp_cosmo *mgp_cosmo_create(mgp_params *p, char *err, size_t errlen) {
   struct mgp_cosmo *c = calloc(1, sizeof(*c));

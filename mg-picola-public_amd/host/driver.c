@@ -63,9 +63,15 @@ static void adjust_pofk(const mgp_params *p, int *nbins, int *bintype, int *sub,
   if (*kmax > sqrt(3.0) * (double)p->Nmesh) *kmax = (double)p->Nmesh;
 }
 
+/* assign_displacment_field_to_particles for both LPT orders at once
+ * (src/2LPT.c:1758-2005): the growth goes to the GPU as ln k slices */
 static int lpt_fields(sim *s, int slot, int field, double A, double AFF) {
-  mgp_lpt_tables(s->c, field, A, AFF, s->g1, s->g2);
-  GPU(mgpu_lpt_fields(s->g, slot, s->g1, s->g2));
+  mgpu_kslice o1, o2;
+  if (mgp_lpt_kslices(s->c, field, A, AFF, &o1, &o2, s->g1)) {
+    snprintf(s->err, s->errlen, "no scale-dependent growth tables");
+    return -1;
+  }
+  GPU(mgpu_lpt_fields_slices(s->g, slot, &o1, &o2));
   return 0;
 }
 
@@ -208,8 +214,9 @@ mgp_sim *mgp_sim_create(mgp_params *p, int verbose, char *err, size_t errlen) {
   s->pk = calloc(s->nbins, sizeof(double));
   s->pkk = calloc(s->nbins, sizeof(double));
   s->pkn = calloc(s->nbins, sizeof(double));
+  /* g1 also holds the 8 growth slices of 1000 ln k nodes (mgp_lpt_kslices) */
   const size_t n2len = mgpu_n2_table_len(p->Nmesh);
-  s->g1 = malloc(sizeof(double) * n2len);
+  s->g1 = malloc(sizeof(double) * (n2len > 8000 ? n2len : 8000));
   s->g2 = malloc(sizeof(double) * n2len);
   return s;
 fail:

@@ -107,6 +107,9 @@ void mgp_ic_pk_table(const mgp_cosmo *c, double *pk_n2);
 /* scale-dependent LPT field tables: g_o(|n|^2) including normfactor for
  * field = MGP_FIELD_*, orders 1 and 2 */
 void mgp_lpt_tables(const mgp_cosmo *c, int field, double A, double AFF, double *g1_n2, double *g2_n2);
+/* the same as ln k slices for mgpu_lpt_fields_slices; buf: 8 * nk doubles (nk = 1000) */
+int mgp_lpt_kslices(const mgp_cosmo *c, int field, double A, double AFF, mgpu_kslice *o1, mgpu_kslice *o2,
+                    double *buf);
 
 enum { MGP_G_D = 0, MGP_G_dDdy = 1, MGP_G_ddDddy = 2, MGP_G_D2 = 3, MGP_G_dD2dy = 4, MGP_G_ddD2ddy = 5 };
 enum { MGP_FIELD_D = 0, MGP_FIELD_dDdy = 1, MGP_FIELD_ddDddy = 2, MGP_FIELD_deltaD = 3 };

@@ -24,7 +24,7 @@ MGPU_SYMBOLS = [
     "mgpu_pm_density", "mgpu_pm_deposit", "mgpu_pm_transform", "mgpu_power_spectrum", "mgpu_fifth_force", "mgpu_forces", "mgpu_mesh_to_particles",
     "mgpu_get_displacements", "mgpu_kick", "mgpu_drift", "mgpu_ic_displacement_fields", "mgpu_init_particles",
     "mgpu_lpt_fields", "mgpu_get_mesh", "mgpu_set_mesh", "mgpu_get_timers", "mgpu_enable_timers",
-    "mgpu_kernel_name", "mgpu_enable_kernel_timing", "mgpu_kernel_timing",
+    "mgpu_kernel_name", "mgpu_enable_kernel_timing", "mgpu_kernel_timing", "mgpu_lpt_fields_slices",
 ]
 
 
@@ -545,3 +545,27 @@ class Sim:
             self.close()
         except Exception:
             pass
+
+
+class KSlice(C.Structure):
+    _fields_ = [("lnk0", C.c_double), ("dlnk", C.c_double), ("nk", C.c_int),
+                ("num", C.POINTER(C.c_double)), ("dnum", C.POINTER(C.c_double)),
+                ("den", C.POINTER(C.c_double)), ("dden", C.POINTER(C.c_double)), ("norm", C.c_double)]
+
+
+def lpt_kslices(cosmo, field, A, AFF):
+    """(KSlice order 1, KSlice order 2, backing buffer) from the host's scale-dependent tables."""
+    L = hlib()
+    L.mgp_lpt_kslices.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.POINTER(KSlice),
+                                  C.POINTER(KSlice), C.c_void_p]
+    buf = np.zeros(8 * 1000)
+    o1, o2 = KSlice(), KSlice()
+    if L.mgp_lpt_kslices(cosmo.h, field, A, AFF, C.byref(o1), C.byref(o2), buf.ctypes.data):
+        raise MGPUError("no scale-dependent growth tables")
+    return o1, o2, buf
+
+
+def lpt_fields_slices(g, slot, o1, o2):
+    L = lib()
+    L.mgpu_lpt_fields_slices.argtypes = [C.c_void_p, C.c_int, C.POINTER(KSlice), C.POINTER(KSlice)]
+    g._ck(L.mgpu_lpt_fields_slices(g.ctx, slot, C.byref(o1), C.byref(o2)), "lpt_fields_slices")

@@ -35,9 +35,20 @@ def oracle_run(hp, cosmo, nsteps):
     Di, Di2 = cosmo.growth(M.G_D, A), cosmo.growth(M.G_D2, A)
     cd = O.ic_delta(c, hp.Seed, cosmo.ic_pk_table(), hp.SphereMode, hp.amplitude_fixed_initial_condition,
                     hp.inverted_initial_condition)
-    za, lp = O.ic_displacements(c, cd, 0)
+    sd = bool(hp.scale_dependent)
     q = (np.indices((Ns, Ns, Ns)).reshape(3, -1).astype(np.float64)) * (box / Ns)
-    pos = wrapf((q + za.astype(np.float64) * Di + lp.astype(np.float64) * Di2).astype(np.float32), box)
+    if sd:
+        (cd2,) = O.ic_displacements(c, cd, 1)
+
+        def fields(field, a0, a1):
+            g1, g2 = cosmo.lpt_tables(field, a0, a1)
+            return O.lpt_field(c, cd, g1), O.lpt_field(c, cd2, g2)
+
+        za, lp = fields(M.FIELD_D, A, A)
+        pos = wrapf((q + za.astype(np.float64) + lp.astype(np.float64)).astype(np.float32), box)
+    else:
+        za, lp = O.ic_displacements(c, cd, 0)
+        pos = wrapf((q + za.astype(np.float64) * Di + lp.astype(np.float64) * Di2).astype(np.float32), box)
     vel = np.zeros_like(pos)
     AI = AF = A
     da = (1.0 - A) / nsteps
@@ -57,12 +68,21 @@ def oracle_run(hp, cosmo, nsteps):
         f = O.forces(c, d, phik)
         disp, sD = O.gather(c, pos, f)
         dda = cosmo.Sphi(AI, AF, A)
-        sV = O.kick(vel, disp, za, lp, cosmo.growth(M.G_ddDddy, A), cosmo.growth(M.G_ddD2ddy, A), 0, hp.Omega,
-                    dda, A, 1, sD)
+        if sd:
+            dd1, dd2 = fields(M.FIELD_deltaD, A, AFF)
+            k1, k2 = fields(M.FIELD_ddDddy, A, AFF)
+            sV = O.kick(vel, disp, k1, k2, 0, 0, 1, hp.Omega, dda, A, 1, sD)
+        else:
+            sV = O.kick(vel, disp, za, lp, cosmo.growth(M.G_ddDddy, A), cosmo.growth(M.G_ddD2ddy, A), 0, hp.Omega,
+                        dda, A, 1, sD)
         if final:
             break
         dyyy = cosmo.Sq(A, AFF, AF)
-        O.drift(pos, vel, za, lp, cosmo.growth(M.G_D, AFF) - Di, cosmo.growth(M.G_D2, AFF) - Di2, 0, box, dyyy, 1, sV)
+        if sd:
+            O.drift(pos, vel, dd1, dd2, 0, 0, 1, box, dyyy, 1, sV)
+        else:
+            O.drift(pos, vel, za, lp, cosmo.growth(M.G_D, AFF) - Di, cosmo.growth(M.G_D2, AFF) - Di2, 0, box, dyyy,
+                    1, sV)
         A, AI = AFF, AF
         Di, Di2 = cosmo.growth(M.G_D, A), cosmo.growth(M.G_D2, A)
     return pos, vel, sV

@@ -32,6 +32,8 @@ def make_params(tmp_path, name, nsteps, **over):
     dict(name="lcdm_64_128", nsteps=10, over=dict()),
     dict(name="lcdm_64_128", nsteps=5, over=dict(Nmesh=48, Nsample=24, Box=96.0, modified_gravity_active=1,
                                                  include_screening=1)),
+    # f(R) with scale-dependent growth and screening (the benchmark's path) at 32^3 / 64^3
+    dict(name="fofr_sd_512_1024", nsteps=6, over=dict(Nmesh=64, Nsample=32, Box=64.0, pofk_nbins=32)),
 ])
 def test_end_to_end_matches_oracle(tmp_path, case):
     hp = make_params(tmp_path, case["name"], case["nsteps"], **case["over"])
@@ -49,12 +51,19 @@ def test_end_to_end_matches_oracle(tmp_path, case):
     assert d.max() <= 1e-4 * hp.Box
     # output files of the reference's formats were written
     files = os.listdir(str(tmp_path))
-    assert any(f.startswith("lcdm_64_128_z0p000") for f in files), files
-    assert any(f.startswith("info_lcdm_64_128") for f in files), files
+    assert any(f.startswith(case["name"] + "_z0p000") for f in files), files
+    assert any(f.startswith("info_" + case["name"]) for f in files), files
+    if hp.pofk_compute_every_step:
+        assert sum(f.startswith("pofk_" + case["name"]) for f in files) == case["nsteps"] + 1, files
 
 
-@pytest.mark.parametrize("n,ns,sd", [(32, 16, 0), (32, 32, 0), (24, 12, 1), (48, 24, 1)])
+@pytest.mark.parametrize("n,ns,sd", [(32, 16, 0), (32, 32, 0), (24, 12, 1), (48, 24, 1), (24, 24, 1), (48, 16, 1),
+                                     (32, 12, 1)])
 def test_ic_matches_oracle(n, ns, sd):
+    """ZA/2LPT displacements (non scale-dependent) and the stored spectra plus
+    both-order LPT fields (scale-dependent) against the oracle; the last three
+    cases cover the lattice fold with m = 1 and m = 3 and the full-mesh
+    read-out (N not a multiple of Nsample)."""
     box = 2.0 * n
     rng = np.random.default_rng(n)
     kf = 2 * np.pi / box
@@ -78,3 +87,24 @@ def test_ic_matches_oracle(n, ns, sd):
         za, lp = O.ic_displacements(c, cd, 0)
         assert rel_err(g.get_lpt(0), za) < 1e-5
         assert rel_err(g.get_lpt(1), lp) < 2e-5
+
+
+def test_lpt_slices_match_host_tables(tmp_path):
+    """Growth evaluated on the GPU from ln k slices equals the host's per-|n|^2
+    tables (both restate the bicubic lookup of src/cosmo.c:987-1010)."""
+    hp = make_params(tmp_path, "lcdm_64_128", 4, Nmesh=32, Nsample=16, Box=64.0, modified_gravity_active=1,
+                     include_screening=1)
+    hp.scale_dependent = 1
+    cosmo = M.Cosmo(hp)
+    g = M.MGPU(32, 16, 64.0, 0.3, model=M.FOFR, mg_active=1, include_screening=1, scale_dependent=1)
+    rng = np.random.default_rng(3)
+    pk = np.where(np.arange(M.n2_table_len(32)) > 0, 1e4, 0.0)
+    g.ic_displacement_fields(5001, pk)
+    for field, A, AFF in ((M.FIELD_deltaD, 0.2, 0.25), (M.FIELD_ddDddy, 0.5, 0.55), (M.FIELD_D, 0.05, 0.05)):
+        g1, g2 = cosmo.lpt_tables(field, A, AFF)
+        g.lpt_fields(0, g1, g2)
+        ref = g.get_lpt(0)
+        o1, o2, buf = M.lpt_kslices(cosmo, field, A, AFF)
+        M.lpt_fields_slices(g, 1, o1, o2)
+        assert rel_err(g.get_lpt(1), ref) < 1e-6
+    del rng
