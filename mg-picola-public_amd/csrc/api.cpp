@@ -182,6 +182,8 @@ int mgpu_create(const mgpu_params *p, mgpu_ctx **out) {
   if ((rc = alloc(ctx, (void **)&force, 3 * sb, "force meshes"))) return fail(rc);
   for (int a = 0; a < 3; a++) ctx->slot[3 + a] = force + a * ctx->slot_elems;
   ctx->acc = (unsigned long long *)force;  // needs 8 N^3 bytes <= 2 slots
+  ctx->dep_scratch_bytes = deposit_scratch_bytes(ctx->N, ctx->np);
+  if ((rc = alloc(ctx, &ctx->dep_scratch, ctx->dep_scratch_bytes, "deposit scratch"))) return fail(rc);
   if (p->scale_dependent) {
     for (int s = 0; s < 2; s++)
       if ((rc = alloc(ctx, (void **)&ctx->cdelta[s], sb, "stored delta(k)"))) return fail(rc);
@@ -220,7 +222,7 @@ void mgpu_destroy(mgpu_ctx *ctx) {
   void *ptrs[] = {ctx->pos,      ctx->vel,       ctx->disp,      ctx->lpt[0],    ctx->lpt[1],
                   ctx->slot[0],  ctx->slot[1],   ctx->slot[2],   ctx->slot[3],   ctx->cdelta[0],
                   ctx->cdelta[1], ctx->partial,  ctx->dsum,      ctx->pk,        ctx->n2tab[0],
-                  ctx->n2tab[1], ctx->lattice_slot, ctx->slice_buf};
+                  ctx->n2tab[1], ctx->lattice_slot, ctx->slice_buf, ctx->dep_scratch};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   fft_plan_free(&ctx->fft);

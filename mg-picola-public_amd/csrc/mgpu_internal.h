@@ -60,6 +60,8 @@ struct mgpu_ctx {
   float2 *lattice_slot = nullptr;  // 3 Nsample^3 half spectra
   double *slice_buf = nullptr;     // growth slices uploaded by mgpu_lpt_fields_slices
   size_t slice_len = 0;
+  void *dep_scratch = nullptr;     // tiled deposit: tile ids, ranks, sorted indices, halo faces
+  size_t dep_scratch_bytes = 0;
 
   // state
   bool have_deposit = false, have_density = false, have_forces = false, have_disp = false;
@@ -121,6 +123,10 @@ int fft_c2r_3d(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf, float *out = nul
 // individual passes (used by fused paths and tests)
 int fft_pass_z(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf, int forward, float *out = nullptr);
 int fft_pass_col(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf, int axis, int forward);
+
+// ---------------------------------------------------------------- tiled deposit (deposit.hip)
+size_t deposit_scratch_bytes(int N, size_t np);
+int deposit_tiled(mgpu_ctx *ctx);  // particles -> 64-bit fixed-point accumulator ctx->acc
 
 // ---------------------------------------------------------------- PM kernels (pm.hip)
 int pm_deposit(mgpu_ctx *ctx);                         // particles -> acc -> delta(x) in slot0 (+ copy slot1)

@@ -9,8 +9,7 @@
 
 namespace {
 
-constexpr double kFix = 4294967296.0;        // 2^32: fixed-point scale of the deposit
-constexpr double kInvFix = 1.0 / 4294967296.0;
+constexpr double kInvFix = 1.0 / 4294967296.0;  // 2^-32: fixed-point scale of the deposit (deposit.hip)
 constexpr int kBlock = 256;
 
 // Cell index and fractional offset of a scaled coordinate X = pos * N / Box,
@@ -25,40 +24,6 @@ __device__ __forceinline__ void cell_of(double X, int N, unsigned &I, double &D)
   I = (unsigned)X;
   D = X - (double)I;
   if (I >= (unsigned)N) I = 0;
-}
-
-// ------------------------------------------------------------------ deposit
-// PtoMesh CIC scatter (src/auxPM.c:295-343): weights in FP64 from the float
-// positions, WPAR folded into the y weights, accumulated as unsigned 64-bit
-// fixed point (2^-32 resolution) so the sum is independent of the order the
-// atomics land in.  x periodicity replaces the ghost slab of src/auxPM.c:350-356.
-__global__ __launch_bounds__(kBlock) void k_deposit(const float *__restrict__ pos, size_t np, int N, int HP2,
-                                                    double scale, double wpar, unsigned long long *__restrict__ acc) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
-  unsigned IX, IY, IZ;
-  double DX, DY, DZ;
-  cell_of((double)pos[i] * scale, N, IX, DX);
-  cell_of((double)pos[np + i] * scale, N, IY, DY);
-  cell_of((double)pos[2 * np + i] * scale, N, IZ, DZ);
-  double TX = 1.0 - DX, TY = 1.0 - DY, TZ = 1.0 - DZ;
-  DY *= wpar;
-  TY *= wpar;
-  unsigned IXn = IX + 1 >= (unsigned)N ? 0 : IX + 1;
-  unsigned IYn = IY + 1 >= (unsigned)N ? 0 : IY + 1;
-  unsigned IZn = IZ + 1 >= (unsigned)N ? 0 : IZ + 1;
-  const size_t r00 = ((size_t)IX * N + IY) * N, r01 = ((size_t)IX * N + IYn) * N;
-  const size_t r10 = ((size_t)IXn * N + IY) * N, r11 = ((size_t)IXn * N + IYn) * N;
-#define DEP(idx, w) atomicAdd(acc + (idx), (unsigned long long)__double2ll_rn((w) * kFix))
-  DEP(r00 + IZ, TX * TY * TZ);
-  DEP(r00 + IZn, TX * TY * DZ);
-  DEP(r01 + IZ, TX * DY * TZ);
-  DEP(r01 + IZn, TX * DY * DZ);
-  DEP(r10 + IZ, DX * TY * TZ);
-  DEP(r10 + IZn, DX * TY * DZ);
-  DEP(r11 + IZ, DX * DY * TZ);
-  DEP(r11 + IZn, DX * DY * DZ);
-#undef DEP
 }
 
 // density = -1 + sum of weights (src/auxPM.c:292-293), written as FP32 into
@@ -423,16 +388,8 @@ int pm_deposit(mgpu_ctx *ctx) {
   PhaseTimer pt(ctx, &ctx->t.ptomesh);
   const int N = ctx->N;
   const size_t cells = (size_t)N * N * N;
-  MGPU_HIP(ctx, hipMemsetAsync(ctx->acc, 0, cells * sizeof(unsigned long long), ctx->stream));
-  const double scale = (double)N / ctx->p.box;
-  const double wpar = std::pow((double)N / (double)ctx->Np, 3);
-  if (ctx->np) {
-    // compulsory: positions read once, the accumulator mesh written once
-    KScope ks(ctx, MGPU_K_DEPOSIT, 12.0 * ctx->np + 8.0 * cells);
-    hipLaunchKernelGGL(k_deposit, dim3(cdiv(ctx->np, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->pos, ctx->np, N,
-                       2 * ctx->HP, scale, wpar, ctx->acc);
-    MGPU_HIP(ctx, hipGetLastError());
-  }
+  int rc = deposit_tiled(ctx);
+  if (rc) return rc;
   const bool need_dx = ctx->p.mg_active && ctx->p.model != MGPU_BRANSDICKE && ctx->p.model != MGPU_LCDM;
   const size_t total = (size_t)N * N * 2 * ctx->HP;
   {
