@@ -71,7 +71,7 @@ def gpu_bench(steps, warmup, workload):
     return dict(elapsed=elapsed, npart=npart, nmesh=hp.Nmesh, box=hp.Box, kernels=kt, t_tables=t_tables, t_ic=t_ic)
 
 
-def cpu_baseline(ns=128, nmesh=256, box=256.0):
+def cpu_baseline(ns=256, nmesh=512, box=512.0):
     """One COLA step of the same f(R) + screening + scale-dependent path in the
     FP64 oracle: deposit, r2c, P(k), fifth force (2 FFTs), Forces (3 c2r),
     gather, 4 LPT fields (3 c2r each), Kick, Drift."""

@@ -193,9 +193,12 @@ int mgpu_enable_timers(mgpu_ctx *ctx, int on);
  * events on the context's stream; mgpu_kernel_timing synchronises and reports
  * launches, summed device time and summed algorithmic bytes (compulsory HBM
  * reads + writes) since the last reset. */
+/* FFT classes: z-pass r2c / c2r, column (y and x) passes forward / inverse of
+ * the Nmesh transforms; the Nsample-lattice transforms count as MGPU_K_LPT. */
 enum {
-  MGPU_K_DEPOSIT = 0, MGPU_K_DENSITY, MGPU_K_FFT_Z, MGPU_K_FFT_Y, MGPU_K_FFT_X, MGPU_K_POFK, MGPU_K_SPECTRAL,
-  MGPU_K_REAL, MGPU_K_GATHER, MGPU_K_KICK, MGPU_K_DRIFT, MGPU_K_LPT, MGPU_K_READOUT, MGPU_K_COUNT
+  MGPU_K_DEPOSIT = 0, MGPU_K_DENSITY, MGPU_K_FFT_R2C, MGPU_K_FFT_C2R, MGPU_K_FFT_COL_FWD, MGPU_K_FFT_COL_INV,
+  MGPU_K_POFK, MGPU_K_SPECTRAL, MGPU_K_REAL, MGPU_K_GATHER, MGPU_K_KICK, MGPU_K_DRIFT, MGPU_K_LPT, MGPU_K_READOUT,
+  MGPU_K_COUNT
 };
 const char *mgpu_kernel_name(int kernel);
 int mgpu_enable_kernel_timing(mgpu_ctx *ctx, int on);

@@ -84,8 +84,9 @@ static int flush_ktimers(mgpu_ctx *ctx) {
 extern "C" {
 
 const char *mgpu_kernel_name(int k) {
-  static const char *names[MGPU_K_COUNT] = {"deposit", "density", "fft_z",   "fft_y",  "fft_x",   "pofk",   "spectral",
-                                            "real",    "gather",  "kick",    "drift",  "lpt",     "readout"};
+  static const char *names[MGPU_K_COUNT] = {"deposit", "density", "fft_r2c", "fft_c2r", "fft_col_fwd", "fft_col_inv",
+                                            "pofk",    "spectral", "real",   "gather",  "kick",        "drift",
+                                            "lpt",     "readout"};
   return (k >= 0 && k < MGPU_K_COUNT) ? names[k] : "?";
 }
 
@@ -166,7 +167,7 @@ int mgpu_create(const mgpu_params *p, mgpu_ctx **out) {
   if (hipGetDeviceProperties(&pr, ctx->device) == hipSuccess) ctx->lds_max = pr.sharedMemPerBlock;
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(mgpu_fail(ctx, MGPU_E_HIP, "stream creation failed"));
-  if ((rc = fft_plan_init(ctx, &ctx->fft, ctx->N))) return fail(rc);
+  if ((rc = fft_plan_init(ctx, &ctx->fft, ctx->N, MGPU_FFT_TCOLS))) return fail(rc);
   ctx->HP = ctx->fft.hp;
   ctx->slot_elems = (size_t)ctx->N * ctx->N * ctx->HP;
   const size_t pb = 3 * ctx->np * sizeof(float);
@@ -189,7 +190,8 @@ int mgpu_create(const mgpu_params *p, mgpu_ctx **out) {
       if ((rc = alloc(ctx, (void **)&ctx->cdelta[s], sb, "stored delta(k)"))) return fail(rc);
     // LPT fields by the exact lattice fold when the mesh is a multiple of the lattice
     if (ctx->N % ctx->Np == 0 && fft_size_supported(ctx->Np)) {
-      if ((rc = fft_plan_init(ctx, &ctx->fft_lattice, ctx->Np))) return fail(rc);
+      if ((rc = fft_plan_init(ctx, &ctx->fft_lattice, ctx->Np, MGPU_FFT_TCOLS))) return fail(rc);
+      ctx->fft_lattice.kcat = MGPU_K_LPT;
       const size_t nl = (size_t)ctx->Np * ctx->Np * ctx->fft_lattice.hp;
       if ((rc = alloc(ctx, (void **)&ctx->lattice_slot, 3 * nl * sizeof(float2), "lattice spectra"))) return fail(rc);
     }

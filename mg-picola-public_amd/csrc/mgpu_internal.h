@@ -19,11 +19,16 @@
 
 #define MGPU_PI 3.14159265358979323846
 #define MGPU_INVERSE_H0_MPCH 2997.92458
+// columns per tile of the strided FFT passes (16: 128-byte row segments, 8: 64-byte)
+#ifndef MGPU_FFT_TCOLS
+#define MGPU_FFT_TCOLS 16
+#endif
 
 struct FFTPlan {
   int n = 0;       // transform size per dimension
   int hp = 0;      // padded complex row length
   int tcols = 16;  // columns per tile in the strided passes
+  int kcat = -1;   // kernel-timing category of all passes (-1: by pass kind)
   float2 *tw = nullptr;  // W_N^m = exp(-2 pi i m / N), m < N
 };
 
@@ -113,7 +118,7 @@ struct PhaseTimer {
 };
 
 // ---------------------------------------------------------------- FFT (fft.hip)
-int fft_plan_init(mgpu_ctx *ctx, FFTPlan *plan, int n);
+int fft_plan_init(mgpu_ctx *ctx, FFTPlan *plan, int n, int tcols = 16);
 void fft_plan_free(FFTPlan *plan);
 bool fft_size_supported(int n);
 // in-place 3D transforms on a slot with the padded layout above

@@ -449,8 +449,8 @@ def run(hp, verbose=False):
     return {"step_seconds": st.step_seconds, "steps": st.steps, "final_A": st.final_A}
 
 
-KERNELS = ["deposit", "density", "fft_z", "fft_y", "fft_x", "pofk", "spectral", "real", "gather", "kick", "drift",
-           "lpt", "readout"]
+KERNELS = ["deposit", "density", "fft_r2c", "fft_c2r", "fft_col_fwd", "fft_col_inv", "pofk", "spectral", "real",
+           "gather", "kick", "drift", "lpt", "readout"]
 
 
 def _kt_setup():
