@@ -194,11 +194,13 @@ int mgpu_enable_timers(mgpu_ctx *ctx, int on);
  * launches, summed device time and summed algorithmic bytes (compulsory HBM
  * reads + writes) since the last reset. */
 /* FFT classes: z-pass r2c / c2r, column (y and x) passes forward / inverse of
- * the Nmesh transforms; the Nsample-lattice transforms count as MGPU_K_LPT. */
+ * the Nmesh transforms (the Nsample-lattice transforms count as MGPU_K_LPT),
+ * and the FFT passes with fused multipliers: Green's x pass, dual y pass,
+ * z screening pass, x pass with the MG epilogue. */
 enum {
   MGPU_K_DEPOSIT = 0, MGPU_K_DENSITY, MGPU_K_FFT_R2C, MGPU_K_FFT_C2R, MGPU_K_FFT_COL_FWD, MGPU_K_FFT_COL_INV,
   MGPU_K_POFK, MGPU_K_SPECTRAL, MGPU_K_REAL, MGPU_K_GATHER, MGPU_K_KICK, MGPU_K_DRIFT, MGPU_K_LPT, MGPU_K_READOUT,
-  MGPU_K_COUNT
+  MGPU_K_FFT_GREEN, MGPU_K_FFT_DUAL, MGPU_K_FFT_SCREEN, MGPU_K_FFT_EPI, MGPU_K_COUNT
 };
 const char *mgpu_kernel_name(int kernel);
 int mgpu_enable_kernel_timing(mgpu_ctx *ctx, int on);

@@ -84,9 +84,10 @@ static int flush_ktimers(mgpu_ctx *ctx) {
 extern "C" {
 
 const char *mgpu_kernel_name(int k) {
-  static const char *names[MGPU_K_COUNT] = {"deposit", "density", "fft_r2c", "fft_c2r", "fft_col_fwd", "fft_col_inv",
-                                            "pofk",    "spectral", "real",   "gather",  "kick",        "drift",
-                                            "lpt",     "readout"};
+  static const char *names[MGPU_K_COUNT] = {"deposit",   "density",  "fft_r2c",    "fft_c2r", "fft_col_fwd",
+                                            "fft_col_inv", "pofk",   "spectral",   "real",    "gather",
+                                            "kick",      "drift",    "lpt",        "readout", "fft_green",
+                                            "fft_dual",  "fft_screen", "fft_epi"};
   return (k >= 0 && k < MGPU_K_COUNT) ? names[k] : "?";
 }
 
@@ -182,7 +183,6 @@ int mgpu_create(const mgpu_params *p, mgpu_ctx **out) {
   float2 *force = nullptr;
   if ((rc = alloc(ctx, (void **)&force, 3 * sb, "force meshes"))) return fail(rc);
   for (int a = 0; a < 3; a++) ctx->slot[3 + a] = force + a * ctx->slot_elems;
-  ctx->acc = (unsigned long long *)force;  // needs 8 N^3 bytes <= 2 slots
   ctx->dep_scratch_bytes = deposit_scratch_bytes(ctx->N, ctx->np);
   if ((rc = alloc(ctx, &ctx->dep_scratch, ctx->dep_scratch_bytes, "deposit scratch"))) return fail(rc);
   if (p->scale_dependent) {
@@ -202,8 +202,10 @@ int mgpu_create(const mgpu_params *p, mgpu_ctx **out) {
   const int nb = std::max(1, p->pofk_nbins);
   if ((rc = alloc(ctx, (void **)&ctx->pk, 3 * (size_t)nb * sizeof(double), "P(k) bins"))) return fail(rc);
   ctx->n2len = mgpu_n2_table_len(ctx->N);
-  for (int s = 0; s < 2; s++)
+  for (int s = 0; s < 2; s++) {
     if ((rc = alloc(ctx, (void **)&ctx->n2tab[s], ctx->n2len * sizeof(double), "n2 table"))) return fail(rc);
+    if ((rc = alloc(ctx, (void **)&ctx->n2tabf[s], ctx->n2len * sizeof(float), "n2 table"))) return fail(rc);
+  }
   if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess)
     return fail(mgpu_fail(ctx, MGPU_E_HIP, "event creation failed"));
   // defined contents everywhere
@@ -211,6 +213,15 @@ int mgpu_create(const mgpu_params *p, mgpu_ctx **out) {
       hipMemsetAsync(ctx->lpt[0], 0, pb, ctx->stream) != hipSuccess ||
       hipMemsetAsync(ctx->lpt[1], 0, pb, ctx->stream) != hipSuccess ||
       hipMemsetAsync(ctx->disp, 0, pb, ctx->stream) != hipSuccess)
+    return fail(mgpu_fail(ctx, MGPU_E_HIP, "memset failed"));
+  // meshes too: the padding entries of each row are never written by the passes
+  for (int s = 0; s < 3; s++)
+    if (hipMemsetAsync(ctx->slot[s], 0, sb, ctx->stream) != hipSuccess)
+      return fail(mgpu_fail(ctx, MGPU_E_HIP, "memset failed"));
+  if (hipMemsetAsync(ctx->slot[3], 0, 3 * sb, ctx->stream) != hipSuccess ||
+      (ctx->lattice_slot && hipMemsetAsync(ctx->lattice_slot, 0,
+                                           3 * (size_t)ctx->Np * ctx->Np * ctx->fft_lattice.hp * sizeof(float2),
+                                           ctx->stream) != hipSuccess))
     return fail(mgpu_fail(ctx, MGPU_E_HIP, "memset failed"));
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(mgpu_fail(ctx, MGPU_E_HIP, "sync failed"));
   *out = ctx;
@@ -224,7 +235,8 @@ void mgpu_destroy(mgpu_ctx *ctx) {
   void *ptrs[] = {ctx->pos,      ctx->vel,       ctx->disp,      ctx->lpt[0],    ctx->lpt[1],
                   ctx->slot[0],  ctx->slot[1],   ctx->slot[2],   ctx->slot[3],   ctx->cdelta[0],
                   ctx->cdelta[1], ctx->partial,  ctx->dsum,      ctx->pk,        ctx->n2tab[0],
-                  ctx->n2tab[1], ctx->lattice_slot, ctx->slice_buf, ctx->dep_scratch};
+                  ctx->n2tab[1], ctx->n2tabf[0], ctx->n2tabf[1], ctx->lattice_slot, ctx->slice_buf,
+                  ctx->dep_scratch};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   fft_plan_free(&ctx->fft);
@@ -298,7 +310,8 @@ int mgpu_pm_transform(mgpu_ctx *ctx, mgpu_pofk *pk) {
   int rc;
   {
     PhaseTimer pt(ctx, &ctx->t.ptomesh);
-    if ((rc = fft_r2c_3d(ctx, &ctx->fft, ctx->slot[0]))) return rc;
+    // delta(x) stays in slot dx_slot when it is needed later (MG screening)
+    if ((rc = fft_r2c_3d(ctx, &ctx->fft, ctx->slot[0], (const float *)ctx->slot[ctx->dx_slot]))) return rc;
   }
   ctx->have_deposit = false;
   ctx->have_density = true;
@@ -395,12 +408,12 @@ int mgpu_lpt_fields_slices(mgpu_ctx *ctx, int slot, const mgpu_kslice *o1, const
 static float2 *mesh_ptr(mgpu_ctx *ctx, int mesh, bool &is_real) {
   is_real = false;
   switch (mesh) {
-    case MGPU_MESH_DENSITY_X: is_real = true; return ctx->slot[0];
+    case MGPU_MESH_DENSITY_X: is_real = true; return ctx->slot[ctx->dx_slot];
     case MGPU_MESH_DENSITY_K: return ctx->slot[0];
-    case MGPU_MESH_PHI_K: return ctx->slot[1];
-    case MGPU_MESH_FORCE_X: is_real = true; return ctx->slot[3];
-    case MGPU_MESH_FORCE_Y: is_real = true; return ctx->slot[4];
-    case MGPU_MESH_FORCE_Z: is_real = true; return ctx->slot[5];
+    case MGPU_MESH_PHI_K: return ctx->slot[ctx->phi_slot];
+    case MGPU_MESH_FORCE_X: is_real = true; return ctx->slot[ctx->force_slot[0]];
+    case MGPU_MESH_FORCE_Y: is_real = true; return ctx->slot[ctx->force_slot[1]];
+    case MGPU_MESH_FORCE_Z: is_real = true; return ctx->slot[ctx->force_slot[2]];
     case MGPU_MESH_CDELTA1: return ctx->cdelta[0] ? ctx->cdelta[0] : ctx->slot[0];
     case MGPU_MESH_CDELTA2: return ctx->cdelta[1];
     default: return nullptr;

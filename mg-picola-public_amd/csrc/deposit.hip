@@ -3,13 +3,17 @@
 // Global 64-bit atomics on an 8 N^3-byte accumulator are bound by the L2's
 // atomic rate (one per particle corner).  Here every particle is instead
 // binned into its 16^3-cell tile (wave-aggregated counters, prefix sum,
-// scatter of particle indices), each tile accumulates its particles in LDS
-// with 64-bit integer atomics on a (16+1)^3 block, writes the cells it owns
-// with plain stores and its +1 halo faces to a per-tile buffer, and a fix-up
-// pass lets every tile add the halo faces of its lower neighbours to its own
-// boundary cells.  The weights and their 2^-32 fixed-point rounding are those
-// of the direct deposit, and integer sums do not depend on order, so the
-// accumulator is bit-identical to a serial deposit of the same particles.
+// scatter of particle indices), and each tile accumulates its particles in
+// LDS with 64-bit integer atomics on a (16+1)^3 block.  The tile stores the
+// cells it owns into a 64-bit accumulator mesh (full 128-byte row segments)
+// and its +1 halo layer into plane, edge and corner buffers laid out like the
+// mesh cells they belong to (the x = 16(ti+1) plane, ...).  A streaming pass
+// then reads the accumulator, adds to every low-face cell (a local coordinate
+// 0) the parts its lower neighbours left in those buffers -- coalesced reads,
+// no per-cell search -- and writes delta = acc 2^-32 - 1 in the real mesh
+// layout.  The weights and their 2^-32 fixed-point rounding are those of the
+// direct deposit, and integer sums do not depend on order, so delta is
+// bit-identical to a serial deposit of the same particles.
 #include "mgpu_internal.h"
 
 #include <cmath>
@@ -19,10 +23,11 @@ namespace {
 constexpr double kFix = 4294967296.0;  // 2^32
 constexpr int kTB = 256;
 
+constexpr double kInvFix = 1.0 / 4294967296.0;
+
 struct TileGeom {
-  int N, tx, ty, tz;   // mesh size and tile dims
+  int N, tx, ty, tz;   // mesh size and tile dims (cubic tiles: tx = ty = tz)
   int ntx, nty, ntz;   // tiles per dimension
-  int hs;              // halo cells per tile
 };
 
 __device__ __forceinline__ void cell_of(double X, int N, unsigned &I, double &D) {
@@ -40,18 +45,32 @@ __device__ __forceinline__ unsigned tile_of(const TileGeom &g, unsigned IX, unsi
   return ((IX / g.tx) * g.nty + IY / g.ty) * g.ntz + IZ / g.tz;
 }
 
-// position of a halo cell (one local coordinate equal to the tile size) in the per-tile halo buffer
-__device__ __forceinline__ int halo_index(const TileGeom &g, int lx, int ly, int lz) {
-  const bool fx = lx == g.tx, fy = ly == g.ty, fz = lz == g.tz;
-  const int F = g.ty * g.tz + g.tx * g.tz + g.tx * g.ty;
-  if (fx && !fy && !fz) return ly * g.tz + lz;
-  if (!fx && fy && !fz) return g.ty * g.tz + lx * g.tz + lz;
-  if (!fx && !fy && fz) return g.ty * g.tz + g.tx * g.tz + lx * g.ty + ly;
-  if (fx && fy && !fz) return F + lz;
-  if (fx && !fy && fz) return F + g.tz + ly;
-  if (!fx && fy && fz) return F + g.tz + g.ty + lx;
-  return F + g.tz + g.ty + g.tx;
+// halo parts of the tiles, stored at the mesh cell they belong to.  With nt
+// tiles per dimension and p, q, r the tile-plane indices (cell / tile size):
+//   hx[p][y][z], hy[x][q][z], hz[r][x][y]       (faces: x, y or z local 0)
+//   exy[p][q][z], exz[p][r][y], eyz[q][r][x]     (edges)
+//   c[p][q][r]                                   (corner)
+struct Halo {
+  unsigned long long *hx, *hy, *hz, *exy, *exz, *eyz, *c;
+};
+size_t halo_cells(int N, int t) {
+  const size_t n = N, nt = N / t;
+  return 3 * nt * n * n + 3 * nt * nt * n + nt * nt * nt;
 }
+Halo halo_carve(unsigned long long *base, int N, int t) {
+  const size_t n = N, nt = N / t;
+  Halo h;
+  h.hx = base;
+  h.hy = h.hx + nt * n * n;
+  h.hz = h.hy + nt * n * n;
+  h.exy = h.hz + nt * n * n;
+  h.exz = h.exy + nt * nt * n;
+  h.eyz = h.exz + nt * nt * n;
+  h.c = h.eyz + nt * nt * n;
+  return h;
+}
+
+__device__ __forceinline__ float to_delta(unsigned long long v) { return (float)((double)v * kInvFix - 1.0); }
 
 // 1. tile of every particle and its rank inside the tile (wave-aggregated atomics)
 __global__ __launch_bounds__(kTB) void k_dep_bin(const float *__restrict__ pos, size_t np, TileGeom g, double scale,
@@ -151,8 +170,7 @@ __global__ __launch_bounds__(kTB) void k_dep_scatter(const unsigned *__restrict_
 __global__ __launch_bounds__(kTB) void k_dep_tile(const float *__restrict__ pos, size_t np,
                                                   const unsigned *__restrict__ idx, const unsigned *__restrict__ off,
                                                   TileGeom g, double scale, double wpar,
-                                                  unsigned long long *__restrict__ acc,
-                                                  unsigned long long *__restrict__ halo) {
+                                                  unsigned long long *__restrict__ acc, Halo h) {
   extern __shared__ unsigned long long blk[];
   const int bx = g.tx + 1, by = g.ty + 1, bz = g.tz + 1, nblk = bx * by * bz;
   const unsigned t = blockIdx.x;
@@ -186,39 +204,47 @@ __global__ __launch_bounds__(kTB) void k_dep_tile(const float *__restrict__ pos,
 #undef DEP
   }
   __syncthreads();
-  const size_t N = g.N;
+  const size_t N = g.N, nt = g.ntx;
+  const int T = g.tx;
+  const size_t p = (ti + 1) % g.ntx, q = (tj + 1) % g.nty, r = (tk + 1) % g.ntz;
   for (int e = threadIdx.x; e < nblk; e += blockDim.x) {
     const int lz = e % bz, ly = (e / bz) % by, lx = e / (bz * by);
     const unsigned long long v = blk[e];
-    if (lx < g.tx && ly < g.ty && lz < g.tz)
-      acc[((size_t)(X0 + lx) * N + (Y0 + ly)) * N + (Z0 + lz)] = v;
-    else
-      halo[(size_t)t * g.hs + halo_index(g, lx, ly, lz)] = v;
+    const bool fx = lx == T, fy = ly == T, fz = lz == T;
+    const size_t x = X0 + lx, y = Y0 + ly, z = Z0 + lz;
+    if (!fx && !fy && !fz) acc[(x * N + y) * N + z] = v;
+    else if (fx && !fy && !fz) h.hx[(p * N + y) * N + z] = v;
+    else if (!fx && fy && !fz) h.hy[(x * nt + q) * N + z] = v;
+    else if (!fx && !fy && fz) h.hz[(r * N + x) * N + y] = v;
+    else if (fx && fy && !fz) h.exy[(p * nt + q) * N + z] = v;
+    else if (fx && !fy && fz) h.exz[(p * nt + r) * N + y] = v;
+    else if (!fx && fy && fz) h.eyz[(q * nt + r) * N + x] = v;
+    else h.c[(p * nt + q) * nt + r] = v;
   }
 }
 
-// 5. every tile adds the halo faces its lower neighbours produced for its cells
-__global__ __launch_bounds__(kTB) void k_dep_fixup(unsigned long long *__restrict__ acc,
-                                                   const unsigned long long *__restrict__ halo, TileGeom g) {
-  const unsigned t = blockIdx.x;
-  const int ti = t / (g.nty * g.ntz), tj = (t / g.ntz) % g.nty, tk = t % g.ntz;
-  const size_t N = g.N;
-  const int ncell = g.tx * g.ty * g.tz;
-  for (int e = threadIdx.x; e < ncell; e += blockDim.x) {
-    const int lz = e % g.tz, ly = (e / g.tz) % g.ty, lx = e / (g.tz * g.ty);
-    if (lx != 0 && ly != 0 && lz != 0) continue;
-    unsigned long long s = 0;
-    for (int d = 1; d < 8; d++) {
-      const int dx = d & 1, dy = (d >> 1) & 1, dz = (d >> 2) & 1;
-      if ((dx && lx) || (dy && ly) || (dz && lz)) continue;
-      const int ni = dx ? (ti == 0 ? g.ntx - 1 : ti - 1) : ti;
-      const int nj = dy ? (tj == 0 ? g.nty - 1 : tj - 1) : tj;
-      const int nk = dz ? (tk == 0 ? g.ntz - 1 : tk - 1) : tk;
-      const size_t nb = ((size_t)ni * g.nty + nj) * g.ntz + nk;
-      s += halo[nb * g.hs + halo_index(g, dx ? g.tx : lx, dy ? g.ty : ly, dz ? g.tz : lz)];
-    }
-    acc[((size_t)(ti * g.tx + lx) * N + (tj * g.ty + ly)) * N + (tk * g.tz + lz)] += s;
-  }
+// 5. streaming pass over the cells: accumulator + (low-face cells) the halo
+// parts of the lower neighbours, converted to delta in the real layout.
+// Grid (N*N rows, z blocks); tile sizes are powers of two (lt = log2 of the tile size).
+__global__ __launch_bounds__(kTB) void k_dep_finish(const unsigned long long *__restrict__ acc, Halo h, int N,
+                                                    int lt, int HP2, float *__restrict__ d0) {
+  const int z = blockIdx.y * blockDim.x + threadIdx.x;
+  if (z >= N) return;
+  const unsigned row = blockIdx.x;
+  const size_t n = N, nt = N >> lt;
+  const size_t y = row % (unsigned)N, x = row / (unsigned)N;
+  unsigned long long s = acc[(size_t)row * N + z];
+  const int tm = (1 << lt) - 1;
+  const bool ox = (x & tm) == 0, oy = (y & tm) == 0, oz = (z & tm) == 0;
+  const size_t p = x >> lt, q = y >> lt, r = (size_t)z >> lt;
+  if (ox) s += h.hx[(p * n + y) * n + z];
+  if (oy) s += h.hy[(x * nt + q) * n + z];
+  if (oz) s += h.hz[(r * n + x) * n + y];
+  if (ox && oy) s += h.exy[(p * nt + q) * n + z];
+  if (ox && oz) s += h.exz[(p * nt + r) * n + y];
+  if (oy && oz) s += h.eyz[(q * nt + r) * n + x];
+  if (ox && oy && oz) s += h.c[(p * nt + q) * nt + r];
+  d0[(size_t)row * HP2 + z] = to_delta(s);
 }
 
 TileGeom make_geom(int N) {
@@ -229,7 +255,6 @@ TileGeom make_geom(int N) {
   g.ntx = N / g.tx;
   g.nty = N / g.ty;
   g.ntz = N / g.tz;
-  g.hs = g.ty * g.tz + g.tx * g.tz + g.tx * g.ty + g.tz + g.ty + g.tx + 1;
   return g;
 }
 
@@ -240,10 +265,11 @@ size_t deposit_scratch_bytes(int N, size_t np) {
   TileGeom g = make_geom(N);
   const size_t nt = (size_t)g.ntx * g.nty * g.ntz;
   const size_t nsum = (nt + 1023) / 1024;
-  return sizeof(unsigned) * (2 * (nt + 1) + nsum + 3 * np) + sizeof(unsigned long long) * nt * g.hs + 256 * 8;
+  return sizeof(unsigned) * (2 * (nt + 1) + nsum + 3 * np) + sizeof(unsigned long long) * halo_cells(N, g.tx) +
+         256 * 8;
 }
 
-int deposit_tiled(mgpu_ctx *ctx) {
+int deposit_tiled(mgpu_ctx *ctx, unsigned long long *acc, float *d) {
   const int N = ctx->N;
   const size_t np = ctx->np;
   TileGeom g = make_geom(N);
@@ -257,7 +283,7 @@ int deposit_tiled(mgpu_ctx *ctx) {
     b += (bytes + 255) / 256 * 256;
     return r;
   };
-  unsigned long long *halo = (unsigned long long *)take(sizeof(unsigned long long) * nt * g.hs);
+  const Halo halo = halo_carve((unsigned long long *)take(sizeof(unsigned long long) * halo_cells(N, g.tx)), N, g.tx);
   unsigned *count = (unsigned *)take(sizeof(unsigned) * (nt + 1));
   unsigned *off = (unsigned *)take(sizeof(unsigned) * (nt + 1));
   unsigned *sums = (unsigned *)take(sizeof(unsigned) * nsum);
@@ -285,17 +311,21 @@ int deposit_tiled(mgpu_ctx *ctx) {
   }
   MGPU_HIP(ctx, hipGetLastError());
   const size_t shmem = sizeof(unsigned long long) * (g.tx + 1) * (g.ty + 1) * (g.tz + 1);
+  const int HP2 = 2 * ctx->HP;
+  const double cells = (double)N * N * N, hcells = (double)halo_cells(N, g.tx);
   {
-    // compulsory: indices + positions read, accumulator and halo written
-    KScope ks(ctx, MGPU_K_DEPOSIT, 16.0 * np + 8.0 * ((double)N * N * N + (double)nt * g.hs));
+    // compulsory: indices + positions read, accumulator and halo parts written
+    KScope ks(ctx, MGPU_K_DEPOSIT, 16.0 * np + 8.0 * (cells + hcells));
     hipLaunchKernelGGL(k_dep_tile, dim3(nt), dim3(kTB), shmem, ctx->stream, ctx->pos, np, idx, off, g, scale, wpar,
-                       ctx->acc, halo);
+                       acc, halo);
   }
   MGPU_HIP(ctx, hipGetLastError());
   {
-    const double bnd = (double)nt * (g.tx * g.ty * g.tz - (g.tx - 1) * (g.ty - 1) * (g.tz - 1));
-    KScope ks(ctx, MGPU_K_DEPOSIT, 8.0 * ((double)nt * g.hs + 2.0 * bnd));
-    hipLaunchKernelGGL(k_dep_fixup, dim3(nt), dim3(kTB), 0, ctx->stream, ctx->acc, halo, g);
+    // accumulator and halo parts read, delta written
+    KScope ks(ctx, MGPU_K_DENSITY, 8.0 * (cells + hcells) + 4.0 * cells);
+    const int lt = g.tx == 16 ? 4 : g.tx == 8 ? 3 : 2;
+    hipLaunchKernelGGL(k_dep_finish, dim3((unsigned)((size_t)N * N), cdiv((size_t)N, kTB)), dim3(kTB), 0,
+                       ctx->stream, acc, halo, N, lt, HP2, d);
   }
   MGPU_HIP(ctx, hipGetLastError());
   return MGPU_OK;

@@ -1,0 +1,382 @@
+// Spectral passes fused into FFT passes (gfx950).
+//
+// The reference applies every k-space multiplier and every real-space
+// screening factor as a separate sweep over FP64 meshes between FFTW calls
+// (src/mg.h:21-309, src/auxPM.c:450-535).  Here each multiplier rides on the
+// FFT pass that touches the same data, so no mesh is read or written only to
+// be scaled:
+//   green x pass (inverse)   S = delta(k) [+ phi(k)] times the Green's function
+//                            or smoothing window, then the x transform; the
+//                            force variant writes two transforms, of i n_x G
+//                            and of G (the y and z components share G);
+//   dual y pass (inverse)    y transform of T and of i n_y T from one read;
+//   z screen pass            c2r of the potential (or smoothed density) row,
+//                            the real-space screening against the delta(x)
+//                            row, and the r2c of the effective density row, in
+//                            one kernel with the row kept in LDS;
+//   epilogue x pass (fwd)    x transform followed by coupling n^2/(n^2 + m^2);
+//   z c2r with i n_z         (fft.hip) the z force component.
+// Integer wavenumbers follow the Forces convention n in (-N/2, N/2] (the
+// Nyquist index counts as +N/2, src/auxPM.c:467-479); only n^2 enters the
+// MG and smoothing multipliers, where the sign convention does not matter.
+#include "fft_core.h"
+
+#include <cmath>
+
+namespace {
+
+using namespace fftc;
+
+__device__ __forceinline__ int nsgn(int i, int N) { return i > N / 2 ? i - N : i; }
+
+enum { GREEN_FORCE = FUSED_GREEN_FORCE, GREEN_LAPLACE = FUSED_GREEN_LAPLACE, GREEN_SMOOTH = FUSED_GREEN_SMOOTH };
+struct GreenArgs {
+  float cnorm;  // GREEN_FORCE: 1 / (N^3 2 pi / Box)
+  float nf;     // GREEN_LAPLACE: 1.5 Omega / a (Box / (2 pi c/H0))^2 / N^3; GREEN_SMOOTH: 1 / N^3
+  float kfac;   // GREEN_SMOOTH: 2 pi R / Box
+  int filter;   // GREEN_SMOOTH: MGPU_FILTER_*
+};
+
+__device__ __forceinline__ float window(int filter, float kR) {
+  switch (filter) {
+    case MGPU_FILTER_GAUSSIAN: return expf(-0.5f * kR * kR);
+    case MGPU_FILTER_TOPHAT: return kR < 1e-5f ? 1.f : 3.f / (kR * kR * kR) * (sinf(kR) - kR * cosf(kR));
+    case MGPU_FILTER_SHARPK: return kR < 1.f ? 1.f : 0.f;
+    default: return 1.f;
+  }
+}
+
+// x pass, inverse.  Tile: columns kz = blockIdx.x*T + c, outer index ky = blockIdx.y,
+// line index kx.  out1 is written by GREEN_FORCE only.
+template <int L, int T, int MODE>
+__global__ __launch_bounds__(col_threads(L, T)) void k_col_green(const float2 *__restrict__ a,
+                                                                   const float2 *__restrict__ b, float2 *out0,
+                                                                   float2 *out1, size_t ls, size_t os,
+                                                                   const float2 *__restrict__ tw, GreenArgs ga) {
+  extern __shared__ float2 lds[];
+  constexpr int NT = col_threads(L, T), E = col_ept(L, T);
+  constexpr int R0 = pick_radix(L, E), NB0 = E / R0, M0 = L / R0;
+  constexpr int N = L;
+  const int tid = threadIdx.x;
+  const int ky = blockIdx.y, kz0 = blockIdx.x * T;
+  const size_t tile = (size_t)ky * os + kz0;
+  float2 p1[E], p2[E];
+#pragma unroll
+  for (int q = 0; q < NB0; ++q) {
+    const int g = tid + q * NT, c = g % T, j = g / T;
+    const float2 *pa = a + tile + (size_t)j * ls + c;
+    const float2 *pb = b ? b + tile + (size_t)j * ls + c : nullptr;
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+      float2 v = pa[(size_t)(r * M0) * ls];
+      if (pb) v = cadd(v, pb[(size_t)(r * M0) * ls]);
+      p1[q * R0 + r] = v;
+    }
+  }
+  const int ny = nsgn(ky, N);
+#pragma unroll
+  for (int q = 0; q < NB0; ++q) {
+    const int g = tid + q * NT, c = g % T, j = g / T;
+    const int kz = kz0 + c;
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+      const int nx = nsgn(j + r * M0, N);
+      const int n2 = nx * nx + ny * ny + kz * kz;
+      const float2 S = p1[q * R0 + r];
+      float f = 0.f;
+      if (kz <= N / 2) {
+        if (MODE == GREEN_FORCE) f = n2 > 0 ? -ga.cnorm / (float)n2 : 0.f;
+        if (MODE == GREEN_LAPLACE) f = n2 > 0 ? -ga.nf / (float)n2 : 0.f;
+        if (MODE == GREEN_SMOOTH) f = window(ga.filter, sqrtf((float)n2) * ga.kfac) * ga.nf;
+      }
+      const float2 G = cscale(S, f);
+      if (MODE == GREEN_FORCE) {
+        p1[q * R0 + r] = cscale(mul_i<+1>(G), (float)nx);
+        p2[q * R0 + r] = G;
+      } else {
+        p1[q * R0 + r] = G;
+      }
+    }
+  }
+  auto ld = [&](int, int, int) { return make_float2(0.f, 0.f); };
+  auto st0 = [&](int c, int j, int o, float2 v) { out0[tile + (size_t)j * ls + c + (size_t)o * ls] = v; };
+  fft_stages<L, T, NT, +1, 1, T, true, false, 1>(lds, tw, 1, tid, ld, st0, p1);
+  if (MODE == GREEN_FORCE) {
+    __syncthreads();
+    auto st1 = [&](int c, int j, int o, float2 v) { out1[tile + (size_t)j * ls + c + (size_t)o * ls] = v; };
+    fft_stages<L, T, NT, +1, 1, T, true, false, 1>(lds, tw, 1, tid, ld, st1, p2);
+  }
+}
+
+// y pass, inverse, two outputs from one read: buf <- Y[buf], oy <- Y[i n_y buf].
+// Tile: columns kz, outer index kx = blockIdx.y, line index ky.
+template <int L, int T>
+__global__ __launch_bounds__(col_threads(L, T)) void k_col_dual(float2 *buf, float2 *__restrict__ oy, size_t ls,
+                                                                  size_t os, const float2 *__restrict__ tw) {
+  extern __shared__ float2 lds[];
+  constexpr int NT = col_threads(L, T), E = col_ept(L, T);
+  constexpr int R0 = pick_radix(L, E), NB0 = E / R0, M0 = L / R0;
+  const int tid = threadIdx.x;
+  const size_t tile = (size_t)blockIdx.y * os + (size_t)blockIdx.x * T;
+  float2 p1[E], p2[E];
+#pragma unroll
+  for (int q = 0; q < NB0; ++q) {
+    const int g = tid + q * NT, c = g % T, j = g / T;
+    const float2 *pa = buf + tile + (size_t)j * ls + c;
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+      const float2 v = pa[(size_t)(r * M0) * ls];
+      p2[q * R0 + r] = v;
+      p1[q * R0 + r] = cscale(mul_i<+1>(v), (float)nsgn(j + r * M0, L));
+    }
+  }
+  auto ld = [&](int, int, int) { return make_float2(0.f, 0.f); };
+  auto st0 = [&](int c, int j, int o, float2 v) { oy[tile + (size_t)j * ls + c + (size_t)o * ls] = v; };
+  fft_stages<L, T, NT, +1, 1, T, true, false, 1>(lds, tw, 1, tid, ld, st0, p1);
+  __syncthreads();
+  auto st1 = [&](int c, int j, int o, float2 v) { buf[tile + (size_t)j * ls + c + (size_t)o * ls] = v; };
+  fft_stages<L, T, NT, +1, 1, T, true, false, 1>(lds, tw, 1, tid, ld, st1, p2);
+}
+
+// x pass, forward, then phi = coupling n^2/(n^2 + m2) X (0 at n = 0): EffDensitykToPhiofk
+// (src/mg.h:74-116) on the transform's output.  Tile: columns kz, outer ky, line kx.
+template <int L, int T>
+__global__ __launch_bounds__(col_threads(L, T)) void k_col_epi(float2 *buf, size_t ls, size_t os,
+                                                                 const float2 *__restrict__ tw, float coupling,
+                                                                 float m2) {
+  extern __shared__ float2 lds[];
+  constexpr int NT = col_threads(L, T);
+  constexpr int N = L;
+  const int tid = threadIdx.x;
+  const int ky = blockIdx.y, kz0 = blockIdx.x * T;
+  const size_t tile = (size_t)ky * os + kz0;
+  const int ny = nsgn(ky, N);
+  auto ld = [&](int c, int j, int o) { return buf[tile + (size_t)j * ls + c + (size_t)o * ls]; };
+  auto st = [&](int c, int j, int o, float2 v) {
+    const int kz = kz0 + c, nx = nsgn(j + o, N);
+    const int n2 = nx * nx + ny * ny + kz * kz;
+    const float f = (kz <= N / 2 && n2 > 0) ? coupling * ((float)n2 / ((float)n2 + m2)) : 0.f;
+    buf[tile + (size_t)j * ls + c + (size_t)o * ls] = cscale(v, f);
+  };
+  fft_stages<L, T, NT, -1, 1, T, true, false, 1>(lds, tw, 1, tid, ld, st);
+}
+
+enum { SCREEN_POTENTIAL = 0, SCREEN_DENSITY = 1 };
+
+// z pass: c2r of the row (potential Phi_N or smoothed density delta_R), the
+// screening against the delta(x) row, r2c of the effective density; the row
+// stays in LDS between the three steps.
+//   SCREEN_POTENTIAL (f(R), m-beta; src/mg.h:169-171, src/user_defined_functions.h:725-752):
+//       delta_eff = delta * min(1, |p0 / Phi|) where Phi < 0, else delta       (p0 = phi_crit)
+//   SCREEN_DENSITY (DGP; src/mg.h:235-244, src/user_defined_functions.h:757-773):
+//       delta_eff = delta * p0 * 2 (sqrt(1 + f) - 1) / f, f = p1 (1 + delta_R), 1 if f < 1e-5
+//                                                                   (p0 = coupling, p1 = dgp_fac)
+template <int N, int KIND>
+__global__ __launch_bounds__(RowCfg<N>::NT) void k_z_screen(float2 *buf, const float2 *__restrict__ dx, int HP,
+                                                             size_t nrows, const float2 *__restrict__ tw, float p0,
+                                                             float p1) {
+  using C = RowCfg<N>;
+  constexpr int ITER = C::RB * C::M / C::NT;
+  static_assert(C::RB * C::M % C::NT == 0, "row block not a multiple of the block size");
+  extern __shared__ float2 lds[];
+  const int tid = threadIdx.x;
+  const size_t row0 = (size_t)blockIdx.x * C::RB;
+  // the delta(x) pairs this thread screens, loaded first so they arrive during the c2r
+  float2 dv[ITER];
+#pragma unroll
+  for (int it = 0; it < ITER; ++it) {
+    const int e = tid + it * C::NT, r = e / C::M, n = e % C::M;
+    const size_t row = row0 + r < nrows ? row0 + r : nrows - 1;
+    dv[it] = dx[row * HP + n];
+  }
+  z_merge<N, false>(lds, buf, HP, row0, nrows, tw, tid);
+  __syncthreads();
+  auto ldl = [&](int r, int j, int o) { return lds[r * C::LS + j + o]; };
+  auto stl = [&](int r, int j, int o, float2 v) { lds[r * C::LS + j + o] = v; };
+  fft_stages<C::M, C::RB, C::NT, +1, C::LS, 1, false, true, 1>(lds, tw, 2, tid, ldl, stl);
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < ITER; ++it) {
+    const int e = tid + it * C::NT, r = e / C::M, n = e % C::M;
+    const float2 f = lds[r * C::LS + n];
+    const float2 d = dv[it];
+    float s0, s1;
+    if (KIND == SCREEN_POTENTIAL) {
+      s0 = f.x < 0.f ? fminf(1.f, fabsf(p0 / f.x)) : 1.f;
+      s1 = f.y < 0.f ? fminf(1.f, fabsf(p0 / f.y)) : 1.f;
+    } else {
+      const float a0 = p1 * (1.f + f.x), a1 = p1 * (1.f + f.y);
+      s0 = p0 * (a0 < 1e-5f ? 1.f : 2.f * (sqrtf(1.f + a0) - 1.f) / a0);
+      s1 = p0 * (a1 < 1e-5f ? 1.f : 2.f * (sqrtf(1.f + a1) - 1.f) / a1);
+    }
+    lds[r * C::LS + n] = make_float2(d.x * s0, d.y * s1);
+  }
+  __syncthreads();
+  float2 w[ZSplit<N>::ITER];
+  z_split_twiddles<N>(w, tw, tid);
+  fft_stages<C::M, C::RB, C::NT, -1, C::LS, 1, false, true, 1>(lds, tw, 2, tid, ldl, stl);
+  __syncthreads();
+  z_split<N>(lds, w, buf, HP, row0, nrows, tid);
+}
+
+template <class K>
+int set_lds(mgpu_ctx *ctx, K kern, size_t lds) {
+  if (lds > 65536) MGPU_HIP(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  return MGPU_OK;
+}
+
+template <int N, int T>
+int green_launch(mgpu_ctx *ctx, int mode, const float2 *a, const float2 *b, float2 *o0, float2 *o1, GreenArgs ga) {
+  const size_t HP = ctx->fft.hp, ls = (size_t)N * HP, os = HP;
+  const size_t lds = (size_t)N * T * sizeof(float2);
+  dim3 grid(HP / T, N), block(col_threads(N, T));
+  int rc;
+  if (mode == GREEN_FORCE) {
+    if ((rc = set_lds(ctx, k_col_green<N, T, GREEN_FORCE>, lds))) return rc;
+    hipLaunchKernelGGL((k_col_green<N, T, GREEN_FORCE>), grid, block, lds, ctx->stream, a, b, o0, o1, ls, os,
+                       ctx->fft.tw, ga);
+  } else if (mode == GREEN_LAPLACE) {
+    if ((rc = set_lds(ctx, k_col_green<N, T, GREEN_LAPLACE>, lds))) return rc;
+    hipLaunchKernelGGL((k_col_green<N, T, GREEN_LAPLACE>), grid, block, lds, ctx->stream, a, b, o0, o1, ls, os,
+                       ctx->fft.tw, ga);
+  } else {
+    if ((rc = set_lds(ctx, k_col_green<N, T, GREEN_SMOOTH>, lds))) return rc;
+    hipLaunchKernelGGL((k_col_green<N, T, GREEN_SMOOTH>), grid, block, lds, ctx->stream, a, b, o0, o1, ls, os,
+                       ctx->fft.tw, ga);
+  }
+  MGPU_HIP(ctx, hipGetLastError());
+  return MGPU_OK;
+}
+
+template <int N, int T>
+int dual_launch(mgpu_ctx *ctx, float2 *buf, float2 *oy) {
+  const size_t HP = ctx->fft.hp, ls = HP, os = (size_t)N * HP;
+  const size_t lds = (size_t)N * T * sizeof(float2);
+  int rc;
+  if ((rc = set_lds(ctx, k_col_dual<N, T>, lds))) return rc;
+  hipLaunchKernelGGL((k_col_dual<N, T>), dim3(HP / T, N), dim3(col_threads(N, T)), lds, ctx->stream, buf, oy, ls, os,
+                     ctx->fft.tw);
+  MGPU_HIP(ctx, hipGetLastError());
+  return MGPU_OK;
+}
+
+template <int N, int T>
+int epi_launch(mgpu_ctx *ctx, float2 *buf, float coupling, float m2) {
+  const size_t HP = ctx->fft.hp, ls = (size_t)N * HP, os = HP;
+  const size_t lds = (size_t)N * T * sizeof(float2);
+  int rc;
+  if ((rc = set_lds(ctx, k_col_epi<N, T>, lds))) return rc;
+  hipLaunchKernelGGL((k_col_epi<N, T>), dim3(HP / T, N), dim3(col_threads(N, T)), lds, ctx->stream, buf, ls, os,
+                     ctx->fft.tw, coupling, m2);
+  MGPU_HIP(ctx, hipGetLastError());
+  return MGPU_OK;
+}
+
+template <int N>
+int screen_launch(mgpu_ctx *ctx, int kind, float2 *buf, const float2 *dx, float p0, float p1) {
+  using C = RowCfg<N>;
+  const size_t nrows = (size_t)N * N;
+  const size_t lds = (size_t)C::RB * C::LS * sizeof(float2);
+  dim3 grid(cdiv(nrows, C::RB)), block(C::NT);
+  int rc;
+  if (kind == SCREEN_POTENTIAL) {
+    if ((rc = set_lds(ctx, k_z_screen<N, SCREEN_POTENTIAL>, lds))) return rc;
+    hipLaunchKernelGGL((k_z_screen<N, SCREEN_POTENTIAL>), grid, block, lds, ctx->stream, buf, dx, (int)ctx->fft.hp,
+                       nrows, ctx->fft.tw, p0, p1);
+  } else {
+    if ((rc = set_lds(ctx, k_z_screen<N, SCREEN_DENSITY>, lds))) return rc;
+    hipLaunchKernelGGL((k_z_screen<N, SCREEN_DENSITY>), grid, block, lds, ctx->stream, buf, dx, (int)ctx->fft.hp,
+                       nrows, ctx->fft.tw, p0, p1);
+  }
+  MGPU_HIP(ctx, hipGetLastError());
+  return MGPU_OK;
+}
+
+// column kernels are instantiated for the tile width the plan chose
+template <int N, template <int, int> class F, class... A>
+int by_tile(mgpu_ctx *ctx, A... args) {
+  if constexpr (col_threads(N, 16) <= 1024) {
+    if (ctx->fft.tcols == 16) return F<N, 16>::run(ctx, args...);
+  }
+  return F<N, 8>::run(ctx, args...);
+}
+template <int N, int T>
+struct GreenF {
+  static int run(mgpu_ctx *c, int m, const float2 *a, const float2 *b, float2 *o0, float2 *o1, GreenArgs g) {
+    return green_launch<N, T>(c, m, a, b, o0, o1, g);
+  }
+};
+template <int N, int T>
+struct DualF {
+  static int run(mgpu_ctx *c, float2 *buf, float2 *oy) { return dual_launch<N, T>(c, buf, oy); }
+};
+template <int N, int T>
+struct EpiF {
+  static int run(mgpu_ctx *c, float2 *buf, float cp, float m2) { return epi_launch<N, T>(c, buf, cp, m2); }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------- host entry points
+
+int fused_green_x(mgpu_ctx *ctx, int mode, const float2 *a, const float2 *b, float2 *o0, float2 *o1, double p0,
+                  double p1, int filter) {
+  GreenArgs ga{0.f, 0.f, 0.f, filter};
+  if (mode == GREEN_FORCE) ga.cnorm = (float)p0;
+  else ga.nf = (float)p0;
+  ga.kfac = (float)p1;
+  const double modes = (double)ctx->N * ctx->N * (ctx->N / 2 + 1);
+  KScope ks(ctx, MGPU_K_FFT_GREEN, 8.0 * modes * ((b ? 2 : 1) + (mode == GREEN_FORCE ? 2 : 1)));
+  switch (ctx->N) {
+#define X(S) \
+  case S:    \
+    return by_tile<S, GreenF>(ctx, mode, a, b, o0, o1, ga);
+    MGPU_FFT_SIZES(X)
+#undef X
+    default:
+      return mgpu_fail(ctx, MGPU_E_INVALID, "unsupported FFT size");
+  }
+}
+
+int fused_dual_y(mgpu_ctx *ctx, float2 *buf, float2 *oy) {
+  const double modes = (double)ctx->N * ctx->N * (ctx->N / 2 + 1);
+  KScope ks(ctx, MGPU_K_FFT_DUAL, 8.0 * modes * 3);
+  switch (ctx->N) {
+#define X(S) \
+  case S:    \
+    return by_tile<S, DualF>(ctx, buf, oy);
+    MGPU_FFT_SIZES(X)
+#undef X
+    default:
+      return mgpu_fail(ctx, MGPU_E_INVALID, "unsupported FFT size");
+  }
+}
+
+int fused_epi_phi_x(mgpu_ctx *ctx, float2 *buf, double coupling, double massterm2) {
+  const double modes = (double)ctx->N * ctx->N * (ctx->N / 2 + 1);
+  KScope ks(ctx, MGPU_K_FFT_EPI, 16.0 * modes);
+  switch (ctx->N) {
+#define X(S) \
+  case S:    \
+    return by_tile<S, EpiF>(ctx, buf, (float)coupling, (float)massterm2);
+    MGPU_FFT_SIZES(X)
+#undef X
+    default:
+      return mgpu_fail(ctx, MGPU_E_INVALID, "unsupported FFT size");
+  }
+}
+
+int fused_z_screen(mgpu_ctx *ctx, int density_kind, float2 *buf, const float *dx, double p0, double p1) {
+  const double rows = (double)ctx->N * ctx->N;
+  KScope ks(ctx, MGPU_K_FFT_SCREEN, rows * (16.0 * (ctx->N / 2 + 1) + 4.0 * ctx->N));
+  const int kind = density_kind ? SCREEN_DENSITY : SCREEN_POTENTIAL;
+  switch (ctx->N) {
+#define X(S) \
+  case S:    \
+    return screen_launch<S>(ctx, kind, buf, (const float2 *)dx, (float)p0, (float)p1);
+    MGPU_FFT_SIZES(X)
+#undef X
+    default:
+      return mgpu_fail(ctx, MGPU_E_INVALID, "unsupported FFT size");
+  }
+}

@@ -12,6 +12,7 @@
 #include "mgpu_internal.h"
 #include "ranlux.h"
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -239,35 +240,72 @@ __global__ __launch_bounds__(kBlock) void k_init_particles(float *__restrict__ p
 
 // ---------------------------------------------------------------- lattice fold
 // LPT field components X_a = i k_a/k^2 (g1 cd1 + g2 cd2) at half-spectrum index
-// (i, j, l), l in [0, N/2], 2LPT.c wavenumbers (src/2LPT.c:1583-1629).
-__device__ __forceinline__ void lpt_x(int i, int j, int l, int N, int HP, int Np, double kf,
-                                      const float2 *__restrict__ cd1, const float2 *__restrict__ cd2,
-                                      const double *__restrict__ g1, const double *__restrict__ g2, double X[6]) {
-  const double kv[3] = {kc(i, N, kf), kc(j, N, kf), kc(l, N, kf)};
-  const double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+// (i, j, l), l in [0, N/2], 2LPT.c wavenumbers (src/2LPT.c:1583-1629), in FP32
+// (the stored spectra and the output fields are FP32).  ikf = 1 / (2 pi/Box).
+// growth g_o(|n|^2) from FP32 per-|n|^2 tables
+struct GTab {
+  const float *t1, *t2;
+  __device__ __forceinline__ float g1(int n2) const { return t1[n2]; }
+  __device__ __forceinline__ float g2(int n2) const { return t2[n2]; }
+};
+// growth from the cubic Hermite ln k slices held in LDS (num, dnum, den, dden of
+// order 1, then of order 2), g = norm num / den at ln k = ln kf + ln|n|
+struct KSlf {
+  float y0, dy, norm;
+  int ny;
+};
+__device__ __forceinline__ float hermf(const float *v, const float *dv, const KSlf &s, float y) {
+  const float ymax = s.y0 + s.dy * (float)(s.ny - 1);
+  y = fminf(fmaxf(y, s.y0), ymax);
+  int j = (int)floorf((y - s.y0) / s.dy);
+  j = min(max(j, 0), s.ny - 2);
+  const float h = s.dy, t = (y - (s.y0 + (float)j * s.dy)) / h, t2 = t * t, t3 = t2 * t;
+  return (2.f * t3 - 3.f * t2 + 1.f) * v[j] + (t3 - 2.f * t2 + t) * h * dv[j] + (-2.f * t3 + 3.f * t2) * v[j + 1] +
+         (t3 - t2) * h * dv[j + 1];
+}
+struct GSlice {
+  const float *sl;
+  KSlf s1, s2;
+  float lnkf;
+  __device__ __forceinline__ float g(const float *b, const KSlf &s, int n2) const {
+    const float y = lnkf + 0.5f * __logf((float)n2);
+    return s.norm * (hermf(b, b + s.ny, s, y) / hermf(b + 2 * s.ny, b + 3 * s.ny, s, y));
+  }
+  __device__ __forceinline__ float g1(int n2) const { return g(sl, s1, n2); }
+  __device__ __forceinline__ float g2(int n2) const { return g(sl + 4 * s1.ny, s2, n2); }
+};
+
+template <class G>
+__device__ __forceinline__ void lpt_xf(int i, int j, int l, int N, int HP, int Np, float ikf,
+                                       const float2 *__restrict__ cd1, const float2 *__restrict__ cd2, const G &gr,
+                                       float X[6]) {
+  const int ni = i < N / 2 ? i : i - N, nj = j < N / 2 ? j : j - N, nl = l < N / 2 ? l : l - N;
+  const int n2 = ni * ni + nj * nj + nl * nl;
 #pragma unroll
-  for (int q = 0; q < 6; q++) X[q] = 0.0;
-  if (k2 == 0.0) return;
-  const int a = nabs(i, N), b = nabs(j, N), c = nabs(l, N);
-  const int n2 = a * a + b * b + c * c;
+  for (int q = 0; q < 6; q++) X[q] = 0.f;
+  if (n2 == 0) return;
   const size_t e = ((size_t)i * N + j) * HP + l;
-  double dr = 0.0, di = 0.0;
+  float dr = 0.f, di = 0.f;
   // the 1LPT field is zero outside the initial-condition cube |n_a| <= Nsample/2
-  if (a <= Np / 2 && b <= Np / 2 && c <= Np / 2) {
+  if (abs(ni) <= Np / 2 && abs(nj) <= Np / 2 && abs(nl) <= Np / 2) {
     const float2 d = cd1[e];
-    dr = d.x * g1[n2];
-    di = d.y * g1[n2];
+    const float g = gr.g1(n2);
+    dr = d.x * g;
+    di = d.y * g;
   }
   if (cd2) {
     const float2 d = cd2[e];
-    dr += d.x * g2[n2];
-    di += d.y * g2[n2];
+    const float g = gr.g2(n2);
+    dr += d.x * g;
+    di += d.y * g;
   }
-  const double inv = 1.0 / k2;
+  // k_a / k^2 = n_a / (kf n^2)
+  const float s = ikf / (float)n2;
+  const float nv[3] = {(float)ni * s, (float)nj * s, (float)nl * s};
 #pragma unroll
   for (int q = 0; q < 3; q++) {
-    X[2 * q] = -kv[q] * inv * di;
-    X[2 * q + 1] = kv[q] * inv * dr;
+    X[2 * q] = -nv[q] * di;
+    X[2 * q + 1] = nv[q] * dr;
   }
 }
 
@@ -277,53 +315,104 @@ __device__ __forceinline__ void lpt_x(int i, int j, int l, int N, int HP, int Np
 //   F_f(k') = sum_{k = k' (mod Nsample)} F(k),
 // F the Hermitian extension of the half spectrum with the kz = 0, N/2 planes
 // symmetrised as the c2r's real-part rule implies.  One thread per output mode;
-// the folded DC mode is zeroed, which removes the lattice mean exactly
-// (src/2LPT.c:1719-1729).
-__global__ __launch_bounds__(kBlock) void k_lpt_fold(const float2 *__restrict__ cd1, const float2 *__restrict__ cd2,
-                                                     const double *__restrict__ g1, const double *__restrict__ g2,
-                                                     int N, int HP, int Np, int HPn, int m, double kf,
-                                                     float2 *__restrict__ o0, float2 *__restrict__ o1,
-                                                     float2 *__restrict__ o2) {
-  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (size_t)Np * Np * HPn) return;
-  const int lp = (int)(e % HPn);
-  const size_t row = e / HPn;
-  const int jp = (int)(row % Np), ip = (int)(row / Np);
-  double acc[6] = {0, 0, 0, 0, 0, 0}, X[6], Y[6];
-  if (lp <= Np / 2 && !(ip == 0 && jp == 0 && lp == 0)) {
-    for (int a = 0; a < m; a++) {
-      const int i = ip + a * Np, mi = i == 0 ? 0 : N - i;
-      for (int b = 0; b < m; b++) {
-        const int j = jp + b * Np, mj = j == 0 ? 0 : N - j;
-        for (int c = 0; c < m; c++) {
-          const int l = lp + c * Np;
-          if (l > 0 && l < N / 2) {
-            lpt_x(i, j, l, N, HP, Np, kf, cd1, cd2, g1, g2, X);
+// consecutive threads take consecutive kz, so every alias read is a coalesced
+// row segment.  The folded DC mode is zeroed, which removes the lattice mean
+// exactly (src/2LPT.c:1719-1729).
+// One folded mode (ip, jp, lp), lp <= Np/2: the sum over its m^3 aliases.  MF > 0
+// is the fold factor m = N / Nsample as a compile-time constant, so the alias
+// loop unrolls and all its loads are in flight together; MF = 0 takes m at run time.
+template <int MF, class G>
+__device__ __forceinline__ void fold_mode(int ip, int jp, int lp, int N, int HP, int Np, int m_rt, float ikf,
+                                          const float2 *__restrict__ cd1, const float2 *__restrict__ cd2, const G &gr,
+                                          float acc[6]) {
+  const int m = MF > 0 ? MF : m_rt;
+  float X[6], Y[6];
 #pragma unroll
-            for (int q = 0; q < 6; q++) acc[q] += X[q];
-          } else if (l > N / 2) {
-            lpt_x(mi, mj, N - l, N, HP, Np, kf, cd1, cd2, g1, g2, X);
+  for (int q = 0; q < 6; q++) acc[q] = 0.f;
+  if (ip == 0 && jp == 0 && lp == 0) return;
 #pragma unroll
-            for (int q = 0; q < 3; q++) {
-              acc[2 * q] += X[2 * q];
-              acc[2 * q + 1] -= X[2 * q + 1];
-            }
-          } else {  // kz = 0 or N/2 plane: (X(k) + conj X(-k)) / 2
-            lpt_x(i, j, l, N, HP, Np, kf, cd1, cd2, g1, g2, X);
-            lpt_x(mi, mj, l, N, HP, Np, kf, cd1, cd2, g1, g2, Y);
+  for (int a = 0; a < m; a++) {
+    const int i = ip + a * Np, mi = i == 0 ? 0 : N - i;
 #pragma unroll
-            for (int q = 0; q < 3; q++) {
-              acc[2 * q] += 0.5 * (X[2 * q] + Y[2 * q]);
-              acc[2 * q + 1] += 0.5 * (X[2 * q + 1] - Y[2 * q + 1]);
-            }
+    for (int b = 0; b < m; b++) {
+      const int j = jp + b * Np, mj = j == 0 ? 0 : N - j;
+#pragma unroll
+      for (int c = 0; c < m; c++) {
+        const int l = lp + c * Np;
+        if (l > 0 && l < N / 2) {
+          lpt_xf(i, j, l, N, HP, Np, ikf, cd1, cd2, gr, X);
+#pragma unroll
+          for (int q = 0; q < 6; q++) acc[q] += X[q];
+        } else if (l > N / 2) {
+          lpt_xf(mi, mj, N - l, N, HP, Np, ikf, cd1, cd2, gr, X);
+#pragma unroll
+          for (int q = 0; q < 3; q++) {
+            acc[2 * q] += X[2 * q];
+            acc[2 * q + 1] -= X[2 * q + 1];
+          }
+        } else {  // kz = 0 or N/2 plane: (X(k) + conj X(-k)) / 2
+          lpt_xf(i, j, l, N, HP, Np, ikf, cd1, cd2, gr, X);
+          lpt_xf(mi, mj, l, N, HP, Np, ikf, cd1, cd2, gr, Y);
+#pragma unroll
+          for (int q = 0; q < 3; q++) {
+            acc[2 * q] += 0.5f * (X[2 * q] + Y[2 * q]);
+            acc[2 * q + 1] += 0.5f * (X[2 * q + 1] - Y[2 * q + 1]);
           }
         }
       }
     }
   }
-  o0[e] = make_float2((float)acc[0], (float)acc[1]);
-  o1[e] = make_float2((float)acc[2], (float)acc[3]);
-  o2[e] = make_float2((float)acc[4], (float)acc[5]);
+}
+
+// growth from per-|n|^2 tables; grid (Nsample^2 rows, kz blocks)
+template <int MF>
+__global__ __launch_bounds__(kBlock) void k_lpt_fold(const float2 *__restrict__ cd1, const float2 *__restrict__ cd2,
+                                                     GTab gr, int N, int HP, int Np, int HPn, int m_rt, float ikf,
+                                                     float2 *__restrict__ o0, float2 *__restrict__ o1,
+                                                     float2 *__restrict__ o2) {
+  const int lp = blockIdx.y * blockDim.x + threadIdx.x;
+  if (lp > Np / 2) return;
+  const unsigned row = blockIdx.x;
+  const int jp = (int)(row % (unsigned)Np), ip = (int)(row / (unsigned)Np);
+  float acc[6];
+  fold_mode<MF>(ip, jp, lp, N, HP, Np, m_rt, ikf, cd1, cd2, gr, acc);
+  const size_t e = (size_t)row * HPn + lp;
+  o0[e] = make_float2(acc[0], acc[1]);
+  o1[e] = make_float2(acc[2], acc[3]);
+  o2[e] = make_float2(acc[4], acc[5]);
+}
+
+// growth from the ln k slices, staged once per block in LDS; the blocks stride
+// over the Nsample^2 (Nsample/2 + 1) folded modes
+template <int MF>
+__global__ __launch_bounds__(kBlock) void k_lpt_fold_sl(const float2 *__restrict__ cd1,
+                                                        const float2 *__restrict__ cd2,
+                                                        const double *__restrict__ slices, KSlf s1, KSlf s2,
+                                                        float lnkf, int N, int HP, int Np, int HPn, int m_rt,
+                                                        float ikf, float2 *__restrict__ o0, float2 *__restrict__ o1,
+                                                        float2 *__restrict__ o2) {
+  extern __shared__ float sl[];
+  const int nsl = 4 * (s1.ny + s2.ny);
+  for (int e = threadIdx.x; e < nsl; e += blockDim.x) sl[e] = (float)slices[e];
+  __syncthreads();
+  const GSlice gr{sl, s1, s2, lnkf};
+  const unsigned K = Np / 2 + 1;
+  const size_t total = (size_t)Np * Np * K;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const unsigned row = (unsigned)(e / K), lp = (unsigned)(e - (size_t)row * K);
+    const int jp = (int)(row % (unsigned)Np), ip = (int)(row / (unsigned)Np);
+    float acc[6];
+    fold_mode<MF>(ip, jp, (int)lp, N, HP, Np, m_rt, ikf, cd1, cd2, gr, acc);
+    const size_t o = (size_t)row * HPn + lp;
+    o0[o] = make_float2(acc[0], acc[1]);
+    o1[o] = make_float2(acc[2], acc[3]);
+    o2[o] = make_float2(acc[4], acc[5]);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_d2f(size_t n, const double *__restrict__ a, float *__restrict__ b) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
 }
 
 // g_o(|n|^2) = norm_o num_o(ln k) / den_o(ln k) from cubic Hermite slices on a
@@ -346,16 +435,20 @@ __device__ __forceinline__ double herm(const double *__restrict__ v, const doubl
 }
 __global__ __launch_bounds__(kBlock) void k_growth_n2(size_t n2len, double kf, KSl s1, const double *__restrict__ t1,
                                                       KSl s2, const double *__restrict__ t2, double *__restrict__ g1,
-                                                      double *__restrict__ g2) {
+                                                      double *__restrict__ g2, float *__restrict__ g1f,
+                                                      float *__restrict__ g2f) {
   const size_t n2 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n2 >= n2len) return;
-  if (n2 == 0) {
-    g1[0] = g2[0] = 0.0;
-    return;
+  double a = 0.0, b = 0.0;
+  if (n2 > 0) {
+    const double lk = log(kf * sqrt((double)n2));
+    a = s1.norm * (herm(t1, t1 + s1.ny, s1, lk) / herm(t1 + 2 * s1.ny, t1 + 3 * s1.ny, s1, lk));
+    b = s2.norm * (herm(t2, t2 + s2.ny, s2, lk) / herm(t2 + 2 * s2.ny, t2 + 3 * s2.ny, s2, lk));
   }
-  const double lk = log(kf * sqrt((double)n2));
-  g1[n2] = s1.norm * (herm(t1, t1 + s1.ny, s1, lk) / herm(t1 + 2 * s1.ny, t1 + 3 * s1.ny, s1, lk));
-  g2[n2] = s2.norm * (herm(t2, t2 + s2.ny, s2, lk) / herm(t2 + 2 * s2.ny, t2 + 3 * s2.ny, s2, lk));
+  g1[n2] = a;
+  g2[n2] = b;
+  g1f[n2] = (float)a;
+  g2f[n2] = (float)b;
 }
 
 // seed table on the host (a single serial stream), src/2LPT.c:259-271
@@ -409,6 +502,9 @@ int readout(mgpu_ctx *ctx, const float2 *mesh, double scale, float *dst) {
 
 int upload_n2(mgpu_ctx *ctx, int which, const double *h) {
   MGPU_HIP(ctx, hipMemcpyAsync(ctx->n2tab[which], h, ctx->n2len * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_d2f, dim3(cdiv(ctx->n2len, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->n2len,
+                     (const double *)ctx->n2tab[which], ctx->n2tabf[which]);
+  MGPU_HIP(ctx, hipGetLastError());
   return MGPU_OK;
 }
 
@@ -483,20 +579,39 @@ int ic_init_particles(mgpu_ctx *ctx, double Di, double Di2) {
 }
 
 // LPT fields from the device growth tables n2tab[0], n2tab[1]
-static int lpt_fields_from_tables(mgpu_ctx *ctx, int slot) {
+// With sl1/sl2 (growth slices already in ctx->slice_buf) the lattice fold
+// evaluates the growth from them in LDS; otherwise from the n2 tables.
+static int lpt_fields_from_tables(mgpu_ctx *ctx, int slot, const KSl *sl1 = nullptr, const KSl *sl2 = nullptr) {
   int rc;
   if (ctx->lattice_slot) {
     // exact lattice fold: N = m * Nsample
     const int Np = ctx->Np, HPn = ctx->fft_lattice.hp, m = ctx->N / Np;
     const size_t nl = (size_t)Np * Np * HPn;
     float2 *o[3] = {ctx->lattice_slot, ctx->lattice_slot + nl, ctx->lattice_slot + 2 * nl};
+    const float ikf = (float)(ctx->p.box / (2 * MGPU_PI));
+    const size_t slds = sl1 ? 4 * sizeof(float) * (size_t)(sl1->ny + sl2->ny) : 0;
     {
       // compulsory: both stored spectra read once (the 1LPT one inside the IC cube only), 3 folded outputs written
       const double modes = (double)ctx->N * ctx->N * (ctx->N / 2 + 1);
       KScope ks(ctx, MGPU_K_LPT, 8.0 * modes * (1.0 + 1.0 / (double)(m * m * m)) + 3.0 * 8.0 * nl);
-      hipLaunchKernelGGL(k_lpt_fold, dim3(cdiv(nl, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->cdelta[0],
-                         ctx->cdelta[1], ctx->n2tab[0], ctx->n2tab[1], ctx->N, ctx->HP, Np, HPn, m,
-                         2 * MGPU_PI / ctx->p.box, o[0], o[1], o[2]);
+      if (sl1 && slds <= 65536) {
+        auto kern = m == 1   ? k_lpt_fold_sl<1>
+                    : m == 2 ? k_lpt_fold_sl<2>
+                    : m == 3 ? k_lpt_fold_sl<3>
+                             : k_lpt_fold_sl<0>;
+        const KSlf f1{(float)sl1->y0, (float)sl1->dy, (float)sl1->norm, sl1->ny};
+        const KSlf f2{(float)sl2->y0, (float)sl2->dy, (float)sl2->norm, sl2->ny};
+        const size_t total = (size_t)Np * Np * (Np / 2 + 1);
+        const unsigned nb = (unsigned)std::min<size_t>(cdiv(total, kBlock), 8 * 256);
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), slds, ctx->stream, ctx->cdelta[0], ctx->cdelta[1],
+                           (const double *)ctx->slice_buf, f1, f2, (float)std::log(2 * MGPU_PI / ctx->p.box), ctx->N,
+                           ctx->HP, Np, HPn, m, ikf, o[0], o[1], o[2]);
+      } else {
+        auto kern = m == 1 ? k_lpt_fold<1> : m == 2 ? k_lpt_fold<2> : m == 3 ? k_lpt_fold<3> : k_lpt_fold<0>;
+        hipLaunchKernelGGL(kern, dim3((unsigned)((size_t)Np * Np), cdiv(Np / 2 + 1, kBlock)), dim3(kBlock), 0,
+                           ctx->stream, ctx->cdelta[0], ctx->cdelta[1], GTab{ctx->n2tabf[0], ctx->n2tabf[1]}, ctx->N,
+                           ctx->HP, Np, HPn, m, ikf, o[0], o[1], o[2]);
+      }
     }
     MGPU_HIP(ctx, hipGetLastError());
     for (int a = 0; a < 3; a++)
@@ -535,11 +650,15 @@ int ic_lpt_fields_slices(mgpu_ctx *ctx, int slot, const mgpu_kslice *o1, const m
   MGPU_HIP(ctx, hipMemcpyAsync(ctx->slice_buf, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
                                ctx->stream));
   KSl k1{o1->lnk0, o1->dlnk, o1->norm, o1->nk}, k2{o2->lnk0, o2->dlnk, o2->norm, o2->nk};
-  hipLaunchKernelGGL(k_growth_n2, dim3(cdiv(ctx->n2len, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->n2len,
-                     2 * MGPU_PI / ctx->p.box, k1, (const double *)ctx->slice_buf, k2,
-                     (const double *)ctx->slice_buf + 4 * o1->nk, ctx->n2tab[0], ctx->n2tab[1]);
-  MGPU_HIP(ctx, hipGetLastError());
-  int rc = lpt_fields_from_tables(ctx, slot);
+  const bool lds_slices = ctx->lattice_slot && 4 * sizeof(float) * (size_t)(o1->nk + o2->nk) <= 65536;
+  if (!lds_slices) {
+    hipLaunchKernelGGL(k_growth_n2, dim3(cdiv(ctx->n2len, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->n2len,
+                       2 * MGPU_PI / ctx->p.box, k1, (const double *)ctx->slice_buf, k2,
+                       (const double *)ctx->slice_buf + 4 * o1->nk, ctx->n2tab[0], ctx->n2tab[1], ctx->n2tabf[0],
+                       ctx->n2tabf[1]);
+    MGPU_HIP(ctx, hipGetLastError());
+  }
+  int rc = lds_slices ? lpt_fields_from_tables(ctx, slot, &k1, &k2) : lpt_fields_from_tables(ctx, slot);
   if (rc) return rc;
   // the host buffer h must outlive the async copy
   MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));

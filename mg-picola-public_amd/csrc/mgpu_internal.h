@@ -45,12 +45,14 @@ struct mgpu_ctx {
   float *pos = nullptr, *vel = nullptr, *disp = nullptr;
   float *lpt[2] = {nullptr, nullptr};
 
-  // mesh slots: 0 delta(k)   1 delta(x) copy / effective density   2 scratch spectrum
-  //             3..5 force components (3..4 double as the 64-bit deposit accumulator)
+  // mesh slots: 0 delta(x) -> delta(k)   1 delta(x) copy (MG)   2 MG potential work, phi(k)
+  //             3..5 force components (x in 3, y in 5, z in 4; deposit accumulator before Forces)
   float2 *slot[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-  unsigned long long *acc = nullptr;  // aliases slot[3]..slot[4]
   float2 *cdelta[2] = {nullptr, nullptr};
   bool have_phik = false;
+  int dx_slot = 0;                  // slot holding delta(x) (1 when the MG screening needs it kept)
+  int phi_slot = 2;                 // slot holding phi(k) after the fifth force
+  int force_slot[3] = {3, 5, 4};    // slots of the x, y, z force meshes
 
   // reductions
   double *partial = nullptr;  // per-block partial sums
@@ -58,6 +60,7 @@ struct mgpu_ctx {
   double *dsum = nullptr;     // final sums
   double *pk = nullptr;       // 3 * nbins
   double *n2tab[2] = {nullptr, nullptr};  // per-|n|^2 device tables
+  float *n2tabf[2] = {nullptr, nullptr};  // FP32 copies of n2tab (lattice fold)
   size_t n2len = 0;
 
   FFTPlan fft;
@@ -122,19 +125,41 @@ int fft_plan_init(mgpu_ctx *ctx, FFTPlan *plan, int n, int tcols = 16);
 void fft_plan_free(FFTPlan *plan);
 bool fft_size_supported(int n);
 // in-place 3D transforms on a slot with the padded layout above
-int fft_r2c_3d(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf);
+// in != nullptr: the real input is read from in (same padded layout) and buf only written
+int fft_r2c_3d(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf, const float *in = nullptr);
 // c2r; with out != nullptr the real result goes to a compact [N][N][N] array
 int fft_c2r_3d(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf, float *out = nullptr);
 // individual passes (used by fused paths and tests)
-int fft_pass_z(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf, int forward, float *out = nullptr);
+int fft_pass_z(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf, int forward, float *out = nullptr,
+               const float *in = nullptr);
 int fft_pass_col(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf, int axis, int forward);
+
+// z c2r of the force's z component: X(k) -> i kz X(k) before the transform (in place)
+int fft_pass_z_ikz(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf);
+
+// ---------------------------------------------------------------- fused spectral passes (fused.hip)
+enum { FUSED_GREEN_FORCE = 0, FUSED_GREEN_LAPLACE = 1, FUSED_GREEN_SMOOTH = 2 };
+// inverse x pass of (a [+ b]) times the Green's function:
+//   FORCE    o0 = X[i n_x G], o1 = X[G], G = S (-p0 / n^2)           (p0 = 1/(N^3 2pi/Box))
+//   LAPLACE  o0 = X[S (-p0 / n^2)]                                   (p0 = DivideByLaplacian norm)
+//   SMOOTH   o0 = X[S W(|n| p1) p0]                                  (p0 = 1/N^3, p1 = 2 pi R/Box)
+int fused_green_x(mgpu_ctx *ctx, int mode, const float2 *a, const float2 *b, float2 *o0, float2 *o1, double p0,
+                  double p1, int filter);
+// inverse y pass: buf <- Y[buf], oy <- Y[i n_y buf]
+int fused_dual_y(mgpu_ctx *ctx, float2 *buf, float2 *oy);
+// forward x pass then times coupling n^2/(n^2 + massterm2)
+int fused_epi_phi_x(mgpu_ctx *ctx, float2 *buf, double coupling, double massterm2);
+// z pass: c2r, screening against delta(x) (dx, real layout), r2c; in place on buf
+int fused_z_screen(mgpu_ctx *ctx, int density_kind, float2 *buf, const float *dx, double p0, double p1);
 
 // ---------------------------------------------------------------- tiled deposit (deposit.hip)
 size_t deposit_scratch_bytes(int N, size_t np);
-int deposit_tiled(mgpu_ctx *ctx);  // particles -> 64-bit fixed-point accumulator ctx->acc
+// particles -> delta(x) in real layout in d, through the 64-bit fixed-point
+// accumulator acc (8 N^3 bytes, free scratch)
+int deposit_tiled(mgpu_ctx *ctx, unsigned long long *acc, float *d);
 
 // ---------------------------------------------------------------- PM kernels (pm.hip)
-int pm_deposit(mgpu_ctx *ctx);                         // particles -> acc -> delta(x) in slot0 (+ copy slot1)
+int pm_deposit(mgpu_ctx *ctx);                         // particles -> delta(x) in slot0 (+ copy slot1)
 int pm_pofk(mgpu_ctx *ctx, const float2 *dk, mgpu_pofk *out);
 int pm_fifth_force(mgpu_ctx *ctx, const mgpu_mg *mg);
 int pm_forces(mgpu_ctx *ctx);

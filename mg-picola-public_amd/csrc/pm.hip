@@ -9,7 +9,6 @@
 
 namespace {
 
-constexpr double kInvFix = 1.0 / 4294967296.0;  // 2^-32: fixed-point scale of the deposit (deposit.hip)
 constexpr int kBlock = 256;
 
 // Cell index and fractional offset of a scaled coordinate X = pos * N / Box,
@@ -29,19 +28,6 @@ __device__ __forceinline__ void cell_of(double X, int N, unsigned &I, double &D)
 // density = -1 + sum of weights (src/auxPM.c:292-293), written as FP32 into
 // the padded real layout of slot0 and, when the MG solver needs delta(x)
 // (CopyDensityArray, src/mg.h:381-386), also into slot1.
-__global__ __launch_bounds__(kBlock) void k_acc_to_density(const unsigned long long *__restrict__ acc, int N, int HP,
-                                                           float *__restrict__ d0, float *__restrict__ d1) {
-  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t HP2 = 2 * (size_t)HP;
-  size_t total = (size_t)N * N * HP2;
-  if (e >= total) return;
-  size_t row = e / HP2, iz = e % HP2;
-  float v = 0.f;
-  if (iz < (size_t)N) v = (float)((double)acc[row * N + iz] * kInvFix - 1.0);
-  d0[e] = v;
-  if (d1) d1[e] = v;
-}
-
 // ------------------------------------------------------------------ P(k)
 // compute_power_spectrum (src/compute_pofk.c:71-236): every mode of the half
 // spectrum with weight 1 (kz = 0, N/2) or 2, |n| binned linearly or
@@ -85,54 +71,9 @@ __device__ __forceinline__ int n2_of(int i, int j, int k, int N) {
   return a * a + b * b + k * k;
 }
 
-// DivideByLaplacian (src/mg.h:21-64): out = delta(k) * nf * (-1/n^2), out(0) = 0
-__global__ __launch_bounds__(kBlock) void k_divide_laplacian(const float2 *__restrict__ in, float2 *__restrict__ out,
-                                                             int N, int HP, double nf) {
-  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (size_t)N * N * HP) return;
-  const int k = (int)(e % HP);
-  const size_t row = e / HP;
-  const int j = (int)(row % N), i = (int)(row / N);
-  float2 r = make_float2(0.f, 0.f);
-  if (k <= N / 2) {
-    const int n2 = n2_of(i, j, k, N);
-    if (n2 > 0) {
-      const double f = nf * (-1.0 / (double)n2);
-      const float2 v = in[e];
-      r = make_float2((float)(v.x * f), (float)(v.y * f));
-    }
-  }
-  out[e] = r;
-}
-
-// SmoothDensityField (src/mg.h:268-309): out = delta(k) * W(|k| R) / N^3
-__global__ __launch_bounds__(kBlock) void k_smooth(const float2 *__restrict__ in, float2 *__restrict__ out, int N,
-                                                   int HP, int filter, double kfac, double nf) {
-  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (size_t)N * N * HP) return;
-  const int k = (int)(e % HP);
-  const size_t row = e / HP;
-  const int j = (int)(row % N), i = (int)(row / N);
-  float2 r = make_float2(0.f, 0.f);
-  if (k <= N / 2) {
-    const double kR = sqrt((double)n2_of(i, j, k, N)) * kfac;
-    double W;
-    switch (filter) {
-      case MGPU_FILTER_GAUSSIAN: W = exp(-0.5 * kR * kR); break;
-      case MGPU_FILTER_TOPHAT: W = kR < 1e-5 ? 1.0 : 3.0 / (kR * kR * kR) * (sin(kR) - kR * cos(kR)); break;
-      case MGPU_FILTER_SHARPK: W = kR < 1.0 ? 1.0 : 0.0; break;
-      default: W = 1.0;
-    }
-    const double f = W * nf;
-    const float2 v = in[e];
-    r = make_float2((float)(v.x * f), (float)(v.y * f));
-  }
-  out[e] = r;
-}
-
 // EffDensitykToPhiofk (src/mg.h:74-116): out = coupling * n^2/(n^2 + massterm2) * in, out(0) = 0
-__global__ __launch_bounds__(kBlock) void k_eff_to_phi(float2 *__restrict__ buf, int N, int HP, double coupling,
-                                                       double massterm2) {
+__global__ __launch_bounds__(kBlock) void k_eff_to_phi(const float2 *__restrict__ in, float2 *__restrict__ out, int N,
+                                                       int HP, double coupling, double massterm2) {
   size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (size_t)N * N * HP) return;
   const int k = (int)(e % HP);
@@ -144,11 +85,11 @@ __global__ __launch_bounds__(kBlock) void k_eff_to_phi(float2 *__restrict__ buf,
     if (n2 > 0) {
       const double RK = (double)n2;
       const double f = coupling * (RK / (RK + massterm2));
-      const float2 v = buf[e];
+      const float2 v = in[e];
       r = make_float2((float)(v.x * f), (float)(v.y * f));
     }
   }
-  buf[e] = r;
+  out[e] = r;
 }
 
 // out = in * s over the whole slot (DGP without screening, src/mg.h:212-217;
@@ -159,70 +100,6 @@ __global__ __launch_bounds__(kBlock) void k_scale(const float2 *__restrict__ in,
   if (e >= n) return;
   const float2 v = in[e];
   out[e] = make_float2((float)(v.x * s), (float)(v.y * s));
-}
-
-// real-space screening passes on padded real meshes:
-//   f(R)/m-beta (src/mg.h:169-171): dx *= screening_factor_potential(a, phi(x))
-//   DGP         (src/mg.h:235-244): dx *= coupling * screening_factor_density(a, delta_R(x))
-__global__ __launch_bounds__(kBlock) void k_screen_potential(float *__restrict__ dx, const float *__restrict__ phi,
-                                                             size_t n, int screening, double phi_crit) {
-  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  const double p = phi[e];
-  double s = 1.0;
-  if (screening && p < 0.0) {
-    s = fabs(phi_crit / p);
-    if (s > 1.0) s = 1.0;
-  }
-  dx[e] = (float)(dx[e] * s);
-}
-
-__global__ __launch_bounds__(kBlock) void k_screen_density(float *__restrict__ dx, const float *__restrict__ dR,
-                                                           size_t n, int screening, double coupling,
-                                                           double dgp_fac) {
-  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  double s = 1.0;
-  if (screening) {
-    const double fac = dgp_fac * (1.0 + (double)dR[e]);
-    s = fac < 1e-5 ? 1.0 : 2.0 * (sqrt(1.0 + fac) - 1.0) / fac;
-  }
-  dx[e] = (float)(dx[e] * (coupling * s));
-}
-
-// ------------------------------------------------------------------ Forces
-// src/auxPM.c:450-535: P = delta(k) (+ phi(k)); F_a = i n_a (-1/n^2) P / (N^3 2pi/Box),
-// n_x, n_y in (-N/2, N/2] (Nyquist +N/2), n_z in [0, N/2]; F(0) = 0.
-__global__ __launch_bounds__(kBlock) void k_forces(const float2 *__restrict__ dk, const float2 *__restrict__ phik,
-                                                   float2 *__restrict__ f0, float2 *__restrict__ f1,
-                                                   float2 *__restrict__ f2, int N, int HP, double cnorm) {
-  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (size_t)N * N * HP) return;
-  const int k = (int)(e % HP);
-  const size_t row = e / HP;
-  const int j = (int)(row % N), i = (int)(row / N);
-  float2 r0 = make_float2(0.f, 0.f), r1 = r0, r2 = r0;
-  if (k <= N / 2) {
-    const int dd0 = i > N / 2 ? i - N : i, dd1 = j > N / 2 ? j - N : j, dd2 = k;
-    const int rk = dd0 * dd0 + dd1 * dd1 + dd2 * dd2;
-    if (rk > 0) {
-      float2 P = dk[e];
-      if (phik) {
-        const float2 q = phik[e];
-        P.x += q.x;
-        P.y += q.y;
-      }
-      const double kk = -1.0 / (double)rk * cnorm;
-      // i * P * kk * dd
-      const double re = -(double)P.y * kk, im = (double)P.x * kk;
-      r0 = make_float2((float)(re * dd0), (float)(im * dd0));
-      r1 = make_float2((float)(re * dd1), (float)(im * dd1));
-      r2 = make_float2((float)(re * dd2), (float)(im * dd2));
-    }
-  }
-  f0[e] = r0;
-  f1[e] = r1;
-  f2[e] = r2;
 }
 
 // ------------------------------------------------------------------ gather
@@ -386,19 +263,12 @@ int pm_reduce3(mgpu_ctx *ctx, int nblocks, double out[3]) {
 
 int pm_deposit(mgpu_ctx *ctx) {
   PhaseTimer pt(ctx, &ctx->t.ptomesh);
-  const int N = ctx->N;
-  const size_t cells = (size_t)N * N * N;
-  int rc = deposit_tiled(ctx);
-  if (rc) return rc;
   const bool need_dx = ctx->p.mg_active && ctx->p.model != MGPU_BRANSDICKE && ctx->p.model != MGPU_LCDM;
-  const size_t total = (size_t)N * N * 2 * ctx->HP;
-  {
-    KScope ks(ctx, MGPU_K_DENSITY, 8.0 * cells + (need_dx ? 8.0 : 4.0) * cells);
-    hipLaunchKernelGGL(k_acc_to_density, dim3(cdiv(total, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->acc, N,
-                       ctx->HP, (float *)ctx->slot[0], need_dx ? (float *)ctx->slot[1] : nullptr);
-  }
-  MGPU_HIP(ctx, hipGetLastError());
-  return MGPU_OK;
+  // the force meshes (slots 3..5) are free until Forces: the accumulator uses 8 N^3 bytes of them.
+  // With MG screening delta(x) goes to slot 1, where it stays for the screening
+  // pass, and the r2c writes delta(k) to slot 0.
+  ctx->dx_slot = need_dx ? 1 : 0;
+  return deposit_tiled(ctx, (unsigned long long *)ctx->slot[3], (float *)ctx->slot[ctx->dx_slot]);
 }
 
 int pm_pofk(mgpu_ctx *ctx, const float2 *dk, mgpu_pofk *out) {
@@ -447,21 +317,26 @@ int pm_pofk(mgpu_ctx *ctx, const float2 *dk, mgpu_pofk *out) {
   return MGPU_OK;
 }
 
+// ComputeFifthForce (src/user_defined_functions.h:687-719).  With screening the
+// chain c2r -> real-space screening -> r2c runs as five passes over slot 2:
+// Green's/smoothing x pass, y pass, z screening pass, y pass, x pass with the
+// EffDensitykToPhiofk epilogue (f(R), m-beta) or plain (DGP).  phi(k) ends in slot 2.
 int pm_fifth_force(mgpu_ctx *ctx, const mgpu_mg *mg) {
   PhaseTimer pt(ctx, &ctx->t.modified_gravity);
   ctx->have_phik = false;
   if (!ctx->p.mg_active || ctx->p.model == MGPU_LCDM) return MGPU_OK;
   if (!mg) return mgpu_fail(ctx, MGPU_E_INVALID, "modified gravity active but no mgpu_mg coefficients given");
   const int N = ctx->N, HP = ctx->HP;
-  const size_t nc = ctx->slot_elems, nr = 2 * nc;
-  const unsigned gc = cdiv(nc, kBlock), gr = cdiv(nr, kBlock);
-  float2 *dk = ctx->slot[0], *dx = ctx->slot[1], *one = ctx->slot[2];
+  const size_t nc = ctx->slot_elems;
+  const unsigned gc = cdiv(nc, kBlock);
+  float2 *dk = ctx->slot[0], *phi = ctx->slot[2];
+  const float *dx = (const float *)ctx->slot[1];
+  ctx->phi_slot = 2;
   int rc;
-  // compulsory bytes of one spectral (read + write C) and one real-space pass (2 reads + 1 write R)
-  const double modes = (double)N * N * (N / 2 + 1), cells = (double)N * N * N;
-  const double cbytes = 16.0 * modes, rbytes = 12.0 * cells;
+  const double cbytes = 16.0 * (double)N * N * (N / 2 + 1);
   switch (ctx->p.model) {
     case MGPU_BRANSDICKE: {
+      // _TimeDepGeffModels (src/mg.h:127-137): P3D *= Geff(a)
       KScope ks(ctx, MGPU_K_SPECTRAL, cbytes);
       hipLaunchKernelGGL(k_scale, dim3(gc), dim3(kBlock), 0, ctx->stream, dk, dk, nc, mg->geff);
       MGPU_HIP(ctx, hipGetLastError());
@@ -471,54 +346,40 @@ int pm_fifth_force(mgpu_ctx *ctx, const mgpu_mg *mg) {
     case MGPU_MBETA: {
       const double massterm2 = mg->a * mg->a * mg->mass2 / std::pow(2.0 * MGPU_PI * MGPU_INVERSE_H0_MPCH / ctx->p.box, 2);
       if (ctx->p.include_screening) {
+        // _PotentialScreening (src/mg.h:147-189) with DivideByLaplacian (src/mg.h:21-64)
         double nf = 1.0 / std::pow((double)N, 3);
         nf *= 1.5 * ctx->p.omega / mg->a * std::pow(ctx->p.box / MGPU_INVERSE_H0_MPCH / (2.0 * MGPU_PI), 2);
-        {
-          KScope ks(ctx, MGPU_K_SPECTRAL, cbytes);
-          hipLaunchKernelGGL(k_divide_laplacian, dim3(gc), dim3(kBlock), 0, ctx->stream, dk, one, N, HP, nf);
-        }
-        MGPU_HIP(ctx, hipGetLastError());
-        if ((rc = fft_c2r_3d(ctx, &ctx->fft, one))) return rc;
-        {
-          KScope ks(ctx, MGPU_K_REAL, rbytes);
-          hipLaunchKernelGGL(k_screen_potential, dim3(gr), dim3(kBlock), 0, ctx->stream, (float *)dx,
-                             (const float *)one, nr, 1, mg->phi_crit);
-        }
-        MGPU_HIP(ctx, hipGetLastError());
-        if ((rc = fft_r2c_3d(ctx, &ctx->fft, dx))) return rc;
+        if ((rc = fused_green_x(ctx, FUSED_GREEN_LAPLACE, dk, nullptr, phi, nullptr, nf, 0.0, 0))) return rc;
+        if ((rc = fft_pass_col(ctx, &ctx->fft, phi, 1, 0))) return rc;
+        if ((rc = fused_z_screen(ctx, 0, phi, dx, mg->phi_crit, 0.0))) return rc;
+        if ((rc = fft_pass_col(ctx, &ctx->fft, phi, 1, 1))) return rc;
+        if ((rc = fused_epi_phi_x(ctx, phi, mg->coupling, massterm2))) return rc;
       } else {
         // the reference re-transforms its delta(x) copy (src/mg.h:153-156), which reproduces delta(k)
-        MGPU_HIP(ctx, hipMemcpyAsync(dx, dk, nc * sizeof(float2), hipMemcpyDeviceToDevice, ctx->stream));
-      }
-      {
         KScope ks(ctx, MGPU_K_SPECTRAL, cbytes);
-        hipLaunchKernelGGL(k_eff_to_phi, dim3(gc), dim3(kBlock), 0, ctx->stream, dx, N, HP, mg->coupling, massterm2);
+        hipLaunchKernelGGL(k_eff_to_phi, dim3(gc), dim3(kBlock), 0, ctx->stream, dk, phi, N, HP, mg->coupling,
+                           massterm2);
+        MGPU_HIP(ctx, hipGetLastError());
       }
-      MGPU_HIP(ctx, hipGetLastError());
       ctx->have_phik = true;
       return MGPU_OK;
     }
     case MGPU_DGP: {
       if (!ctx->p.include_screening) {
+        // src/mg.h:212-217: phi = delta(k) / (3 beta)
         KScope ks(ctx, MGPU_K_SPECTRAL, cbytes);
-        hipLaunchKernelGGL(k_scale, dim3(gc), dim3(kBlock), 0, ctx->stream, dk, dx, nc, mg->coupling);
+        hipLaunchKernelGGL(k_scale, dim3(gc), dim3(kBlock), 0, ctx->stream, dk, phi, nc, mg->coupling);
         MGPU_HIP(ctx, hipGetLastError());
       } else {
+        // _DensityScreening (src/mg.h:197-260) with SmoothDensityField (src/mg.h:268-309)
         const double nf = 1.0 / std::pow((double)N, 3);
-        {
-          KScope ks(ctx, MGPU_K_SPECTRAL, cbytes);
-          hipLaunchKernelGGL(k_smooth, dim3(gc), dim3(kBlock), 0, ctx->stream, dk, one, N, HP, ctx->p.filter,
-                             2.0 * MGPU_PI / ctx->p.box * mg->rsmooth, nf);
-        }
-        MGPU_HIP(ctx, hipGetLastError());
-        if ((rc = fft_c2r_3d(ctx, &ctx->fft, one))) return rc;
-        {
-          KScope ks(ctx, MGPU_K_REAL, rbytes);
-          hipLaunchKernelGGL(k_screen_density, dim3(gr), dim3(kBlock), 0, ctx->stream, (float *)dx,
-                             (const float *)one, nr, 1, mg->coupling, mg->dgp_fac);
-        }
-        MGPU_HIP(ctx, hipGetLastError());
-        if ((rc = fft_r2c_3d(ctx, &ctx->fft, dx))) return rc;
+        if ((rc = fused_green_x(ctx, FUSED_GREEN_SMOOTH, dk, nullptr, phi, nullptr, nf,
+                                2.0 * MGPU_PI / ctx->p.box * mg->rsmooth, ctx->p.filter)))
+          return rc;
+        if ((rc = fft_pass_col(ctx, &ctx->fft, phi, 1, 0))) return rc;
+        if ((rc = fused_z_screen(ctx, 1, phi, dx, mg->coupling, mg->dgp_fac))) return rc;
+        if ((rc = fft_pass_col(ctx, &ctx->fft, phi, 1, 1))) return rc;
+        if ((rc = fft_pass_col(ctx, &ctx->fft, phi, 0, 1))) return rc;
       }
       ctx->have_phik = true;
       return MGPU_OK;
@@ -528,22 +389,26 @@ int pm_fifth_force(mgpu_ctx *ctx, const mgpu_mg *mg) {
   }
 }
 
+// Forces (src/auxPM.c:437-555): F_a = c2r(i n_a (-1/n^2) P / (N^3 2pi/Box)), P = delta(k) (+ phi(k)).
+// G = -P/(n^2 N^3 2pi/Box) is formed once in the x pass, which also writes the x
+// transform of i n_x G; one y pass yields the y transforms of G and i n_y G, and
+// the z component's i n_z is applied in its c2r.  Meshes: F_x slot 3, F_y slot 5, F_z slot 4.
 int pm_forces(mgpu_ctx *ctx) {
   PhaseTimer pt(ctx, &ctx->t.forces);
   const int N = ctx->N;
-  const double scale = 2. * M_PI / ctx->p.box;
-  const double cnorm = 1.0 / (std::pow((double)N, 3) * scale);
-  {
-    const double modes = (double)N * N * (N / 2 + 1);
-    KScope ks(ctx, MGPU_K_SPECTRAL, 8.0 * modes * ((ctx->have_phik ? 2 : 1) + 3));
-    hipLaunchKernelGGL(k_forces, dim3(cdiv(ctx->slot_elems, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->slot[0],
-                       ctx->have_phik ? ctx->slot[1] : nullptr, ctx->slot[3], ctx->slot[4], ctx->slot[5], N, ctx->HP,
-                       cnorm);
-  }
-  MGPU_HIP(ctx, hipGetLastError());
+  const double cnorm = 1.0 / (std::pow((double)N, 3) * (2. * M_PI / ctx->p.box));
   int rc;
-  for (int a = 0; a < 3; a++)
-    if ((rc = fft_c2r_3d(ctx, &ctx->fft, ctx->slot[3 + a]))) return rc;
+  if ((rc = fused_green_x(ctx, FUSED_GREEN_FORCE, ctx->slot[0], ctx->have_phik ? ctx->slot[ctx->phi_slot] : nullptr,
+                          ctx->slot[3], ctx->slot[4], cnorm, 0.0, 0)))
+    return rc;
+  if ((rc = fused_dual_y(ctx, ctx->slot[4], ctx->slot[5]))) return rc;
+  if ((rc = fft_pass_col(ctx, &ctx->fft, ctx->slot[3], 1, 0))) return rc;
+  if ((rc = fft_pass_z(ctx, &ctx->fft, ctx->slot[3], 0, nullptr))) return rc;
+  if ((rc = fft_pass_z(ctx, &ctx->fft, ctx->slot[5], 0, nullptr))) return rc;
+  if ((rc = fft_pass_z_ikz(ctx, &ctx->fft, ctx->slot[4]))) return rc;
+  ctx->force_slot[0] = 3;
+  ctx->force_slot[1] = 5;
+  ctx->force_slot[2] = 4;
   return MGPU_OK;
 }
 
@@ -559,8 +424,9 @@ int pm_gather(mgpu_ctx *ctx, double sumD[3]) {
     // compulsory: positions and Disp (12 B each per particle) and the three force meshes once
     KScope ks(ctx, MGPU_K_GATHER, 24.0 * ctx->np + 12.0 * (double)N * N * N);
     hipLaunchKernelGGL(k_gather, dim3(nb), dim3(kBlock), 0, ctx->stream, ctx->pos, ctx->np, N, 2 * ctx->HP,
-                       (double)N / ctx->p.box, (const float *)ctx->slot[3], (const float *)ctx->slot[4],
-                       (const float *)ctx->slot[5], ctx->disp, ctx->partial);
+                       (double)N / ctx->p.box, (const float *)ctx->slot[ctx->force_slot[0]],
+                       (const float *)ctx->slot[ctx->force_slot[1]], (const float *)ctx->slot[ctx->force_slot[2]],
+                       ctx->disp, ctx->partial);
   }
   MGPU_HIP(ctx, hipGetLastError());
   double s[3];

@@ -450,7 +450,7 @@ def run(hp, verbose=False):
 
 
 KERNELS = ["deposit", "density", "fft_r2c", "fft_c2r", "fft_col_fwd", "fft_col_inv", "pofk", "spectral", "real",
-           "gather", "kick", "drift", "lpt", "readout"]
+           "gather", "kick", "drift", "lpt", "readout", "fft_green", "fft_dual", "fft_screen", "fft_epi"]
 
 
 def _kt_setup():
