@@ -201,6 +201,19 @@ int mgpu_create(const mgpu_params *p, mgpu_ctx **out) {
   if ((rc = alloc(ctx, (void **)&ctx->dsum, 8 * sizeof(double), "sums"))) return fail(rc);
   const int nb = std::max(1, p->pofk_nbins);
   if ((rc = alloc(ctx, (void **)&ctx->pk, 3 * (size_t)nb * sizeof(double), "P(k) bins"))) return fail(rc);
+  {
+    // compute_power_spectrum's CIC deconvolution 1/(W_x W_y W_z)^4 with the 1/N^6 of the
+    // transform split over the three axes (src/compute_pofk.c:128-134)
+    std::vector<double> winv(ctx->N / 2 + 1);
+    const double n6 = 1.0 / std::pow((double)ctx->N, 6);
+    for (int d = 0; d <= ctx->N / 2; d++) {
+      const double g = d == 0 ? 1.0 : std::sin((MGPU_PI * d) / (double)ctx->N) / ((MGPU_PI * d) / (double)ctx->N);
+      winv[d] = 1.0 / std::pow(g, 4.0) * std::cbrt(n6);
+    }
+    if ((rc = alloc(ctx, (void **)&ctx->winv, winv.size() * sizeof(double), "P(k) window"))) return fail(rc);
+    if (hipMemcpy(ctx->winv, winv.data(), winv.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(mgpu_fail(ctx, MGPU_E_HIP, "window upload failed"));
+  }
   ctx->n2len = mgpu_n2_table_len(ctx->N);
   for (int s = 0; s < 2; s++) {
     if ((rc = alloc(ctx, (void **)&ctx->n2tab[s], ctx->n2len * sizeof(double), "n2 table"))) return fail(rc);
@@ -236,7 +249,7 @@ void mgpu_destroy(mgpu_ctx *ctx) {
                   ctx->slot[0],  ctx->slot[1],   ctx->slot[2],   ctx->slot[3],   ctx->cdelta[0],
                   ctx->cdelta[1], ctx->partial,  ctx->dsum,      ctx->pk,        ctx->n2tab[0],
                   ctx->n2tab[1], ctx->n2tabf[0], ctx->n2tabf[1], ctx->lattice_slot, ctx->slice_buf,
-                  ctx->dep_scratch};
+                  ctx->dep_scratch, ctx->winv};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   fft_plan_free(&ctx->fft);
@@ -307,15 +320,11 @@ int mgpu_pm_deposit(mgpu_ctx *ctx) {
 int mgpu_pm_transform(mgpu_ctx *ctx, mgpu_pofk *pk) {
   if (check_ctx(ctx)) return MGPU_E_INVALID;
   if (!ctx->have_deposit) return mgpu_fail(ctx, MGPU_E_STATE, "no deposited density: call mgpu_pm_deposit first");
-  int rc;
-  {
-    PhaseTimer pt(ctx, &ctx->t.ptomesh);
-    // delta(x) stays in slot dx_slot when it is needed later (MG screening)
-    if ((rc = fft_r2c_3d(ctx, &ctx->fft, ctx->slot[0], (const float *)ctx->slot[ctx->dx_slot]))) return rc;
-  }
+  // delta(x) stays in slot dx_slot when it is needed later (MG screening)
+  int rc = pm_transform(ctx, pk);
+  if (rc) return rc;
   ctx->have_deposit = false;
   ctx->have_density = true;
-  if (pk && (rc = pm_pofk(ctx, ctx->slot[0], pk))) return rc;
   return MGPU_OK;
 }
 

@@ -179,13 +179,31 @@ __global__ __launch_bounds__(kTB) void k_dep_tile(const float *__restrict__ pos,
   for (int e = threadIdx.x; e < nblk; e += blockDim.x) blk[e] = 0ull;
   __syncthreads();
   const unsigned p0 = off[t], p1 = off[t + 1];
-  for (unsigned p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-    const size_t i = idx[p];
+  // U particles per thread and round: all index loads, then all position loads,
+  // are in flight before the first deposit
+  constexpr int U = 4;
+  for (unsigned base = p0; base < p1; base += U * blockDim.x) {
+    unsigned ii[U];
+    float px[U], py[U], pz[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const unsigned q = base + u * blockDim.x + threadIdx.x;
+      ii[u] = idx[q < p1 ? q : p1 - 1];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      px[u] = pos[ii[u]];
+      py[u] = pos[np + ii[u]];
+      pz[u] = pos[2 * np + ii[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+    if (base + u * blockDim.x + threadIdx.x >= p1) break;
     unsigned IX, IY, IZ;
     double DX, DY, DZ;
-    cell_of((double)pos[i] * scale, g.N, IX, DX);
-    cell_of((double)pos[np + i] * scale, g.N, IY, DY);
-    cell_of((double)pos[2 * np + i] * scale, g.N, IZ, DZ);
+    cell_of((double)px[u] * scale, g.N, IX, DX);
+    cell_of((double)py[u] * scale, g.N, IY, DY);
+    cell_of((double)pz[u] * scale, g.N, IZ, DZ);
     const double TX = 1.0 - DX, TZ = 1.0 - DZ;
     const double TY = (1.0 - DY) * wpar;
     DY *= wpar;
@@ -202,6 +220,7 @@ __global__ __launch_bounds__(kTB) void k_dep_tile(const float *__restrict__ pos,
     DEP(b11, DX * DY * TZ);
     DEP(b11 + 1, DX * DY * DZ);
 #undef DEP
+    }
   }
   __syncthreads();
   const size_t N = g.N, nt = g.ntx;

@@ -161,6 +161,50 @@ __global__ __launch_bounds__(col_threads(L, T)) void k_col_epi(float2 *buf, size
   fft_stages<L, T, NT, -1, 1, T, true, false, 1>(lds, tw, 1, tid, ld, st);
 }
 
+// x pass, forward, accumulating compute_power_spectrum's binned power
+// (src/compute_pofk.c:71-236) from the transform's output as it is stored:
+// weight 1 (kz = 0, N/2) or 2, CIC deconvolution winv[d0] winv[d1] winv[d2]
+// (1/N^6 folded in), |n| binned linearly (bintype 0) or logarithmically.
+// Bin sums go to LDS doubles, then one global atomic per bin and block.
+template <int L, int T>
+__global__ __launch_bounds__(col_threads(L, T)) void k_col_pofk(float2 *buf, size_t ls, size_t os,
+                                                                  const float2 *__restrict__ tw,
+                                                                  const double *__restrict__ winv, int nbins,
+                                                                  int bintype, double kmin, double kmax,
+                                                                  double *__restrict__ pk) {
+  extern __shared__ float2 lds[];
+  constexpr int NT = col_threads(L, T);
+  constexpr int N = L;
+  double *bins = (double *)(lds + (size_t)L * T);
+  double *wv = bins + nbins;
+  const int tid = threadIdx.x;
+  for (int b = tid; b < nbins; b += NT) bins[b] = 0.0;
+  for (int d = tid; d <= N / 2; d += NT) wv[d] = winv[d];
+  __syncthreads();
+  const int ky = blockIdx.y, kz0 = blockIdx.x * T;
+  const size_t tile = (size_t)ky * os + kz0;
+  const int d1 = ky > N / 2 ? N - ky : ky;
+  const double lnrange = bintype == 1 ? log(kmax / kmin) : 1.0;
+  auto ld = [&](int c, int j, int o) { return buf[tile + (size_t)j * ls + c + (size_t)o * ls]; };
+  auto st = [&](int c, int j, int o, float2 v) {
+    buf[tile + (size_t)j * ls + c + (size_t)o * ls] = v;
+    const int kz = kz0 + c, kx = j + o;
+    if (kz > N / 2) return;
+    const int d0 = kx > N / 2 ? N - kx : kx;
+    const double kmag = sqrt((double)(d0 * d0 + d1 * d1 + kz * kz));
+    int nk;
+    if (bintype == 0) nk = (int)((kmag - kmin) / (kmax - kmin) * nbins + 0.5);
+    else nk = kmag <= 0.0 ? -1 : (int)(log(kmag / kmin) / lnrange * nbins + 0.5);
+    if (nk < 0 || nk >= nbins) return;
+    const double w = (kz == 0 || kz == N / 2) ? 1.0 : 2.0;
+    atomicAdd(&bins[nk], w * ((double)v.x * v.x + (double)v.y * v.y) * wv[d0] * wv[d1] * wv[kz]);
+  };
+  fft_stages<L, T, NT, -1, 1, T, true, false, 1>(lds, tw, 1, tid, ld, st);
+  __syncthreads();
+  for (int b = tid; b < nbins; b += NT)
+    if (bins[b] != 0.0) atomicAdd(&pk[b], bins[b]);
+}
+
 enum { SCREEN_POTENTIAL = 0, SCREEN_DENSITY = 1 };
 
 // z pass: c2r of the row (potential Phi_N or smoothed density delta_R), the
@@ -272,6 +316,20 @@ int epi_launch(mgpu_ctx *ctx, float2 *buf, float coupling, float m2) {
   return MGPU_OK;
 }
 
+template <int N, int T>
+int pofk_launch(mgpu_ctx *ctx, float2 *buf, const double *winv, int nbins, int bintype, double kmin, double kmax,
+                double *pk) {
+  const size_t HP = ctx->fft.hp, ls = (size_t)N * HP, os = HP;
+  const size_t lds = (size_t)N * T * sizeof(float2) + sizeof(double) * ((size_t)nbins + N / 2 + 1);
+  if (lds > ctx->lds_max) return mgpu_fail(ctx, MGPU_E_INVALID, "P(k) bins do not fit the LDS of the x pass");
+  int rc;
+  if ((rc = set_lds(ctx, k_col_pofk<N, T>, lds))) return rc;
+  hipLaunchKernelGGL((k_col_pofk<N, T>), dim3(HP / T, N), dim3(col_threads(N, T)), lds, ctx->stream, buf, ls, os,
+                     ctx->fft.tw, winv, nbins, bintype, kmin, kmax, pk);
+  MGPU_HIP(ctx, hipGetLastError());
+  return MGPU_OK;
+}
+
 template <int N>
 int screen_launch(mgpu_ctx *ctx, int kind, float2 *buf, const float2 *dx, float p0, float p1) {
   using C = RowCfg<N>;
@@ -309,6 +367,12 @@ struct GreenF {
 template <int N, int T>
 struct DualF {
   static int run(mgpu_ctx *c, float2 *buf, float2 *oy) { return dual_launch<N, T>(c, buf, oy); }
+};
+template <int N, int T>
+struct PofkF {
+  static int run(mgpu_ctx *c, float2 *buf, const double *w, int nb, int bt, double k0, double k1, double *pk) {
+    return pofk_launch<N, T>(c, buf, w, nb, bt, k0, k1, pk);
+  }
 };
 template <int N, int T>
 struct EpiF {
@@ -374,6 +438,21 @@ int fused_z_screen(mgpu_ctx *ctx, int density_kind, float2 *buf, const float *dx
 #define X(S) \
   case S:    \
     return screen_launch<S>(ctx, kind, buf, (const float2 *)dx, (float)p0, (float)p1);
+    MGPU_FFT_SIZES(X)
+#undef X
+    default:
+      return mgpu_fail(ctx, MGPU_E_INVALID, "unsupported FFT size");
+  }
+}
+
+int fused_pofk_x(mgpu_ctx *ctx, float2 *buf, const double *winv, int nbins, int bintype, double kmin, double kmax,
+                 double *pk) {
+  const double modes = (double)ctx->N * ctx->N * (ctx->N / 2 + 1);
+  KScope ks(ctx, MGPU_K_FFT_COL_FWD, 16.0 * modes);
+  switch (ctx->N) {
+#define X(S) \
+  case S:    \
+    return by_tile<S, PofkF>(ctx, buf, winv, nbins, bintype, kmin, kmax, pk);
     MGPU_FFT_SIZES(X)
 #undef X
     default:

@@ -59,6 +59,8 @@ struct mgpu_ctx {
   size_t partial_len = 0;
   double *dsum = nullptr;     // final sums
   double *pk = nullptr;       // 3 * nbins
+  double *winv = nullptr;     // P(k) deconvolution per axis: 1/sinc(pi d/N)^4 N^-2, d = 0..N/2
+  std::vector<double> pk_geom;  // mode-weighted |n| sums and mode counts per bin (2 * nbins), fixed per context
   double *n2tab[2] = {nullptr, nullptr};  // per-|n|^2 device tables
   float *n2tabf[2] = {nullptr, nullptr};  // FP32 copies of n2tab (lattice fold)
   size_t n2len = 0;
@@ -149,6 +151,9 @@ int fused_green_x(mgpu_ctx *ctx, int mode, const float2 *a, const float2 *b, flo
 int fused_dual_y(mgpu_ctx *ctx, float2 *buf, float2 *oy);
 // forward x pass then times coupling n^2/(n^2 + massterm2)
 int fused_epi_phi_x(mgpu_ctx *ctx, float2 *buf, double coupling, double massterm2);
+// forward x pass of the density r2c, accumulating the binned P(k) power into pk[nbins]
+int fused_pofk_x(mgpu_ctx *ctx, float2 *buf, const double *winv, int nbins, int bintype, double kmin, double kmax,
+                 double *pk);
 // z pass: c2r, screening against delta(x) (dx, real layout), r2c; in place on buf
 int fused_z_screen(mgpu_ctx *ctx, int density_kind, float2 *buf, const float *dx, double p0, double p1);
 
@@ -161,6 +166,7 @@ int deposit_tiled(mgpu_ctx *ctx, unsigned long long *acc, float *d);
 // ---------------------------------------------------------------- PM kernels (pm.hip)
 int pm_deposit(mgpu_ctx *ctx);                         // particles -> delta(x) in slot0 (+ copy slot1)
 int pm_pofk(mgpu_ctx *ctx, const float2 *dk, mgpu_pofk *out);
+int pm_transform(mgpu_ctx *ctx, mgpu_pofk *out);  // r2c of delta(x) into slot 0 (+ P(k))
 int pm_fifth_force(mgpu_ctx *ctx, const mgpu_mg *mg);
 int pm_forces(mgpu_ctx *ctx);
 int pm_gather(mgpu_ctx *ctx, double sumD[3]);

@@ -271,36 +271,26 @@ int pm_deposit(mgpu_ctx *ctx) {
   return deposit_tiled(ctx, (unsigned long long *)ctx->slot[3], (float *)ctx->slot[ctx->dx_slot]);
 }
 
-int pm_pofk(mgpu_ctx *ctx, const float2 *dk, mgpu_pofk *out) {
-  PhaseTimer pt(ctx, &ctx->t.pofk);
-  const int N = ctx->N, nbins = ctx->p.pofk_nbins;
+namespace {
+// the bin sums are cleared; the per-axis deconvolution table is ctx->winv
+int pofk_setup(mgpu_ctx *ctx, double **dwin) {
+  const int nbins = ctx->p.pofk_nbins;
   if (nbins <= 0) return mgpu_fail(ctx, MGPU_E_INVALID, "pofk_nbins must be positive");
-  // per-axis deconvolution 1/sinc(pi d/N)^4 and the 1/N^6 of the FFT normalisation
-  std::vector<double> winv(N / 2 + 1);
-  const double n6 = 1.0 / std::pow((double)N, 6);
-  for (int d = 0; d <= N / 2; d++) {
-    double g = d == 0 ? 1.0 : std::sin((MGPU_PI * d) / (double)N) / ((MGPU_PI * d) / (double)N);
-    winv[d] = 1.0 / std::pow(g, 4.0) * std::cbrt(n6);
-  }
-  double *dwin = ctx->partial;  // scratch: winv in the partial buffer tail
-  if (ctx->partial_len < (size_t)(N / 2 + 1)) return mgpu_fail(ctx, MGPU_E_INVALID, "scratch too small");
-  MGPU_HIP(ctx, hipMemcpyAsync(dwin, winv.data(), winv.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  *dwin = ctx->winv;
   MGPU_HIP(ctx, hipMemsetAsync(ctx->pk, 0, 3 * nbins * sizeof(double), ctx->stream));
-  const size_t shmem = 3 * (size_t)nbins * sizeof(double);
-  if (shmem > 65536) {
-    MGPU_HIP(ctx, hipFuncSetAttribute((const void *)k_pofk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
-  }
-  const size_t modes = (size_t)N * N * (N / 2 + 1);
-  unsigned grid = std::min<size_t>(cdiv(modes, kBlock), 4096);
-  {
-    KScope ks(ctx, MGPU_K_POFK, 8.0 * modes);
-    hipLaunchKernelGGL(k_pofk, dim3(grid), dim3(kBlock), shmem, ctx->stream, dk, N, ctx->HP, dwin, nbins,
-                       ctx->p.pofk_bintype, ctx->p.pofk_kmin, ctx->p.pofk_kmax, ctx->pk);
-  }
-  MGPU_HIP(ctx, hipGetLastError());
+  return MGPU_OK;
+}
+
+// normalisation of the bin sums (src/compute_pofk.c:200-236); with geometry the
+// |n| sums and mode counts come from ctx->pk_geom and only the power from the device
+int pofk_finish(mgpu_ctx *ctx, mgpu_pofk *out, bool geometry_from_ctx) {
+  const int nbins = ctx->p.pofk_nbins;
   std::vector<double> h(3 * nbins);
-  MGPU_HIP(ctx, hipMemcpyAsync(h.data(), ctx->pk, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  MGPU_HIP(ctx, hipMemcpyAsync(h.data(), ctx->pk, (geometry_from_ctx ? 1 : 3) * nbins * sizeof(double),
+                               hipMemcpyDeviceToHost, ctx->stream));
   MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (geometry_from_ctx) std::copy(ctx->pk_geom.begin(), ctx->pk_geom.end(), h.begin() + nbins);
+  else ctx->pk_geom.assign(h.begin() + nbins, h.end());
   const double box3 = std::pow(ctx->p.box, 3);
   const double shot = std::pow(ctx->p.box / (double)ctx->Np, 3);
   for (int b = 0; b < nbins; b++) {
@@ -315,6 +305,57 @@ int pm_pofk(mgpu_ctx *ctx, const float2 *dk, mgpu_pofk *out) {
     if (out->nmodes) out->nmodes[b] = n;
   }
   return MGPU_OK;
+}
+}  // namespace
+
+// compute_power_spectrum of the delta(k) in dk: power, |n| sums and mode counts
+int pm_pofk(mgpu_ctx *ctx, const float2 *dk, mgpu_pofk *out) {
+  PhaseTimer pt(ctx, &ctx->t.pofk);
+  const int N = ctx->N, nbins = ctx->p.pofk_nbins;
+  double *dwin;
+  int rc;
+  if ((rc = pofk_setup(ctx, &dwin))) return rc;
+  const size_t shmem = 3 * (size_t)nbins * sizeof(double);
+  if (shmem > 65536) {
+    MGPU_HIP(ctx, hipFuncSetAttribute((const void *)k_pofk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+  }
+  const size_t modes = (size_t)N * N * (N / 2 + 1);
+  unsigned grid = std::min<size_t>(cdiv(modes, kBlock), 4096);
+  {
+    KScope ks(ctx, MGPU_K_POFK, 8.0 * modes);
+    hipLaunchKernelGGL(k_pofk, dim3(grid), dim3(kBlock), shmem, ctx->stream, dk, N, ctx->HP, dwin, nbins,
+                       ctx->p.pofk_bintype, ctx->p.pofk_kmin, ctx->p.pofk_kmax, ctx->pk);
+  }
+  MGPU_HIP(ctx, hipGetLastError());
+  return pofk_finish(ctx, out, false);
+}
+
+// PtoMesh's r2c of delta(x) (slot dx_slot) into slot 0, with P(k) when out != NULL.
+// Once the bin geometry is known the power is binned inside the last (x) pass.
+int pm_transform(mgpu_ctx *ctx, mgpu_pofk *out) {
+  float2 *dk = ctx->slot[0];
+  const float *dx = (const float *)ctx->slot[ctx->dx_slot];
+  int rc;
+  if (!out || ctx->pk_geom.size() != 2 * (size_t)ctx->p.pofk_nbins) {
+    {
+      PhaseTimer pt(ctx, &ctx->t.ptomesh);
+      if ((rc = fft_r2c_3d(ctx, &ctx->fft, dk, dx))) return rc;
+    }
+    return out ? pm_pofk(ctx, dk, out) : MGPU_OK;
+  }
+  double *dwin;
+  if ((rc = pofk_setup(ctx, &dwin))) return rc;
+  {
+    PhaseTimer pt(ctx, &ctx->t.ptomesh);
+    PhaseTimer pf(ctx, &ctx->t.fft);
+    if ((rc = fft_pass_z(ctx, &ctx->fft, dk, 1, nullptr, dx))) return rc;
+    if ((rc = fft_pass_col(ctx, &ctx->fft, dk, 1, 1))) return rc;
+    if ((rc = fused_pofk_x(ctx, dk, dwin, ctx->p.pofk_nbins, ctx->p.pofk_bintype, ctx->p.pofk_kmin, ctx->p.pofk_kmax,
+                           ctx->pk)))
+      return rc;
+  }
+  PhaseTimer pt(ctx, &ctx->t.pofk);
+  return pofk_finish(ctx, out, true);
 }
 
 // ComputeFifthForce (src/user_defined_functions.h:687-719).  With screening the

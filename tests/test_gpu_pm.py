@@ -92,6 +92,24 @@ def test_pofk_matches_oracle(pos):
     np.testing.assert_allclose(pk[m], rp[m], rtol=1e-3, atol=1e-3 * np.abs(rp).max())
 
 
+@pytest.mark.parametrize("bintype,kmin,kmax", [(0, 0.0, float(N)), (1, 1.0, float(N))])
+def test_pofk_binned_in_transform_matches_oracle(pos, bintype, kmin, kmax):
+    """From the second call on the power is binned inside the r2c's last pass;
+    mode counts and mean |k| come from the first call's full binning."""
+    g = M.MGPU(N, NS, BOX, OM, pofk_nbins=NBINS, pofk_bintype=bintype, pofk_kmin=kmin, pofk_kmax=kmax)
+    g.set_particles(pos)
+    first = g.pm_density(pofk=True)
+    second = g.pm_density(pofk=True)
+    c = O.cfg(N, NS, BOX, OM)
+    d, _, _ = oracle_density(c, pos)
+    rp, rk, rn = O.pofk(c, d, NBINS, bintype, kmin, kmax, 1)
+    for pk, kk, nm in (first, second):
+        assert np.array_equal(nm, rn)
+        m = rn > 0
+        np.testing.assert_allclose(kk[m], rk[m], rtol=1e-12)
+        np.testing.assert_allclose(pk[m], rp[m], rtol=1e-3, atol=1e-3 * np.abs(rp).max())
+
+
 def test_forces_and_gather_lcdm(pos):
     g = M.MGPU(N, NS, BOX, OM)
     g.set_particles(pos)
