@@ -10,8 +10,10 @@ A "step" is one iteration of the C host's time loop (libmgpicola, the
 reference's src/main.c:405-610): PtoMesh + P(k), the f(R) fifth force with
 screening, Forces, MtoParticles, the two scale-dependent LPT fields, Kick and
 Drift, all on the GPU through libmgpu.  W warm-up steps, then K timed steps
-between device synchronisations.  One further step runs with per-kernel HIP
-event timing to give the roofline of the dominant kernel.
+between device synchronisations (defaults 1 + 29: the whole schedule, whose
+late clustered steps are the slow ones).  A further run of the whole schedule
+with per-kernel HIP event timing gives every kernel class's device time per
+step and the roofline of the dominant kernel.
 
 The CPU baseline is the FP64 oracle (a C restatement of the reference's step)
 timed on the host cores for one step of the same path on a bounded sample.
@@ -29,46 +31,85 @@ sys.path.insert(0, os.path.join(ROOT, "mg-picola-public_amd"))
 METRIC = "particle-steps/s at 512^3/Nmesh 1024; learned vs 3x-mesh P(k) err, k<1"
 HBM_PEAK_GBS = 8000.0
 
+# kernel-timing class -> the kernel (rocprofv3 name) it times, for the f(R)
+# scale-dependent workload's mesh size N and lattice size Ns
+KERNEL_SYMBOL = {
+    "deposit": "k_dep_tile", "mean": "k_sum3", "dep_sort": "k_dep_bin + k_dep_scatter",
+    "fft_r2c": "k_fft_z_r2c<{N}>", "fft_c2r": "k_fft_z_c2r<{N}, *>", "fft_col_fwd": "k_fft_col<{N}, 16, -1, 2>",
+    "fft_col_inv": "k_fft_col<{N}, 16, 1, 2>", "fft_green": "k_col_green<{N}, 16, 0>",
+    "fft_pot": "k_col_green<{N}, 16, 1>", "fft_dual": "k_col_dual<{N}, 16>", "fft_screen": "k_z_screen<{N}, 0>",
+    "fft_epi": "k_col_epi<{N}, 16>", "fft_pk": "k_col_pofk<{N}, 16>", "fold": "k_lpt_fold_gs<{m}, 2>",
+    "gather": "k_gather_tile<16>", "kick": "k_kick", "drift": "k_drift", "pofk": "k_pofk",
+    "lpt": "k_fft_col<{Ns}, ...> + k_fft_z_c2r<{Ns}, false>",
+}
+
 
 def gpu_bench(steps, warmup, workload):
+    """Time `steps` COLA steps after `warmup` untimed ones.  The workload's run
+    has a fixed schedule (30 steps z = 19 -> 0); when more steps are asked for
+    than one run holds, further runs start from fresh initial conditions (the
+    initial conditions themselves are never timed).  Then one more full run
+    with per-kernel HIP-event timing gives each kernel class's device time per
+    step, averaged over the whole schedule."""
     import mgpicola as M
     import workloads as W
 
     out = tempfile.mkdtemp(prefix="mgpicola_bench_")
     model, vals, nsteps = W.values_for(workload, out)
-    if warmup + steps + 1 > nsteps:
-        raise SystemExit(f"warmup + steps + 1 must be <= {nsteps} (the run has {nsteps} steps)")
     with open(vals["OutputRedshiftFile"], "w") as f:
         f.write(f"0, {nsteps}\n")
     pf = os.path.join(out, "params.txt")
     M.write_paramfile(pf, vals, model=model)
     hp = M.read_params(pf, model=model)
-    t0 = time.time()
-    sim = M.Sim(hp)
-    t_tables = time.time() - t0
-    t0 = time.time()
-    sim.ic()
-    sim.gpu.sync()
-    t_ic = time.time() - t0
-    print(f"[bench] tables {t_tables:.1f} s, initial conditions {t_ic:.1f} s", file=sys.stderr, flush=True)
+
+    def new_sim():
+        t0 = time.time()
+        sim = M.Sim(hp)
+        t1 = time.time()
+        sim.ic()
+        sim.gpu.sync()
+        print(f"[bench] tables {t1 - t0:.1f} s, initial conditions {time.time() - t1:.1f} s", file=sys.stderr,
+              flush=True)
+        return sim
+
+    sim = new_sim()
     for _ in range(warmup):
+        if sim.finished:
+            sim.close()
+            sim = new_sim()
         sim.advance()
-    sim.gpu.sync()
-    t0 = time.perf_counter()
-    for i in range(steps):
-        sim.advance()
-        print(f"[bench] step {i + 1}/{steps} a = {sim.a:.4f}", file=sys.stderr, flush=True)
-    sim.gpu.sync()
-    elapsed = time.perf_counter() - t0
-    # one more step with per-kernel event timing
+    timed, elapsed = 0, 0.0
+    while timed < steps:
+        if sim.finished:
+            sim.close()
+            sim = new_sim()
+        sim.gpu.sync()
+        t0 = time.perf_counter()
+        n = 0
+        while timed + n < steps and not sim.finished:
+            sim.advance()
+            n += 1
+        sim.gpu.sync()
+        elapsed += time.perf_counter() - t0
+        timed += n
+        print(f"[bench] {timed}/{steps} steps timed, a = {sim.a:.4f}", file=sys.stderr, flush=True)
+    sim.close()
+    # per-kernel event timing over one whole run
+    sim = new_sim()
     M.enable_kernel_timing(sim.gpu, True)
-    sim.advance()
+    M.kernel_timing(sim.gpu)  # reset (the initial conditions are not counted)
+    nrun = 0
+    while not sim.finished:
+        sim.advance()
+        nrun += 1
     sim.gpu.sync()
     kt = M.kernel_timing(sim.gpu)
     M.enable_kernel_timing(sim.gpu, False)
     sim.close()
+    kt = {k: (v[0] / nrun, v[1] / nrun, v[2] / nrun) for k, v in kt.items()}
     npart = hp.Nsample ** 3
-    return dict(elapsed=elapsed, npart=npart, nmesh=hp.Nmesh, box=hp.Box, kernels=kt, t_tables=t_tables, t_ic=t_ic)
+    return dict(elapsed=elapsed, npart=npart, nmesh=hp.Nmesh, nsample=hp.Nsample, box=hp.Box, kernels=kt,
+                steps_per_run=nsteps)
 
 
 def cpu_baseline(ns=256, nmesh=512, box=512.0):
@@ -108,28 +149,65 @@ def cpu_baseline(ns=256, nmesh=512, box=512.0):
                        f"{nmesh}^3 mesh ({dt:.1f} s)")
 
 
+def traffic_of(workload, symbol):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by the same counter's
+    reading for a plain device copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    k = t.get("kernels", {}).get(symbol)
+    return None if k is None else k.get("bytes_per_dispatch")
+
+
+def copy_probe(nbytes):
+    """A plain device-to-device copy of nbytes (and a fill of the source), to
+    calibrate the FETCH_SIZE / WRITE_SIZE counters of a profiling run."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        raise SystemExit("copy probe: hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(f"[bench] copy probe: {nbytes} bytes copied device to device", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=29)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="fofr_sd_512_1024")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first copy this many MiB device to device (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib << 20)
     r = gpu_bench(args.steps, args.warmup, args.workload)
     value = r["npart"] * args.steps / r["elapsed"]
     kt = r["kernels"]
-    dom = max(kt, key=lambda k: kt[k][1])
+    # the dominant kernel: the single-kernel class with the most device time in one step
+    single = {k: v for k, v in kt.items() if k not in ("lpt", "dep_sort", "fft_c2r")}
+    dom = max(single, key=lambda k: single[k][1])
     n, ms, by = kt[dom]
     achieved = (by / n) / (ms / n * 1e-3) / 1e9 if n and ms > 0 else 0.0
+    symbol = KERNEL_SYMBOL.get(dom, dom).format(N=r["nmesh"], Ns=r["nsample"], m=r["nmesh"] // r["nsample"])
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-            "launches_per_step": n, "ms_per_launch": round(ms / max(n, 1), 4)}
-    per_kernel = {k: {"launches": v[0], "ms": round(v[1], 3),
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_of(args.workload, symbol),
+            "kernel": symbol, "class": dom, "launches_per_step": round(n, 2), "ms_per_launch": round(ms / max(n, 1), 4),
+            "algorithmic_bytes_per_launch": round(by / max(n, 1))}
+    per_kernel = {k: {"launches": round(v[0], 2), "ms": round(v[1], 3),
                       "GBps": round(v[2] / (v[1] * 1e-3) / 1e9, 1) if v[1] > 0 else None}
                   for k, v in kt.items() if v[0]}
     cpu = None
@@ -142,7 +220,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": args.workload, "particles": r["npart"], "nmesh": r["nmesh"], "box": r["box"],
                    "model": "f(R) Hu-Sawicki fR0=1e-5 n=1, scale-dependent, screening, P(k) every step"},
-        "roofline": roof, "cpu_baseline": cpu, "kernels_one_step": per_kernel,
+        "roofline": roof, "cpu_baseline": cpu, "kernels_per_step": per_kernel,
     }
     print(json.dumps(line), flush=True)
 

@@ -176,6 +176,11 @@ typedef struct {
 } mgpu_kslice;
 /* mgpu_lpt_fields with the growth evaluated on the GPU from the two slices. */
 int mgpu_lpt_fields_slices(mgpu_ctx *ctx, int slot, const mgpu_kslice *o1, const mgpu_kslice *o2);
+/* Both LPT slots in one call: the four assign_displacment_field_to_particles
+ * calls that precede Kick (src/main.c:496-501: slot 0 from s0o1 + s0o2, slot 1
+ * from s1o1 + s1o2).  The stored spectra are read once for both slots. */
+int mgpu_lpt_fields_slices2(mgpu_ctx *ctx, const mgpu_kslice *s0o1, const mgpu_kslice *s0o2,
+                            const mgpu_kslice *s1o1, const mgpu_kslice *s1o2);
 
 /* ---- inspection (tests, output) ---- */
 int mgpu_get_mesh(mgpu_ctx *ctx, int mesh, double *host);
@@ -193,14 +198,18 @@ int mgpu_enable_timers(mgpu_ctx *ctx, int on);
  * events on the context's stream; mgpu_kernel_timing synchronises and reports
  * launches, summed device time and summed algorithmic bytes (compulsory HBM
  * reads + writes) since the last reset. */
-/* FFT classes: z-pass r2c / c2r, column (y and x) passes forward / inverse of
- * the Nmesh transforms (the Nsample-lattice transforms count as MGPU_K_LPT),
- * and the FFT passes with fused multipliers: Green's x pass, dual y pass,
- * z screening pass, x pass with the MG epilogue. */
+/* Classes (each one kernel or one kernel family): deposit tile accumulation,
+ * the mean-displacement sum in particle order, z-pass r2c / c2r, column (y and x) passes forward /
+ * inverse of the Nmesh transforms (the Nsample-lattice transforms count as
+ * MGPU_K_LPT), P(k), ..., the FFT passes with fused multipliers (Green's force
+ * x pass, dual y pass, z screening pass, x pass with the MG epilogue), the
+ * deposit's binning and sort, the lattice fold, the potential/smoothing x pass
+ * and the density x pass with P(k) binning. */
 enum {
-  MGPU_K_DEPOSIT = 0, MGPU_K_DENSITY, MGPU_K_FFT_R2C, MGPU_K_FFT_C2R, MGPU_K_FFT_COL_FWD, MGPU_K_FFT_COL_INV,
+  MGPU_K_DEPOSIT = 0, MGPU_K_MEAN, MGPU_K_FFT_R2C, MGPU_K_FFT_C2R, MGPU_K_FFT_COL_FWD, MGPU_K_FFT_COL_INV,
   MGPU_K_POFK, MGPU_K_SPECTRAL, MGPU_K_REAL, MGPU_K_GATHER, MGPU_K_KICK, MGPU_K_DRIFT, MGPU_K_LPT, MGPU_K_READOUT,
-  MGPU_K_FFT_GREEN, MGPU_K_FFT_DUAL, MGPU_K_FFT_SCREEN, MGPU_K_FFT_EPI, MGPU_K_COUNT
+  MGPU_K_FFT_GREEN, MGPU_K_FFT_DUAL, MGPU_K_FFT_SCREEN, MGPU_K_FFT_EPI, MGPU_K_DEP_SORT, MGPU_K_FOLD,
+  MGPU_K_FFT_POT, MGPU_K_FFT_PK, MGPU_K_COUNT
 };
 const char *mgpu_kernel_name(int kernel);
 int mgpu_enable_kernel_timing(mgpu_ctx *ctx, int on);

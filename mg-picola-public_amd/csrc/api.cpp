@@ -84,10 +84,11 @@ static int flush_ktimers(mgpu_ctx *ctx) {
 extern "C" {
 
 const char *mgpu_kernel_name(int k) {
-  static const char *names[MGPU_K_COUNT] = {"deposit",   "density",  "fft_r2c",    "fft_c2r", "fft_col_fwd",
-                                            "fft_col_inv", "pofk",   "spectral",   "real",    "gather",
-                                            "kick",      "drift",    "lpt",        "readout", "fft_green",
-                                            "fft_dual",  "fft_screen", "fft_epi"};
+  static const char *names[MGPU_K_COUNT] = {"deposit",    "mean",       "fft_r2c",  "fft_c2r", "fft_col_fwd",
+                                            "fft_col_inv", "pofk",      "spectral", "real",    "gather",
+                                            "kick",       "drift",      "lpt",      "readout", "fft_green",
+                                            "fft_dual",   "fft_screen", "fft_epi",  "dep_sort", "fold",
+                                            "fft_pot",    "fft_pk"};
   return (k >= 0 && k < MGPU_K_COUNT) ? names[k] : "?";
 }
 
@@ -193,7 +194,7 @@ int mgpu_create(const mgpu_params *p, mgpu_ctx **out) {
       if ((rc = fft_plan_init(ctx, &ctx->fft_lattice, ctx->Np, MGPU_FFT_TCOLS))) return fail(rc);
       ctx->fft_lattice.kcat = MGPU_K_LPT;
       const size_t nl = (size_t)ctx->Np * ctx->Np * ctx->fft_lattice.hp;
-      if ((rc = alloc(ctx, (void **)&ctx->lattice_slot, 3 * nl * sizeof(float2), "lattice spectra"))) return fail(rc);
+      if ((rc = alloc(ctx, (void **)&ctx->lattice_slot, 6 * nl * sizeof(float2), "lattice spectra"))) return fail(rc);
     }
   }
   ctx->partial_len = 3 * (size_t)std::max<size_t>(cdiv(ctx->np, 256), (size_t)ctx->N) + 3 * 1024;
@@ -233,7 +234,7 @@ int mgpu_create(const mgpu_params *p, mgpu_ctx **out) {
       return fail(mgpu_fail(ctx, MGPU_E_HIP, "memset failed"));
   if (hipMemsetAsync(ctx->slot[3], 0, 3 * sb, ctx->stream) != hipSuccess ||
       (ctx->lattice_slot && hipMemsetAsync(ctx->lattice_slot, 0,
-                                           3 * (size_t)ctx->Np * ctx->Np * ctx->fft_lattice.hp * sizeof(float2),
+                                           6 * (size_t)ctx->Np * ctx->Np * ctx->fft_lattice.hp * sizeof(float2),
                                            ctx->stream) != hipSuccess))
     return fail(mgpu_fail(ctx, MGPU_E_HIP, "memset failed"));
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(mgpu_fail(ctx, MGPU_E_HIP, "sync failed"));
@@ -252,6 +253,8 @@ void mgpu_destroy(mgpu_ctx *ctx) {
                   ctx->dep_scratch, ctx->winv};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
+  if (ctx->slice_host) (void)hipHostFree(ctx->slice_host);
+  if (ctx->slice_ev) (void)hipEventDestroy(ctx->slice_ev);
   fft_plan_free(&ctx->fft);
   fft_plan_free(&ctx->fft_lattice);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -267,6 +270,7 @@ int mgpu_sync(mgpu_ctx *ctx) {
 }
 
 int mgpu_set_particles(mgpu_ctx *ctx, const float *pos, const float *vel) {
+  if (ctx) ctx->sorted_valid = false;
   if (check_ctx(ctx)) return MGPU_E_INVALID;
   const size_t b = 3 * ctx->np * sizeof(float);
   if (pos) MGPU_HIP(ctx, hipMemcpyAsync(ctx->pos, pos, b, hipMemcpyHostToDevice, ctx->stream));
@@ -394,6 +398,7 @@ int mgpu_ic_displacement_fields(mgpu_ctx *ctx, unsigned long seed, int sphere_mo
 }
 
 int mgpu_init_particles(mgpu_ctx *ctx, double Di, double Di2) {
+  if (ctx) ctx->sorted_valid = false;
   if (check_ctx(ctx)) return MGPU_E_INVALID;
   int rc = ic_init_particles(ctx, Di, Di2);
   if (rc) return rc;
@@ -411,6 +416,21 @@ int mgpu_lpt_fields_slices(mgpu_ctx *ctx, int slot, const mgpu_kslice *o1, const
       !o2->num || !o2->dnum || !o2->den || !o2->dden)
     return mgpu_fail(ctx, MGPU_E_INVALID, "bad arguments to mgpu_lpt_fields_slices");
   return ic_lpt_fields_slices(ctx, slot, o1, o2);
+}
+
+static bool slice_ok(const mgpu_kslice *q) { return q && q->num && q->dnum && q->den && q->dden; }
+
+int mgpu_lpt_fields_slices2(mgpu_ctx *ctx, const mgpu_kslice *s0o1, const mgpu_kslice *s0o2,
+                            const mgpu_kslice *s1o1, const mgpu_kslice *s1o2) {
+  if (check_ctx(ctx) || !slice_ok(s0o1) || !slice_ok(s0o2) || !slice_ok(s1o1) || !slice_ok(s1o2))
+    return mgpu_fail(ctx, MGPU_E_INVALID, "bad arguments to mgpu_lpt_fields_slices2");
+  const int slots[2] = {0, 1};
+  const mgpu_kslice *q[4] = {s0o1, s0o2, s1o1, s1o2};
+  if (!ctx->lattice_slot) {
+    int rc = ic_lpt_fields_slices_n(ctx, 1, slots, q);
+    return rc ? rc : ic_lpt_fields_slices_n(ctx, 1, slots + 1, q + 2);
+  }
+  return ic_lpt_fields_slices_n(ctx, 2, slots, q);
 }
 
 // ---- mesh inspection: convert between the device layout and the FFTW layout (no ghost slab)

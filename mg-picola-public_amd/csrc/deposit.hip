@@ -2,19 +2,26 @@
 //
 // Global 64-bit atomics on an 8 N^3-byte accumulator are bound by the L2's
 // atomic rate (one per particle corner).  Here every particle is instead
-// binned into its 16^3-cell tile (wave-aggregated counters, prefix sum,
-// scatter of particle indices), and each tile accumulates its particles in
-// LDS with 64-bit integer atomics on a (16+1)^3 block.  The tile stores the
-// cells it owns into a 64-bit accumulator mesh (full 128-byte row segments)
-// and its +1 halo layer into plane, edge and corner buffers laid out like the
-// mesh cells they belong to (the x = 16(ti+1) plane, ...).  A streaming pass
-// then reads the accumulator, adds to every low-face cell (a local coordinate
-// 0) the parts its lower neighbours left in those buffers -- coalesced reads,
-// no per-cell search -- and writes delta = acc 2^-32 - 1 in the real mesh
-// layout.  The weights and their 2^-32 fixed-point rounding are those of the
-// direct deposit, and integer sums do not depend on order, so delta is
-// bit-identical to a serial deposit of the same particles.
+// binned by its T^3-cell tile and by its face class -- which of its cell's
+// local coordinates equal T - 1, the cells whose +1 corners reach into the
+// upper neighbours -- with wave-aggregated counters, a prefix sum and a
+// scatter of particle indices.  One workgroup then owns one tile: it reads
+// the tile's particles and, from its seven lower neighbours, only the
+// particles of the face classes that reach it, adds every corner that lands
+// in the tile with 64-bit integer atomics on a T^3 block in LDS, and writes
+// delta = acc 2^-32 - 1 for its cells straight into the real mesh layout.
+// No accumulator mesh, halo buffer or second pass is needed.  The weights and
+// their 2^-32 fixed-point rounding are those of the direct deposit, and
+// integer sums do not depend on order, so delta is bit-identical to a serial
+// deposit of the same particles.
+//
+// The gather (MtoParticles, src/auxPM.c:560-644) reuses the bins while the
+// positions are unchanged: one workgroup per tile stages the three force
+// meshes on the tile's (T+1)^3 cells in LDS and interpolates its particles
+// from there, so the scattered per-particle traffic is 3 position loads and 3
+// Disp stores instead of 24 mesh loads.
 #include "mgpu_internal.h"
+#include "cic.h"
 
 #include <cmath>
 
@@ -30,49 +37,14 @@ struct TileGeom {
   int ntx, nty, ntz;   // tiles per dimension
 };
 
-__device__ __forceinline__ void cell_of(double X, int N, unsigned &I, double &D) {
-  if (!(X >= 0.0 && X < (double)N)) {
-    if (!isfinite(X)) X = 0.0;
-    X -= floor(X / N) * N;
-    if (!(X >= 0.0 && X < (double)N)) X = 0.0;
-  }
-  I = (unsigned)X;
-  D = X - (double)I;
-  if (I >= (unsigned)N) I = 0;
-}
-
 __device__ __forceinline__ unsigned tile_of(const TileGeom &g, unsigned IX, unsigned IY, unsigned IZ) {
   return ((IX / g.tx) * g.nty + IY / g.ty) * g.ntz + IZ / g.tz;
 }
 
-// halo parts of the tiles, stored at the mesh cell they belong to.  With nt
-// tiles per dimension and p, q, r the tile-plane indices (cell / tile size):
-//   hx[p][y][z], hy[x][q][z], hz[r][x][y]       (faces: x, y or z local 0)
-//   exy[p][q][z], exz[p][r][y], eyz[q][r][x]     (edges)
-//   c[p][q][r]                                   (corner)
-struct Halo {
-  unsigned long long *hx, *hy, *hz, *exy, *exz, *eyz, *c;
-};
-size_t halo_cells(int N, int t) {
-  const size_t n = N, nt = N / t;
-  return 3 * nt * n * n + 3 * nt * nt * n + nt * nt * nt;
-}
-Halo halo_carve(unsigned long long *base, int N, int t) {
-  const size_t n = N, nt = N / t;
-  Halo h;
-  h.hx = base;
-  h.hy = h.hx + nt * n * n;
-  h.hz = h.hy + nt * n * n;
-  h.exy = h.hz + nt * n * n;
-  h.exz = h.exy + nt * nt * n;
-  h.eyz = h.exz + nt * nt * n;
-  h.c = h.eyz + nt * nt * n;
-  return h;
-}
-
 __device__ __forceinline__ float to_delta(unsigned long long v) { return (float)((double)v * kInvFix - 1.0); }
 
-// 1. tile of every particle and its rank inside the tile (wave-aggregated atomics)
+// 1. bin key of every particle, 8 tile + face class (class bits: x, y, z local
+// coordinate = T - 1), and its rank inside the bin (wave-aggregated atomics)
 __global__ __launch_bounds__(kTB) void k_dep_bin(const float *__restrict__ pos, size_t np, TileGeom g, double scale,
                                                  unsigned *__restrict__ count, unsigned *__restrict__ tile,
                                                  unsigned *__restrict__ rank) {
@@ -85,7 +57,9 @@ __global__ __launch_bounds__(kTB) void k_dep_bin(const float *__restrict__ pos, 
     cell_of((double)pos[i] * scale, g.N, IX, D);
     cell_of((double)pos[np + i] * scale, g.N, IY, D);
     cell_of((double)pos[2 * np + i] * scale, g.N, IZ, D);
-    t = tile_of(g, IX, IY, IZ);
+    const unsigned m = (unsigned)g.tx - 1;
+    const unsigned cls = ((IX & m) == m ? 4u : 0u) | ((IY & m) == m ? 2u : 0u) | ((IZ & m) == m ? 1u : 0u);
+    t = 8u * tile_of(g, IX, IY, IZ) + cls;
   }
   const int lane = threadIdx.x % warpSize;
   unsigned r = 0;
@@ -166,29 +140,64 @@ __global__ __launch_bounds__(kTB) void k_dep_scatter(const unsigned *__restrict_
   idx[off[tile[i]] + rank[i]] = (unsigned)i;
 }
 
-// 4. one workgroup per tile: LDS accumulation of the (t+1)^3 block
+// 4. one workgroup per tile.  The particle lists it reads are up to 64 bin
+// ranges: for each lower-neighbour shift (sx, sy, sz) in {0, 1}^3 the bins of
+// the neighbour tile whose face class contains the shift's bits.  A particle
+// read through shift s has local coordinate (I mod T) - s T, so its corners at
+// local coordinate -1 (s = 1) or T (s = 0) fall outside the tile and are
+// skipped; every corner is added by exactly one tile.
 __global__ __launch_bounds__(kTB) void k_dep_tile(const float *__restrict__ pos, size_t np,
                                                   const unsigned *__restrict__ idx, const unsigned *__restrict__ off,
-                                                  TileGeom g, double scale, double wpar,
-                                                  unsigned long long *__restrict__ acc, Halo h) {
+                                                  TileGeom g, double scale, double wpar, int HP2,
+                                                  float *__restrict__ d) {
   extern __shared__ unsigned long long blk[];
-  const int bx = g.tx + 1, by = g.ty + 1, bz = g.tz + 1, nblk = bx * by * bz;
+  __shared__ unsigned rstart[64], rpre[65];
+  const int T = g.tx, T3 = T * T * T;
   const unsigned t = blockIdx.x;
   const int ti = t / (g.nty * g.ntz), tj = (t / g.ntz) % g.nty, tk = t % g.ntz;
-  const unsigned X0 = ti * g.tx, Y0 = tj * g.ty, Z0 = tk * g.tz;
-  for (int e = threadIdx.x; e < nblk; e += blockDim.x) blk[e] = 0ull;
+  for (int e = threadIdx.x; e < T3; e += blockDim.x) blk[e] = 0ull;
+  if (threadIdx.x < 64) {
+    // range q: shift bits q / 8 (x: 4, y: 2, z: 1), face class q % 8
+    const int sh = threadIdx.x >> 3, cls = threadIdx.x & 7;
+    unsigned a = 0, n = 0;
+    if ((cls & sh) == sh) {
+      const int ni = (ti - (sh >> 2) + g.ntx) % g.ntx, nj = (tj - ((sh >> 1) & 1) + g.nty) % g.nty,
+                nk = (tk - (sh & 1) + g.ntz) % g.ntz;
+      const unsigned b = 8u * (unsigned)((ni * g.nty + nj) * g.ntz + nk) + (unsigned)cls;
+      a = off[b];
+      n = off[b + 1] - a;
+    }
+    rstart[threadIdx.x] = a;
+    // inclusive prefix sum of the lengths over the 64 lanes of the first wave
+    unsigned v = n;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const unsigned u = __shfl_up(v, o);
+      if (threadIdx.x >= (unsigned)o) v += u;
+    }
+    rpre[threadIdx.x + 1] = v;
+    if (threadIdx.x == 0) rpre[0] = 0;
+  }
   __syncthreads();
-  const unsigned p0 = off[t], p1 = off[t + 1];
-  // U particles per thread and round: all index loads, then all position loads,
-  // are in flight before the first deposit
+  const unsigned total = rpre[64];
+  const int tm = T - 1;
+  // U virtual indices per thread and round: all index loads, then all position
+  // loads, are in flight before the first deposit
   constexpr int U = 4;
-  for (unsigned base = p0; base < p1; base += U * blockDim.x) {
+  for (unsigned base = 0; base < total; base += U * blockDim.x) {
     unsigned ii[U];
+    int sh[U];
     float px[U], py[U], pz[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const unsigned q = base + u * blockDim.x + threadIdx.x;
-      ii[u] = idx[q < p1 ? q : p1 - 1];
+      const unsigned v = min(base + u * blockDim.x + threadIdx.x, total - 1);
+      // range holding v: the last q with rpre[q] <= v (binary search over 64)
+      int q = 0;
+#pragma unroll
+      for (int stp = 32; stp > 0; stp >>= 1)
+        if (rpre[q + stp] <= v) q += stp;
+      sh[u] = q >> 3;
+      ii[u] = idx[rstart[q] + (v - rpre[q])];
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -198,78 +207,127 @@ __global__ __launch_bounds__(kTB) void k_dep_tile(const float *__restrict__ pos,
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-    if (base + u * blockDim.x + threadIdx.x >= p1) break;
-    unsigned IX, IY, IZ;
-    double DX, DY, DZ;
-    cell_of((double)px[u] * scale, g.N, IX, DX);
-    cell_of((double)py[u] * scale, g.N, IY, DY);
-    cell_of((double)pz[u] * scale, g.N, IZ, DZ);
-    const double TX = 1.0 - DX, TZ = 1.0 - DZ;
-    const double TY = (1.0 - DY) * wpar;
-    DY *= wpar;
-    const int lx = IX - X0, ly = IY - Y0, lz = IZ - Z0;
-    const int b00 = (lx * by + ly) * bz + lz, b01 = (lx * by + ly + 1) * bz + lz;
-    const int b10 = ((lx + 1) * by + ly) * bz + lz, b11 = ((lx + 1) * by + ly + 1) * bz + lz;
-#define DEP(b, w) atomicAdd(&blk[(b)], (unsigned long long)__double2ll_rn((w) * kFix))
-    DEP(b00, TX * TY * TZ);
-    DEP(b00 + 1, TX * TY * DZ);
-    DEP(b01, TX * DY * TZ);
-    DEP(b01 + 1, TX * DY * DZ);
-    DEP(b10, DX * TY * TZ);
-    DEP(b10 + 1, DX * TY * DZ);
-    DEP(b11, DX * DY * TZ);
-    DEP(b11 + 1, DX * DY * DZ);
+      if (base + u * blockDim.x + threadIdx.x >= total) break;
+      unsigned IX, IY, IZ;
+      double DX, DY, DZ;
+      cell_of((double)px[u] * scale, g.N, IX, DX);
+      cell_of((double)py[u] * scale, g.N, IY, DY);
+      cell_of((double)pz[u] * scale, g.N, IZ, DZ);
+      const double TX = 1.0 - DX, TZ = 1.0 - DZ;
+      const double TY = (1.0 - DY) * wpar;
+      DY *= wpar;
+      const int lx = (int)(IX & tm) - ((sh[u] >> 2) ? T : 0), ly = (int)(IY & tm) - (((sh[u] >> 1) & 1) ? T : 0),
+                lz = (int)(IZ & tm) - ((sh[u] & 1) ? T : 0);
+      // corner a of an axis is in the tile iff 0 <= l + a < T
+      const bool x0 = lx >= 0, x1 = lx + 1 < T, y0 = ly >= 0, y1 = ly + 1 < T, z0 = lz >= 0, z1 = lz + 1 < T;
+      const int b00 = (lx * T + ly) * T + lz, b01 = b00 + T, b10 = b00 + T * T, b11 = b10 + T;
+#define DEP(c, b, w) \
+  if (c) atomicAdd(&blk[(b)], (unsigned long long)__double2ll_rn((w) * kFix))
+      DEP(x0 && y0 && z0, b00, TX * TY * TZ);
+      DEP(x0 && y0 && z1, b00 + 1, TX * TY * DZ);
+      DEP(x0 && y1 && z0, b01, TX * DY * TZ);
+      DEP(x0 && y1 && z1, b01 + 1, TX * DY * DZ);
+      DEP(x1 && y0 && z0, b10, DX * TY * TZ);
+      DEP(x1 && y0 && z1, b10 + 1, DX * TY * DZ);
+      DEP(x1 && y1 && z0, b11, DX * DY * TZ);
+      DEP(x1 && y1 && z1, b11 + 1, DX * DY * DZ);
 #undef DEP
     }
   }
   __syncthreads();
-  const size_t N = g.N, nt = g.ntx;
-  const int T = g.tx;
-  const size_t p = (ti + 1) % g.ntx, q = (tj + 1) % g.nty, r = (tk + 1) % g.ntz;
-  for (int e = threadIdx.x; e < nblk; e += blockDim.x) {
-    const int lz = e % bz, ly = (e / bz) % by, lx = e / (bz * by);
-    const unsigned long long v = blk[e];
-    const bool fx = lx == T, fy = ly == T, fz = lz == T;
-    const size_t x = X0 + lx, y = Y0 + ly, z = Z0 + lz;
-    if (!fx && !fy && !fz) acc[(x * N + y) * N + z] = v;
-    else if (fx && !fy && !fz) h.hx[(p * N + y) * N + z] = v;
-    else if (!fx && fy && !fz) h.hy[(x * nt + q) * N + z] = v;
-    else if (!fx && !fy && fz) h.hz[(r * N + x) * N + y] = v;
-    else if (fx && fy && !fz) h.exy[(p * nt + q) * N + z] = v;
-    else if (fx && !fy && fz) h.exz[(p * nt + r) * N + y] = v;
-    else if (!fx && fy && fz) h.eyz[(q * nt + r) * N + x] = v;
-    else h.c[(p * nt + q) * nt + r] = v;
+  const size_t N = g.N, X0 = (size_t)ti * T, Y0 = (size_t)tj * T, Z0 = (size_t)tk * T;
+  for (int e = threadIdx.x; e < T3; e += blockDim.x) {
+    const int lz = e & tm, ly = (e / T) & tm, lx = e / (T * T);
+    d[((X0 + lx) * N + Y0 + ly) * HP2 + Z0 + lz] = to_delta(blk[e]);
   }
 }
 
-// 5. streaming pass over the cells: accumulator + (low-face cells) the halo
-// parts of the lower neighbours, converted to delta in the real layout.
-// Grid (N*N rows, z blocks); tile sizes are powers of two (lt = log2 of the tile size).
-__global__ __launch_bounds__(kTB) void k_dep_finish(const unsigned long long *__restrict__ acc, Halo h, int N,
-                                                    int lt, int HP2, float *__restrict__ d0) {
-  const int z = blockIdx.y * blockDim.x + threadIdx.x;
-  if (z >= N) return;
-  const unsigned row = blockIdx.x;
-  const size_t n = N, nt = N >> lt;
-  const size_t y = row % (unsigned)N, x = row / (unsigned)N;
-  unsigned long long s = acc[(size_t)row * N + z];
-  const int tm = (1 << lt) - 1;
-  const bool ox = (x & tm) == 0, oy = (y & tm) == 0, oz = (z & tm) == 0;
-  const size_t p = x >> lt, q = y >> lt, r = (size_t)z >> lt;
-  if (ox) s += h.hx[(p * n + y) * n + z];
-  if (oy) s += h.hy[(x * nt + q) * n + z];
-  if (oz) s += h.hz[(r * n + x) * n + y];
-  if (ox && oy) s += h.exy[(p * nt + q) * n + z];
-  if (ox && oz) s += h.exz[(p * nt + r) * n + y];
-  if (oy && oz) s += h.eyz[(q * nt + r) * n + x];
-  if (ox && oy && oz) s += h.c[(p * nt + q) * nt + r];
-  d0[(size_t)row * HP2 + z] = to_delta(s);
+// 5. gather by tiles.  Consecutive tiles in z go to one XCD (workgroups are
+// dealt to the 8 XCDs round robin), so the +1 halo a tile reads was mostly
+// fetched into the same L2 by its neighbour.
+template <int T>
+__global__ __launch_bounds__(512) void k_gather_tile(const float *__restrict__ pos, size_t np,
+                                                     const unsigned *__restrict__ idx,
+                                                     const unsigned *__restrict__ off, int N, int nt1, int HP2,
+                                                     double scale, const float *__restrict__ g0,
+                                                     const float *__restrict__ g1, const float *__restrict__ g2,
+                                                     float *__restrict__ disp) {
+  constexpr int B = T + 1, B2 = B * B, B3 = B2 * B;
+  __shared__ float f[3][B3];
+  const unsigned nt = (unsigned)nt1 * nt1 * nt1;
+  const unsigned t = nt % 8 == 0 ? (blockIdx.x & 7) * (nt >> 3) + (blockIdx.x >> 3) : blockIdx.x;
+  const int ti = t / (nt1 * nt1), tj = (t / nt1) % nt1, tk = t % nt1;
+  const int X0 = ti * T, Y0 = tj * T, Z0 = tk * T;
+  // rows (lx, ly) of each mesh: T contiguous cells, then the z = T column.
+  // Every load of the thread is issued before the first LDS store, so the
+  // fill costs one memory latency, not one per row group.
+  constexpr int NT = 512, KR = (B2 * T + NT - 1) / NT, KC = (B2 + NT - 1) / NT;
+  const int gz = Z0 + T < N ? Z0 + T : 0;
+  float v[3][KR + KC];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const float *__restrict__ m = a == 0 ? g0 : a == 1 ? g1 : g2;
+#pragma unroll
+    for (int k = 0; k < KR; k++) {
+      const int e = k * NT + (int)threadIdx.x;
+      if (e < B2 * T) {
+        const int lz = e % T, r = e / T, lx = r / B, ly = r % B;
+        const int gx = X0 + lx < N ? X0 + lx : 0, gy = Y0 + ly < N ? Y0 + ly : 0;
+        v[a][k] = m[((size_t)gx * N + gy) * HP2 + Z0 + lz];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < KC; k++) {
+      const int r = k * NT + (int)threadIdx.x;
+      if (r < B2) {
+        const int lx = r / B, ly = r % B;
+        const int gx = X0 + lx < N ? X0 + lx : 0, gy = Y0 + ly < N ? Y0 + ly : 0;
+        v[a][KR + k] = m[((size_t)gx * N + gy) * HP2 + gz];
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+#pragma unroll
+    for (int k = 0; k < KR; k++) {
+      const int e = k * NT + (int)threadIdx.x;
+      if (e < B2 * T) f[a][(e / T) * B + e % T] = v[a][k];
+    }
+#pragma unroll
+    for (int k = 0; k < KC; k++) {
+      const int r = k * NT + (int)threadIdx.x;
+      if (r < B2) f[a][r * B + T] = v[a][KR + k];
+    }
+  }
+  __syncthreads();
+  const unsigned p1 = off[8 * t + 8];
+  for (unsigned q = off[8 * t] + threadIdx.x; q < p1; q += blockDim.x) {
+    const size_t i = idx[q];
+    unsigned IX, IY, IZ;
+    double DX, DY, DZ;
+    cell_of((double)pos[i] * scale, N, IX, DX);
+    cell_of((double)pos[np + i] * scale, N, IY, DY);
+    cell_of((double)pos[2 * np + i] * scale, N, IZ, DZ);
+    const double TX = 1.0 - DX, TY = 1.0 - DY, TZ = 1.0 - DZ;
+    // the particle was binned into this tile, so 0 <= local index < T
+    const int b00 = ((int)(IX - X0) * B + (int)(IY - Y0)) * B + (int)(IZ - Z0), b01 = b00 + B, b10 = b00 + B2,
+              b11 = b10 + B;
+    const double w000 = TX * TY * TZ, w001 = TX * TY * DZ, w010 = TX * DY * TZ, w011 = TX * DY * DZ;
+    const double w100 = DX * TY * TZ, w101 = DX * TY * DZ, w110 = DX * DY * TZ, w111 = DX * DY * DZ;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const float *m = f[a];
+      const double d = m[b00] * w000 + m[b00 + 1] * w001 + m[b01] * w010 + m[b01 + 1] * w011 + m[b10] * w100 +
+                       m[b10 + 1] * w101 + m[b11] * w110 + m[b11 + 1] * w111;
+      disp[(size_t)a * np + i] = (float)d;
+    }
+  }
 }
 
 TileGeom make_geom(int N) {
   TileGeom g;
   g.N = N;
-  const int t = N % 16 == 0 ? 16 : N % 8 == 0 ? 8 : 4;
+  const int t = cic_tile_size(N);
   g.tx = g.ty = g.tz = t;
   g.ntx = N / g.tx;
   g.nty = N / g.ty;
@@ -279,22 +337,21 @@ TileGeom make_geom(int N) {
 
 }  // namespace
 
-// scratch sizes for the tiled deposit: counters + offsets, tile ids, ranks, indices, halo buffer
+// scratch sizes for the tiled deposit: bin counters + offsets, scan sums, bin keys, ranks, indices
 size_t deposit_scratch_bytes(int N, size_t np) {
   TileGeom g = make_geom(N);
-  const size_t nt = (size_t)g.ntx * g.nty * g.ntz;
-  const size_t nsum = (nt + 1023) / 1024;
-  return sizeof(unsigned) * (2 * (nt + 1) + nsum + 3 * np) + sizeof(unsigned long long) * halo_cells(N, g.tx) +
-         256 * 8;
+  const size_t nb = 8 * (size_t)g.ntx * g.nty * g.ntz;
+  const size_t nsum = (nb + 1023) / 1024;
+  return sizeof(unsigned) * (2 * (nb + 1) + nsum + 3 * np) + 256 * 8;
 }
 
-int deposit_tiled(mgpu_ctx *ctx, unsigned long long *acc, float *d) {
+int deposit_tiled(mgpu_ctx *ctx, float *d) {
   const int N = ctx->N;
   const size_t np = ctx->np;
   TileGeom g = make_geom(N);
-  const size_t nt = (size_t)g.ntx * g.nty * g.ntz;
-  const size_t nsum = (nt + 1023) / 1024;
-  if (nsum > 1024 * 1024) return mgpu_fail(ctx, MGPU_E_INVALID, "too many deposit tiles");
+  const size_t nt = (size_t)g.ntx * g.nty * g.ntz, nb = 8 * nt;
+  const size_t nsum = (nb + 1023) / 1024;
+  if (nsum > 1024 * 1024 || nb > 0xffffffffull) return mgpu_fail(ctx, MGPU_E_INVALID, "too many deposit tiles");
   // carve the scratch buffer (256-byte aligned pieces)
   char *b = (char *)ctx->dep_scratch;
   auto take = [&](size_t bytes) {
@@ -302,50 +359,59 @@ int deposit_tiled(mgpu_ctx *ctx, unsigned long long *acc, float *d) {
     b += (bytes + 255) / 256 * 256;
     return r;
   };
-  const Halo halo = halo_carve((unsigned long long *)take(sizeof(unsigned long long) * halo_cells(N, g.tx)), N, g.tx);
-  unsigned *count = (unsigned *)take(sizeof(unsigned) * (nt + 1));
-  unsigned *off = (unsigned *)take(sizeof(unsigned) * (nt + 1));
+  unsigned *count = (unsigned *)take(sizeof(unsigned) * (nb + 1));
+  unsigned *off = (unsigned *)take(sizeof(unsigned) * (nb + 1));
   unsigned *sums = (unsigned *)take(sizeof(unsigned) * nsum);
-  unsigned *tile = (unsigned *)take(sizeof(unsigned) * np);
+  unsigned *key = (unsigned *)take(sizeof(unsigned) * np);
   unsigned *rank = (unsigned *)take(sizeof(unsigned) * np);
   unsigned *idx = (unsigned *)take(sizeof(unsigned) * np);
   if ((size_t)(b - (char *)ctx->dep_scratch) > ctx->dep_scratch_bytes)
     return mgpu_fail(ctx, MGPU_E_INVALID, "deposit scratch too small");
+  // the sorted indices stay valid for the gather until the positions change
+  ctx->dep_idx = idx;
+  ctx->dep_off = off;
+  ctx->sorted_valid = np > 0;
   const double scale = (double)N / ctx->p.box;
   const double wpar = std::pow((double)N / (double)ctx->Np, 3);
-  MGPU_HIP(ctx, hipMemsetAsync(count, 0, sizeof(unsigned) * (nt + 1), ctx->stream));
+  MGPU_HIP(ctx, hipMemsetAsync(count, 0, sizeof(unsigned) * (nb + 1), ctx->stream));
   if (np) {
-    KScope ks(ctx, MGPU_K_DEPOSIT, 20.0 * np);
-    hipLaunchKernelGGL(k_dep_bin, dim3(cdiv(np, kTB)), dim3(kTB), 0, ctx->stream, ctx->pos, np, g, scale, count, tile,
+    KScope ks(ctx, MGPU_K_DEP_SORT, 20.0 * np);
+    hipLaunchKernelGGL(k_dep_bin, dim3(cdiv(np, kTB)), dim3(kTB), 0, ctx->stream, ctx->pos, np, g, scale, count, key,
                        rank);
   }
   MGPU_HIP(ctx, hipGetLastError());
-  hipLaunchKernelGGL(k_scan_block, dim3(nsum), dim3(1024), 0, ctx->stream, count, nt, off, sums);
+  hipLaunchKernelGGL(k_scan_block, dim3(nsum), dim3(1024), 0, ctx->stream, count, nb, off, sums);
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, ctx->stream, sums, (int)nsum);
-  hipLaunchKernelGGL(k_scan_add, dim3(nsum), dim3(1024), 0, ctx->stream, off, nt, sums, (unsigned)np, nt);
+  hipLaunchKernelGGL(k_scan_add, dim3(nsum), dim3(1024), 0, ctx->stream, off, nb, sums, (unsigned)np, nb);
   MGPU_HIP(ctx, hipGetLastError());
   if (np) {
-    KScope ks(ctx, MGPU_K_DEPOSIT, 12.0 * np);
-    hipLaunchKernelGGL(k_dep_scatter, dim3(cdiv(np, kTB)), dim3(kTB), 0, ctx->stream, tile, rank, off, np, idx);
+    KScope ks(ctx, MGPU_K_DEP_SORT, 12.0 * np);
+    hipLaunchKernelGGL(k_dep_scatter, dim3(cdiv(np, kTB)), dim3(kTB), 0, ctx->stream, key, rank, off, np, idx);
   }
   MGPU_HIP(ctx, hipGetLastError());
-  const size_t shmem = sizeof(unsigned long long) * (g.tx + 1) * (g.ty + 1) * (g.tz + 1);
-  const int HP2 = 2 * ctx->HP;
-  const double cells = (double)N * N * N, hcells = (double)halo_cells(N, g.tx);
+  const size_t shmem = sizeof(unsigned long long) * g.tx * g.ty * g.tz;
+  const double cells = (double)N * N * N;
   {
-    // compulsory: indices + positions read, accumulator and halo parts written
-    KScope ks(ctx, MGPU_K_DEPOSIT, 16.0 * np + 8.0 * (cells + hcells));
+    // compulsory: indices + positions read once, delta written
+    KScope ks(ctx, MGPU_K_DEPOSIT, 16.0 * np + 4.0 * cells);
     hipLaunchKernelGGL(k_dep_tile, dim3(nt), dim3(kTB), shmem, ctx->stream, ctx->pos, np, idx, off, g, scale, wpar,
-                       acc, halo);
+                       2 * ctx->HP, d);
   }
   MGPU_HIP(ctx, hipGetLastError());
-  {
-    // accumulator and halo parts read, delta written
-    KScope ks(ctx, MGPU_K_DENSITY, 8.0 * (cells + hcells) + 4.0 * cells);
-    const int lt = g.tx == 16 ? 4 : g.tx == 8 ? 3 : 2;
-    hipLaunchKernelGGL(k_dep_finish, dim3((unsigned)((size_t)N * N), cdiv((size_t)N, kTB)), dim3(kTB), 0,
-                       ctx->stream, acc, halo, N, lt, HP2, d);
-  }
+  return MGPU_OK;
+}
+
+// Disp at the particles from the force meshes f0, f1, f2 (real layout), by
+// the bins of the last deposit (the positions must not have changed since)
+int gather_tiled(mgpu_ctx *ctx, const float *f0, const float *f1, const float *f2) {
+  const int N = ctx->N;
+  if (!ctx->sorted_valid) return mgpu_fail(ctx, MGPU_E_STATE, "gather by tiles needs the bins of the last deposit");
+  const int T = cic_tile_size(N), nt1 = N / T;
+  const unsigned nt = (unsigned)nt1 * nt1 * nt1;
+  const double scale = (double)N / ctx->p.box;
+  auto kern = T == 16 ? k_gather_tile<16> : T == 8 ? k_gather_tile<8> : k_gather_tile<4>;
+  hipLaunchKernelGGL(kern, dim3(nt), dim3(512), 0, ctx->stream, ctx->pos, ctx->np, ctx->dep_idx, ctx->dep_off, N,
+                     nt1, 2 * ctx->HP, scale, f0, f1, f2, ctx->disp);
   MGPU_HIP(ctx, hipGetLastError());
   return MGPU_OK;
 }

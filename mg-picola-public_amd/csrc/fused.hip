@@ -390,7 +390,8 @@ int fused_green_x(mgpu_ctx *ctx, int mode, const float2 *a, const float2 *b, flo
   else ga.nf = (float)p0;
   ga.kfac = (float)p1;
   const double modes = (double)ctx->N * ctx->N * (ctx->N / 2 + 1);
-  KScope ks(ctx, MGPU_K_FFT_GREEN, 8.0 * modes * ((b ? 2 : 1) + (mode == GREEN_FORCE ? 2 : 1)));
+  KScope ks(ctx, mode == GREEN_FORCE ? MGPU_K_FFT_GREEN : MGPU_K_FFT_POT,
+            8.0 * modes * ((b ? 2 : 1) + (mode == GREEN_FORCE ? 2 : 1)));
   switch (ctx->N) {
 #define X(S) \
   case S:    \
@@ -448,7 +449,7 @@ int fused_z_screen(mgpu_ctx *ctx, int density_kind, float2 *buf, const float *dx
 int fused_pofk_x(mgpu_ctx *ctx, float2 *buf, const double *winv, int nbins, int bintype, double kmin, double kmax,
                  double *pk) {
   const double modes = (double)ctx->N * ctx->N * (ctx->N / 2 + 1);
-  KScope ks(ctx, MGPU_K_FFT_COL_FWD, 16.0 * modes);
+  KScope ks(ctx, MGPU_K_FFT_PK, 16.0 * modes);
   switch (ctx->N) {
 #define X(S) \
   case S:    \

@@ -242,70 +242,81 @@ __global__ __launch_bounds__(kBlock) void k_init_particles(float *__restrict__ p
 // LPT field components X_a = i k_a/k^2 (g1 cd1 + g2 cd2) at half-spectrum index
 // (i, j, l), l in [0, N/2], 2LPT.c wavenumbers (src/2LPT.c:1583-1629), in FP32
 // (the stored spectra and the output fields are FP32).  ikf = 1 / (2 pi/Box).
-// growth g_o(|n|^2) from FP32 per-|n|^2 tables
+// Growth sources of the fold, for NS LPT slots at once: eval(n2, y, g) sets
+// g[s][o] = growth of order o + 1 for slot s at |n|^2 = n2, y = ln k = ln kf +
+// ln|n| (only the slice source uses y).
+// From FP32 per-|n|^2 tables (one slot):
 struct GTab {
+  static constexpr bool kNeedsLnk = false;
   const float *t1, *t2;
-  __device__ __forceinline__ float g1(int n2) const { return t1[n2]; }
-  __device__ __forceinline__ float g2(int n2) const { return t2[n2]; }
-};
-// growth from the cubic Hermite ln k slices held in LDS (num, dnum, den, dden of
-// order 1, then of order 2), g = norm num / den at ln k = ln kf + ln|n|
-struct KSlf {
-  float y0, dy, norm;
-  int ny;
-};
-__device__ __forceinline__ float hermf(const float *v, const float *dv, const KSlf &s, float y) {
-  const float ymax = s.y0 + s.dy * (float)(s.ny - 1);
-  y = fminf(fmaxf(y, s.y0), ymax);
-  int j = (int)floorf((y - s.y0) / s.dy);
-  j = min(max(j, 0), s.ny - 2);
-  const float h = s.dy, t = (y - (s.y0 + (float)j * s.dy)) / h, t2 = t * t, t3 = t2 * t;
-  return (2.f * t3 - 3.f * t2 + 1.f) * v[j] + (t3 - 2.f * t2 + t) * h * dv[j] + (-2.f * t3 + 3.f * t2) * v[j + 1] +
-         (t3 - t2) * h * dv[j + 1];
-}
-struct GSlice {
-  const float *sl;
-  KSlf s1, s2;
-  float lnkf;
-  __device__ __forceinline__ float g(const float *b, const KSlf &s, int n2) const {
-    const float y = lnkf + 0.5f * __logf((float)n2);
-    return s.norm * (hermf(b, b + s.ny, s, y) / hermf(b + 2 * s.ny, b + 3 * s.ny, s, y));
+  __device__ __forceinline__ void eval(int n2, float, float g[1][2]) const {
+    g[0][0] = t1[n2];
+    g[0][1] = t2[n2];
   }
-  __device__ __forceinline__ float g1(int n2) const { return g(sl, s1, n2); }
-  __device__ __forceinline__ float g2(int n2) const { return g(sl + 4 * s1.ny, s2, n2); }
+};
+// From cubic Hermite slices on one uniform ln k grid shared by all slices,
+// held in LDS node-major with 1 + NS float4 per node: (den_1, dden_1, den_2,
+// dden_2), shared by the slots, then per slot (num_1, dnum_1, num_2, dnum_2);
+// g = norm_o num / den.  The Hermite basis is computed once per mode.
+template <int NS>
+struct GShared {
+  static constexpr bool kNeedsLnk = true;
+  const float4 *sl;
+  float y0, dy, idy, ymax, norm1, norm2;
+  int ny;
+  __device__ __forceinline__ void eval(int, float y, float g[NS][2]) const {
+    y = fminf(fmaxf(y, y0), ymax);
+    int j = (int)((y - y0) * idy);
+    j = min(max(j, 0), ny - 2);
+    const float t = (y - (y0 + (float)j * dy)) * idy, t2 = t * t, t3 = t2 * t;
+    const float h00 = 2.f * t3 - 3.f * t2 + 1.f, h10 = (t3 - 2.f * t2 + t) * dy, h01 = -2.f * t3 + 3.f * t2,
+                h11 = (t3 - t2) * dy;
+    const float4 *A = sl + j * (1 + NS), *B = A + (1 + NS);
+    float4 a = A[0], b = B[0];
+    const float r1 = norm1 * __builtin_amdgcn_rcpf(h00 * a.x + h10 * a.y + h01 * b.x + h11 * b.y);
+    const float r2 = norm2 * __builtin_amdgcn_rcpf(h00 * a.z + h10 * a.w + h01 * b.z + h11 * b.w);
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      a = A[1 + s];
+      b = B[1 + s];
+      g[s][0] = r1 * (h00 * a.x + h10 * a.y + h01 * b.x + h11 * b.y);
+      g[s][1] = r2 * (h00 * a.z + h10 * a.w + h01 * b.z + h11 * b.w);
+    }
+  }
 };
 
-template <class G>
-__device__ __forceinline__ void lpt_xf(int i, int j, int l, int N, int HP, int Np, float ikf,
+// LPT field components X[s][a] = i k_a/k^2 (g1 cd1 + g2 cd2) of slot s at half-spectrum
+// index (i, j, l), l in [0, N/2], 2LPT.c wavenumbers (src/2LPT.c:1583-1629), in FP32
+// (the stored spectra and the output fields are FP32).  ikf = 1 / (2 pi/Box).
+template <int NS, class G>
+__device__ __forceinline__ void lpt_xf(int i, int j, int l, int N, int HP, int Np, float ikf, float lnkf,
                                        const float2 *__restrict__ cd1, const float2 *__restrict__ cd2, const G &gr,
-                                       float X[6]) {
+                                       float X[NS][6]) {
   const int ni = i < N / 2 ? i : i - N, nj = j < N / 2 ? j : j - N, nl = l < N / 2 ? l : l - N;
   const int n2 = ni * ni + nj * nj + nl * nl;
 #pragma unroll
-  for (int q = 0; q < 6; q++) X[q] = 0.f;
+  for (int s = 0; s < NS; s++)
+#pragma unroll
+    for (int q = 0; q < 6; q++) X[s][q] = 0.f;
   if (n2 == 0) return;
   const size_t e = ((size_t)i * N + j) * HP + l;
-  float dr = 0.f, di = 0.f;
   // the 1LPT field is zero outside the initial-condition cube |n_a| <= Nsample/2
-  if (abs(ni) <= Np / 2 && abs(nj) <= Np / 2 && abs(nl) <= Np / 2) {
-    const float2 d = cd1[e];
-    const float g = gr.g1(n2);
-    dr = d.x * g;
-    di = d.y * g;
-  }
-  if (cd2) {
-    const float2 d = cd2[e];
-    const float g = gr.g2(n2);
-    dr += d.x * g;
-    di += d.y * g;
-  }
+  const bool cube = abs(ni) <= Np / 2 && abs(nj) <= Np / 2 && abs(nl) <= Np / 2;
+  const float2 d1 = cube ? cd1[e] : make_float2(0.f, 0.f);
+  const float2 d2 = cd2 ? cd2[e] : make_float2(0.f, 0.f);
+  float g[NS][2];
+  gr.eval(n2, G::kNeedsLnk ? lnkf + 0.5f * __logf((float)n2) : 0.f, g);
   // k_a / k^2 = n_a / (kf n^2)
-  const float s = ikf / (float)n2;
-  const float nv[3] = {(float)ni * s, (float)nj * s, (float)nl * s};
+  const float sc = ikf * __builtin_amdgcn_rcpf((float)n2);
+  const float nv[3] = {(float)ni * sc, (float)nj * sc, (float)nl * sc};
 #pragma unroll
-  for (int q = 0; q < 3; q++) {
-    X[2 * q] = -nv[q] * di;
-    X[2 * q + 1] = nv[q] * dr;
+  for (int s = 0; s < NS; s++) {
+    const float dr = d1.x * g[s][0] + d2.x * g[s][1], di = d1.y * g[s][0] + d2.y * g[s][1];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      X[s][2 * q] = -nv[q] * di;
+      X[s][2 * q + 1] = nv[q] * dr;
+    }
   }
 }
 
@@ -318,17 +329,23 @@ __device__ __forceinline__ void lpt_xf(int i, int j, int l, int N, int HP, int N
 // consecutive threads take consecutive kz, so every alias read is a coalesced
 // row segment.  The folded DC mode is zeroed, which removes the lattice mean
 // exactly (src/2LPT.c:1719-1729).
-// One folded mode (ip, jp, lp), lp <= Np/2: the sum over its m^3 aliases.  MF > 0
-// is the fold factor m = N / Nsample as a compile-time constant, so the alias
-// loop unrolls and all its loads are in flight together; MF = 0 takes m at run time.
-template <int MF, class G>
+// One folded mode (ip, jp, lp), lp <= Np/2: the sum over its m^3 aliases, for
+// NS slots.  MF > 0 is the fold factor m = N / Nsample as a compile-time
+// constant, so the alias loop unrolls and all its loads are in flight
+// together; MF = 0 takes m at run time.  PLANES = false is for 0 < lp < Np/2:
+// then no alias lies on the kz = 0 or N/2 plane, and whether alias c lies
+// below or above N/2 does not depend on lp, so the lanes of a wave take one
+// path.
+template <int MF, int NS, class G, bool PLANES = true>
 __device__ __forceinline__ void fold_mode(int ip, int jp, int lp, int N, int HP, int Np, int m_rt, float ikf,
-                                          const float2 *__restrict__ cd1, const float2 *__restrict__ cd2, const G &gr,
-                                          float acc[6]) {
+                                          float lnkf, const float2 *__restrict__ cd1, const float2 *__restrict__ cd2,
+                                          const G &gr, float acc[NS][6]) {
   const int m = MF > 0 ? MF : m_rt;
-  float X[6], Y[6];
+  float X[NS][6], Y[NS][6];
 #pragma unroll
-  for (int q = 0; q < 6; q++) acc[q] = 0.f;
+  for (int s = 0; s < NS; s++)
+#pragma unroll
+    for (int q = 0; q < 6; q++) acc[s][q] = 0.f;
   if (ip == 0 && jp == 0 && lp == 0) return;
 #pragma unroll
   for (int a = 0; a < m; a++) {
@@ -340,73 +357,92 @@ __device__ __forceinline__ void fold_mode(int ip, int jp, int lp, int N, int HP,
       for (int c = 0; c < m; c++) {
         const int l = lp + c * Np;
         if (l > 0 && l < N / 2) {
-          lpt_xf(i, j, l, N, HP, Np, ikf, cd1, cd2, gr, X);
+          lpt_xf<NS>(i, j, l, N, HP, Np, ikf, lnkf, cd1, cd2, gr, X);
 #pragma unroll
-          for (int q = 0; q < 6; q++) acc[q] += X[q];
+          for (int s = 0; s < NS; s++)
+#pragma unroll
+            for (int q = 0; q < 6; q++) acc[s][q] += X[s][q];
         } else if (l > N / 2) {
-          lpt_xf(mi, mj, N - l, N, HP, Np, ikf, cd1, cd2, gr, X);
+          lpt_xf<NS>(mi, mj, N - l, N, HP, Np, ikf, lnkf, cd1, cd2, gr, X);
 #pragma unroll
-          for (int q = 0; q < 3; q++) {
-            acc[2 * q] += X[2 * q];
-            acc[2 * q + 1] -= X[2 * q + 1];
-          }
-        } else {  // kz = 0 or N/2 plane: (X(k) + conj X(-k)) / 2
-          lpt_xf(i, j, l, N, HP, Np, ikf, cd1, cd2, gr, X);
-          lpt_xf(mi, mj, l, N, HP, Np, ikf, cd1, cd2, gr, Y);
+          for (int s = 0; s < NS; s++)
 #pragma unroll
-          for (int q = 0; q < 3; q++) {
-            acc[2 * q] += 0.5f * (X[2 * q] + Y[2 * q]);
-            acc[2 * q + 1] += 0.5f * (X[2 * q + 1] - Y[2 * q + 1]);
-          }
+            for (int q = 0; q < 3; q++) {
+              acc[s][2 * q] += X[s][2 * q];
+              acc[s][2 * q + 1] -= X[s][2 * q + 1];
+            }
+        } else if (PLANES) {  // kz = 0 or N/2 plane: (X(k) + conj X(-k)) / 2
+          lpt_xf<NS>(i, j, l, N, HP, Np, ikf, lnkf, cd1, cd2, gr, X);
+          lpt_xf<NS>(mi, mj, l, N, HP, Np, ikf, lnkf, cd1, cd2, gr, Y);
+#pragma unroll
+          for (int s = 0; s < NS; s++)
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+              acc[s][2 * q] += 0.5f * (X[s][2 * q] + Y[s][2 * q]);
+              acc[s][2 * q + 1] += 0.5f * (X[s][2 * q + 1] - Y[s][2 * q + 1]);
+            }
         }
       }
     }
   }
 }
 
-// growth from per-|n|^2 tables; grid (Nsample^2 rows, kz blocks)
+// growth from per-|n|^2 tables, one slot; grid (Nsample^2 rows, kz blocks)
 template <int MF>
 __global__ __launch_bounds__(kBlock) void k_lpt_fold(const float2 *__restrict__ cd1, const float2 *__restrict__ cd2,
                                                      GTab gr, int N, int HP, int Np, int HPn, int m_rt, float ikf,
-                                                     float2 *__restrict__ o0, float2 *__restrict__ o1,
-                                                     float2 *__restrict__ o2) {
+                                                     float2 *__restrict__ out, size_t nl) {
   const int lp = blockIdx.y * blockDim.x + threadIdx.x;
   if (lp > Np / 2) return;
   const unsigned row = blockIdx.x;
   const int jp = (int)(row % (unsigned)Np), ip = (int)(row / (unsigned)Np);
-  float acc[6];
-  fold_mode<MF>(ip, jp, lp, N, HP, Np, m_rt, ikf, cd1, cd2, gr, acc);
+  float acc[1][6];
+  fold_mode<MF, 1>(ip, jp, lp, N, HP, Np, m_rt, ikf, 0.f, cd1, cd2, gr, acc);
   const size_t e = (size_t)row * HPn + lp;
-  o0[e] = make_float2(acc[0], acc[1]);
-  o1[e] = make_float2(acc[2], acc[3]);
-  o2[e] = make_float2(acc[4], acc[5]);
+#pragma unroll
+  for (int a = 0; a < 3; a++) out[a * nl + e] = make_float2(acc[0][2 * a], acc[0][2 * a + 1]);
 }
 
-// growth from the ln k slices, staged once per block in LDS; the blocks stride
-// over the Nsample^2 (Nsample/2 + 1) folded modes
-template <int MF>
-__global__ __launch_bounds__(kBlock) void k_lpt_fold_sl(const float2 *__restrict__ cd1,
+// growth from the shared-grid ln k slices of NS slots (node-major float4s,
+// copied once per block into LDS).  The blocks stride over the Nsample^2 rows;
+// the interior kz of a row first, then the kz = 0 and Nsample/2 modes of all
+// rows.  Output spectrum (slot s, component a) at out + (3 s + a) nl.
+template <int MF, int NS>
+__global__ __launch_bounds__(kBlock) void k_lpt_fold_gs(const float2 *__restrict__ cd1,
                                                         const float2 *__restrict__ cd2,
-                                                        const double *__restrict__ slices, KSlf s1, KSlf s2,
+                                                        const float4 *__restrict__ nodes, GShared<NS> gs,
                                                         float lnkf, int N, int HP, int Np, int HPn, int m_rt,
-                                                        float ikf, float2 *__restrict__ o0, float2 *__restrict__ o1,
-                                                        float2 *__restrict__ o2) {
-  extern __shared__ float sl[];
-  const int nsl = 4 * (s1.ny + s2.ny);
-  for (int e = threadIdx.x; e < nsl; e += blockDim.x) sl[e] = (float)slices[e];
+                                                        float ikf, float2 *__restrict__ out, size_t nl) {
+  extern __shared__ float4 sl[];
+  const int nn = gs.ny * (1 + NS);
+  for (int e = threadIdx.x; e < nn; e += blockDim.x) sl[e] = nodes[e];
   __syncthreads();
-  const GSlice gr{sl, s1, s2, lnkf};
-  const unsigned K = Np / 2 + 1;
-  const size_t total = (size_t)Np * Np * K;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-    const unsigned row = (unsigned)(e / K), lp = (unsigned)(e - (size_t)row * K);
+  gs.sl = sl;
+  const unsigned rows = (unsigned)Np * (unsigned)Np;
+  const int H = Np / 2;
+  for (unsigned row = blockIdx.x; row < rows; row += gridDim.x) {
     const int jp = (int)(row % (unsigned)Np), ip = (int)(row / (unsigned)Np);
-    float acc[6];
-    fold_mode<MF>(ip, jp, (int)lp, N, HP, Np, m_rt, ikf, cd1, cd2, gr, acc);
+    for (int lp = 1 + (int)threadIdx.x; lp < H; lp += blockDim.x) {
+      float acc[NS][6];
+      fold_mode<MF, NS, GShared<NS>, false>(ip, jp, lp, N, HP, Np, m_rt, ikf, lnkf, cd1, cd2, gs, acc);
+      const size_t o = (size_t)row * HPn + lp;
+#pragma unroll
+      for (int s = 0; s < NS; s++)
+#pragma unroll
+        for (int a = 0; a < 3; a++) out[(3 * s + a) * nl + o] = make_float2(acc[s][2 * a], acc[s][2 * a + 1]);
+    }
+  }
+  for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < 2 * rows; q += gridDim.x * blockDim.x) {
+    const unsigned row = q >> 1;
+    const int lp = (q & 1) ? H : 0;
+    const int jp = (int)(row % (unsigned)Np), ip = (int)(row / (unsigned)Np);
+    float acc[NS][6];
+    fold_mode<MF, NS, GShared<NS>, true>(ip, jp, lp, N, HP, Np, m_rt, ikf, lnkf, cd1, cd2, gs, acc);
     const size_t o = (size_t)row * HPn + lp;
-    o0[o] = make_float2(acc[0], acc[1]);
-    o1[o] = make_float2(acc[2], acc[3]);
-    o2[o] = make_float2(acc[4], acc[5]);
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+#pragma unroll
+      for (int a = 0; a < 3; a++) out[(3 * s + a) * nl + o] = make_float2(acc[s][2 * a], acc[s][2 * a + 1]);
   }
 }
 
@@ -578,91 +614,186 @@ int ic_init_particles(mgpu_ctx *ctx, double Di, double Di2) {
   return MGPU_OK;
 }
 
-// LPT fields from the device growth tables n2tab[0], n2tab[1]
-// With sl1/sl2 (growth slices already in ctx->slice_buf) the lattice fold
-// evaluates the growth from them in LDS; otherwise from the n2 tables.
-static int lpt_fields_from_tables(mgpu_ctx *ctx, int slot, const KSl *sl1 = nullptr, const KSl *sl2 = nullptr) {
+// shared-grid slices as the fold kernel takes them (node float4s in ctx->slice_buf)
+struct SliceFold {
+  float y0, dy, idy, ymax, norm1, norm2;
+  int ny;
+};
+
+template <int MF, int NS>
+int launch_fold_gs(mgpu_ctx *ctx, const SliceFold &f, float lnkf, float ikf, float2 *out, size_t nl) {
+  GShared<NS> gs{};
+  gs.sl = nullptr;
+  gs.y0 = f.y0;
+  gs.dy = f.dy;
+  gs.idy = f.idy;
+  gs.ymax = f.ymax;
+  gs.norm1 = f.norm1;
+  gs.norm2 = f.norm2;
+  gs.ny = f.ny;
+  const int Np = ctx->Np;
+  const size_t lds = sizeof(float4) * (size_t)f.ny * (1 + NS);
+  const unsigned nb = (unsigned)std::min<size_t>((size_t)Np * Np, 2048);
+  hipLaunchKernelGGL((k_lpt_fold_gs<MF, NS>), dim3(nb), dim3(kBlock), lds, ctx->stream, ctx->cdelta[0],
+                     ctx->cdelta[1], (const float4 *)ctx->slice_buf, gs, lnkf, ctx->N, ctx->HP, Np,
+                     ctx->fft_lattice.hp, ctx->N / Np, ikf, out, nl);
+  MGPU_HIP(ctx, hipGetLastError());
+  return MGPU_OK;
+}
+
+// LPT fields of ns slots (slots[s]).  With sl (shared-grid growth slices of
+// every slot, already in ctx->slice_buf) the lattice fold evaluates the growth
+// in LDS; otherwise (ns = 1) from the n2 tables.
+static int lattice_fields(mgpu_ctx *ctx, int ns, const int *slots, const SliceFold *sl) {
   int rc;
   if (ctx->lattice_slot) {
-    // exact lattice fold: N = m * Nsample
+    // exact lattice fold: N = m * Nsample; spectrum (s, a) at lattice_slot + (3 s + a) nl
     const int Np = ctx->Np, HPn = ctx->fft_lattice.hp, m = ctx->N / Np;
     const size_t nl = (size_t)Np * Np * HPn;
-    float2 *o[3] = {ctx->lattice_slot, ctx->lattice_slot + nl, ctx->lattice_slot + 2 * nl};
-    const float ikf = (float)(ctx->p.box / (2 * MGPU_PI));
-    const size_t slds = sl1 ? 4 * sizeof(float) * (size_t)(sl1->ny + sl2->ny) : 0;
+    const float ikf = (float)(ctx->p.box / (2 * MGPU_PI)), lnkf = (float)std::log(2 * MGPU_PI / ctx->p.box);
     {
-      // compulsory: both stored spectra read once (the 1LPT one inside the IC cube only), 3 folded outputs written
+      // compulsory: both stored spectra read once (the 1LPT one inside the IC cube only), 3 ns folded outputs written
       const double modes = (double)ctx->N * ctx->N * (ctx->N / 2 + 1);
-      KScope ks(ctx, MGPU_K_LPT, 8.0 * modes * (1.0 + 1.0 / (double)(m * m * m)) + 3.0 * 8.0 * nl);
-      if (sl1 && slds <= 65536) {
-        auto kern = m == 1   ? k_lpt_fold_sl<1>
-                    : m == 2 ? k_lpt_fold_sl<2>
-                    : m == 3 ? k_lpt_fold_sl<3>
-                             : k_lpt_fold_sl<0>;
-        const KSlf f1{(float)sl1->y0, (float)sl1->dy, (float)sl1->norm, sl1->ny};
-        const KSlf f2{(float)sl2->y0, (float)sl2->dy, (float)sl2->norm, sl2->ny};
-        const size_t total = (size_t)Np * Np * (Np / 2 + 1);
-        const unsigned nb = (unsigned)std::min<size_t>(cdiv(total, kBlock), 8 * 256);
-        hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), slds, ctx->stream, ctx->cdelta[0], ctx->cdelta[1],
-                           (const double *)ctx->slice_buf, f1, f2, (float)std::log(2 * MGPU_PI / ctx->p.box), ctx->N,
-                           ctx->HP, Np, HPn, m, ikf, o[0], o[1], o[2]);
+      KScope ks(ctx, MGPU_K_FOLD, 8.0 * modes * (1.0 + 1.0 / (double)(m * m * m)) + 3.0 * ns * 8.0 * nl);
+      if (sl && ns == 2) {
+        rc = m == 1 ? launch_fold_gs<1, 2>(ctx, *sl, lnkf, ikf, ctx->lattice_slot, nl)
+             : m == 2 ? launch_fold_gs<2, 2>(ctx, *sl, lnkf, ikf, ctx->lattice_slot, nl)
+             : m == 3 ? launch_fold_gs<3, 2>(ctx, *sl, lnkf, ikf, ctx->lattice_slot, nl)
+                      : launch_fold_gs<0, 2>(ctx, *sl, lnkf, ikf, ctx->lattice_slot, nl);
+      } else if (sl) {
+        rc = m == 1 ? launch_fold_gs<1, 1>(ctx, *sl, lnkf, ikf, ctx->lattice_slot, nl)
+             : m == 2 ? launch_fold_gs<2, 1>(ctx, *sl, lnkf, ikf, ctx->lattice_slot, nl)
+             : m == 3 ? launch_fold_gs<3, 1>(ctx, *sl, lnkf, ikf, ctx->lattice_slot, nl)
+                      : launch_fold_gs<0, 1>(ctx, *sl, lnkf, ikf, ctx->lattice_slot, nl);
       } else {
         auto kern = m == 1 ? k_lpt_fold<1> : m == 2 ? k_lpt_fold<2> : m == 3 ? k_lpt_fold<3> : k_lpt_fold<0>;
         hipLaunchKernelGGL(kern, dim3((unsigned)((size_t)Np * Np), cdiv(Np / 2 + 1, kBlock)), dim3(kBlock), 0,
                            ctx->stream, ctx->cdelta[0], ctx->cdelta[1], GTab{ctx->n2tabf[0], ctx->n2tabf[1]}, ctx->N,
-                           ctx->HP, Np, HPn, m, ikf, o[0], o[1], o[2]);
+                           ctx->HP, Np, HPn, m, ikf, ctx->lattice_slot, nl);
+        rc = MGPU_OK;
       }
+      if (rc) return rc;
     }
     MGPU_HIP(ctx, hipGetLastError());
-    for (int a = 0; a < 3; a++)
-      if ((rc = fft_c2r_3d(ctx, &ctx->fft_lattice, o[a], ctx->lpt[slot] + (size_t)a * ctx->np))) return rc;
+    for (int s = 0; s < ns; s++)
+      for (int a = 0; a < 3; a++)
+        if ((rc = fft_c2r_3d(ctx, &ctx->fft_lattice, ctx->lattice_slot + (3 * s + a) * nl,
+                             ctx->lpt[slots[s]] + (size_t)a * ctx->np)))
+          return rc;
     return MGPU_OK;
   }
+  if (ns != 1) return mgpu_fail(ctx, MGPU_E_INVALID, "several LPT slots at once need the lattice fold");
   for (int a = 0; a < 3; a++) {
     if ((rc = spectrum(ctx, SPEC_FIELD, ctx->cdelta[0], ctx->cdelta[1], ctx->slot[3], a, 0, ctx->n2tab[0],
                        ctx->n2tab[1])))
       return rc;
     if ((rc = fft_c2r_3d(ctx, &ctx->fft, ctx->slot[3]))) return rc;
-    if ((rc = readout(ctx, ctx->slot[3], 1.0, ctx->lpt[slot] + (size_t)a * ctx->np))) return rc;
+    if ((rc = readout(ctx, ctx->slot[3], 1.0, ctx->lpt[slots[0]] + (size_t)a * ctx->np))) return rc;
   }
   return MGPU_OK;
 }
 
-int ic_lpt_fields_slices(mgpu_ctx *ctx, int slot, const mgpu_kslice *o1, const mgpu_kslice *o2) {
+// pinned host staging of n bytes; the previous upload from it must be complete before it is refilled
+static int slice_staging(mgpu_ctx *ctx, size_t bytes, void **host) {
+  if (ctx->slice_ev) MGPU_HIP(ctx, hipEventSynchronize(ctx->slice_ev));
+  else MGPU_HIP(ctx, hipEventCreateWithFlags(&ctx->slice_ev, hipEventDisableTiming));
+  const size_t nd = (bytes + sizeof(double) - 1) / sizeof(double);
+  if (ctx->slice_host_len < nd) {
+    if (ctx->slice_host) MGPU_HIP(ctx, hipHostFree(ctx->slice_host));
+    ctx->slice_host = nullptr;
+    MGPU_HIP(ctx, hipHostMalloc((void **)&ctx->slice_host, nd * sizeof(double), hipHostMallocDefault));
+    ctx->slice_host_len = nd;
+  }
+  if (!ctx->slice_buf || ctx->slice_len < nd) {
+    if (ctx->slice_buf) {
+      MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      MGPU_HIP(ctx, hipFree(ctx->slice_buf));
+    }
+    ctx->slice_buf = nullptr;
+    MGPU_HIP(ctx, hipMalloc(&ctx->slice_buf, nd * sizeof(double)));
+    ctx->slice_len = nd;
+  }
+  *host = ctx->slice_host;
+  return MGPU_OK;
+}
+static int slice_upload(mgpu_ctx *ctx, size_t bytes) {
+  MGPU_HIP(ctx, hipMemcpyAsync(ctx->slice_buf, ctx->slice_host, bytes, hipMemcpyHostToDevice, ctx->stream));
+  MGPU_HIP(ctx, hipEventRecord(ctx->slice_ev, ctx->stream));
+  return MGPU_OK;
+}
+
+// ns slots, slices q[2 s + o] for (slot s, order o + 1)
+int ic_lpt_fields_slices_n(mgpu_ctx *ctx, int ns, const int *slots, const mgpu_kslice *const *q) {
   PhaseTimer pt(ctx, &ctx->t.lpt_fields);
   if (!ctx->p.scale_dependent) return mgpu_fail(ctx, MGPU_E_STATE, "LPT fields need a scale-dependent context");
-  if (o1->nk < 2 || o2->nk < 2 || o1->nk > 8192 || o2->nk > 8192 || o1->dlnk <= 0 || o2->dlnk <= 0)
-    return mgpu_fail(ctx, MGPU_E_INVALID, "bad growth slice");
-  std::vector<double> h(4 * (size_t)(o1->nk + o2->nk));
-  double *p = h.data();
-  for (const mgpu_kslice *s : {o1, o2}) {
-    for (const double *src : {s->num, s->dnum, s->den, s->dden}) {
-      std::copy(src, src + s->nk, p);
-      p += s->nk;
+  for (int e = 0; e < 2 * ns; e++)
+    if (!q[e] || q[e]->nk < 2 || q[e]->nk > 8192 || q[e]->dlnk <= 0)
+      return mgpu_fail(ctx, MGPU_E_INVALID, "bad growth slice");
+  const int ny = q[0]->nk;
+  // the shared-grid fold needs one ln k grid for every slice, and (two slots) the
+  // same denominators and norms in both slots
+  bool shared = ctx->lattice_slot != nullptr && 16 * (size_t)ny * (1 + ns) <= 65536;
+  for (int e = 1; e < 2 * ns && shared; e++)
+    shared = q[e]->nk == ny && q[e]->lnk0 == q[0]->lnk0 && q[e]->dlnk == q[0]->dlnk;
+  for (int o = 0; o < 2 && shared && ns == 2; o++)
+    shared = q[2 + o]->norm == q[o]->norm && std::equal(q[o]->den, q[o]->den + ny, q[2 + o]->den) &&
+             std::equal(q[o]->dden, q[o]->dden + ny, q[2 + o]->dden);
+  if (!shared && ns == 2) {
+    int rc = ic_lpt_fields_slices_n(ctx, 1, slots, q);
+    return rc ? rc : ic_lpt_fields_slices_n(ctx, 1, slots + 1, q + 2);
+  }
+  int rc;
+  if (shared) {
+    const size_t bytes = sizeof(float) * 4 * (size_t)ny * (1 + ns);
+    void *h;
+    if ((rc = slice_staging(ctx, bytes, &h))) return rc;
+    float *f = (float *)h;
+    for (int j = 0; j < ny; j++) {
+      *f++ = (float)q[0]->den[j];
+      *f++ = (float)q[0]->dden[j];
+      *f++ = (float)q[1]->den[j];
+      *f++ = (float)q[1]->dden[j];
+      for (int s = 0; s < ns; s++) {
+        *f++ = (float)q[2 * s]->num[j];
+        *f++ = (float)q[2 * s]->dnum[j];
+        *f++ = (float)q[2 * s + 1]->num[j];
+        *f++ = (float)q[2 * s + 1]->dnum[j];
+      }
     }
+    if ((rc = slice_upload(ctx, bytes))) return rc;
+    SliceFold sf;
+    sf.y0 = (float)q[0]->lnk0;
+    sf.dy = (float)q[0]->dlnk;
+    sf.idy = (float)(1.0 / q[0]->dlnk);
+    sf.ymax = (float)(q[0]->lnk0 + q[0]->dlnk * (ny - 1));
+    sf.norm1 = (float)q[0]->norm;
+    sf.norm2 = (float)q[1]->norm;
+    sf.ny = ny;
+    return lattice_fields(ctx, ns, slots, &sf);
   }
-  if (!ctx->slice_buf || ctx->slice_len < h.size()) {
-    if (ctx->slice_buf) MGPU_HIP(ctx, hipFree(ctx->slice_buf));
-    ctx->slice_buf = nullptr;
-    MGPU_HIP(ctx, hipMalloc(&ctx->slice_buf, h.size() * sizeof(double)));
-    ctx->slice_len = h.size();
-  }
-  MGPU_HIP(ctx, hipMemcpyAsync(ctx->slice_buf, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
-                               ctx->stream));
-  KSl k1{o1->lnk0, o1->dlnk, o1->norm, o1->nk}, k2{o2->lnk0, o2->dlnk, o2->norm, o2->nk};
-  const bool lds_slices = ctx->lattice_slot && 4 * sizeof(float) * (size_t)(o1->nk + o2->nk) <= 65536;
-  if (!lds_slices) {
-    hipLaunchKernelGGL(k_growth_n2, dim3(cdiv(ctx->n2len, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->n2len,
-                       2 * MGPU_PI / ctx->p.box, k1, (const double *)ctx->slice_buf, k2,
-                       (const double *)ctx->slice_buf + 4 * o1->nk, ctx->n2tab[0], ctx->n2tab[1], ctx->n2tabf[0],
-                       ctx->n2tabf[1]);
-    MGPU_HIP(ctx, hipGetLastError());
-  }
-  int rc = lds_slices ? lpt_fields_from_tables(ctx, slot, &k1, &k2) : lpt_fields_from_tables(ctx, slot);
-  if (rc) return rc;
-  // the host buffer h must outlive the async copy
-  MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  return MGPU_OK;
+  // general slices: the per-|n|^2 growth tables first (FP64 Hermite), then the table path
+  const size_t total = 4 * (size_t)(q[0]->nk + q[1]->nk);
+  void *h;
+  if ((rc = slice_staging(ctx, total * sizeof(double), &h))) return rc;
+  double *p = (double *)h;
+  for (int e = 0; e < 2; e++)
+    for (const double *src : {q[e]->num, q[e]->dnum, q[e]->den, q[e]->dden}) {
+      std::copy(src, src + q[e]->nk, p);
+      p += q[e]->nk;
+    }
+  if ((rc = slice_upload(ctx, total * sizeof(double)))) return rc;
+  const KSl k0{q[0]->lnk0, q[0]->dlnk, q[0]->norm, q[0]->nk}, k1{q[1]->lnk0, q[1]->dlnk, q[1]->norm, q[1]->nk};
+  hipLaunchKernelGGL(k_growth_n2, dim3(cdiv(ctx->n2len, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->n2len,
+                     2 * MGPU_PI / ctx->p.box, k0, (const double *)ctx->slice_buf, k1,
+                     (const double *)ctx->slice_buf + 4 * q[0]->nk, ctx->n2tab[0], ctx->n2tab[1], ctx->n2tabf[0],
+                     ctx->n2tabf[1]);
+  MGPU_HIP(ctx, hipGetLastError());
+  return lattice_fields(ctx, 1, slots, nullptr);
+}
+
+int ic_lpt_fields_slices(mgpu_ctx *ctx, int slot, const mgpu_kslice *o1, const mgpu_kslice *o2) {
+  const mgpu_kslice *q[2] = {o1, o2};
+  return ic_lpt_fields_slices_n(ctx, 1, &slot, q);
 }
 
 int ic_lpt_fields(mgpu_ctx *ctx, int slot, const double *g1_n2, const double *g2_n2) {
@@ -671,7 +802,7 @@ int ic_lpt_fields(mgpu_ctx *ctx, int slot, const double *g1_n2, const double *g2
   int rc;
   if ((rc = upload_n2(ctx, 0, g1_n2))) return rc;
   if ((rc = upload_n2(ctx, 1, g2_n2))) return rc;
-  if ((rc = lpt_fields_from_tables(ctx, slot))) return rc;
+  if ((rc = lattice_fields(ctx, 1, &slot, nullptr))) return rc;
   MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return MGPU_OK;
 }

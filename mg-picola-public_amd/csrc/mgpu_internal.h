@@ -70,8 +70,14 @@ struct mgpu_ctx {
   float2 *lattice_slot = nullptr;  // 3 Nsample^3 half spectra
   double *slice_buf = nullptr;     // growth slices uploaded by mgpu_lpt_fields_slices
   size_t slice_len = 0;
-  void *dep_scratch = nullptr;     // tiled deposit: tile ids, ranks, sorted indices, halo faces
+  double *slice_host = nullptr;    // pinned staging buffer of the slices
+  size_t slice_host_len = 0;
+  hipEvent_t slice_ev = nullptr;   // the last upload from slice_host
+  void *dep_scratch = nullptr;     // tiled deposit: bin keys, ranks, sorted indices, bin offsets
   size_t dep_scratch_bytes = 0;
+  unsigned *dep_idx = nullptr;     // particle indices in tile order (inside dep_scratch)
+  unsigned *dep_off = nullptr;     // bin offsets of dep_idx, 8 bins (face classes) per tile
+  bool sorted_valid = false;       // dep_idx sorts the current positions
 
   // state
   bool have_deposit = false, have_density = false, have_forces = false, have_disp = false;
@@ -159,9 +165,10 @@ int fused_z_screen(mgpu_ctx *ctx, int density_kind, float2 *buf, const float *dx
 
 // ---------------------------------------------------------------- tiled deposit (deposit.hip)
 size_t deposit_scratch_bytes(int N, size_t np);
-// particles -> delta(x) in real layout in d, through the 64-bit fixed-point
-// accumulator acc (8 N^3 bytes, free scratch)
-int deposit_tiled(mgpu_ctx *ctx, unsigned long long *acc, float *d);
+// particles -> delta(x) in real layout in d (sums in 64-bit fixed point in LDS)
+int deposit_tiled(mgpu_ctx *ctx, float *d);
+// Disp from the three force meshes by the last deposit's bins (sorted_valid must hold)
+int gather_tiled(mgpu_ctx *ctx, const float *f0, const float *f1, const float *f2);
 
 // ---------------------------------------------------------------- PM kernels (pm.hip)
 int pm_deposit(mgpu_ctx *ctx);                         // particles -> delta(x) in slot0 (+ copy slot1)
@@ -180,5 +187,7 @@ int ic_displacement_fields(mgpu_ctx *ctx, unsigned long seed, int sphere_mode, i
 int ic_init_particles(mgpu_ctx *ctx, double Di, double Di2);
 int ic_lpt_fields(mgpu_ctx *ctx, int slot, const double *g1_n2, const double *g2_n2);
 int ic_lpt_fields_slices(mgpu_ctx *ctx, int slot, const mgpu_kslice *o1, const mgpu_kslice *o2);
+// several slots at once (one fold reads the stored spectra once): q[2 s + o] = slice of (slots[s], order o+1)
+int ic_lpt_fields_slices_n(mgpu_ctx *ctx, int ns, const int *slots, const mgpu_kslice *const *q);
 
 static inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }

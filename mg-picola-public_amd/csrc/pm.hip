@@ -3,6 +3,7 @@
 // function, the CIC gather, Kick and Drift.  Each kernel cites the reference
 // routine (HAWinther/MG-PICOLA-PUBLIC) whose arithmetic it carries out.
 #include "mgpu_internal.h"
+#include "cic.h"
 
 #include <cmath>
 #include <vector>
@@ -11,23 +12,6 @@ namespace {
 
 constexpr int kBlock = 256;
 
-// Cell index and fractional offset of a scaled coordinate X = pos * N / Box,
-// as src/auxPM.c:298-322 forms them.  Positions outside [0, Box) (never
-// produced by Drift's periodic wrap) are wrapped so every index stays in range.
-__device__ __forceinline__ void cell_of(double X, int N, unsigned &I, double &D) {
-  if (!(X >= 0.0 && X < (double)N)) {
-    if (!isfinite(X)) X = 0.0;
-    X -= floor(X / N) * N;
-    if (!(X >= 0.0 && X < (double)N)) X = 0.0;
-  }
-  I = (unsigned)X;
-  D = X - (double)I;
-  if (I >= (unsigned)N) I = 0;
-}
-
-// density = -1 + sum of weights (src/auxPM.c:292-293), written as FP32 into
-// the padded real layout of slot0 and, when the MG solver needs delta(x)
-// (CopyDensityArray, src/mg.h:381-386), also into slot1.
 // ------------------------------------------------------------------ P(k)
 // compute_power_spectrum (src/compute_pofk.c:71-236): every mode of the half
 // spectrum with weight 1 (kz = 0, N/2) or 2, |n| binned linearly or
@@ -105,15 +89,18 @@ __global__ __launch_bounds__(kBlock) void k_scale(const float2 *__restrict__ in,
 // ------------------------------------------------------------------ gather
 // MtoParticles (src/auxPM.c:560-644): CIC interpolation of the three force
 // meshes (WPAR = 1), Disp stored as float; per-block sums of the stored values
-// for the mean, reduced later in a fixed order.
+// for the mean, reduced later in a fixed order.  This direct form serves when
+// the deposit's bins do not describe the current positions; otherwise the
+// gather goes by tiles (gather_tiled) and k_sum3 forms the same partials.
 __global__ __launch_bounds__(kBlock) void k_gather(const float *__restrict__ pos, size_t np, int N, int HP2,
                                                    double scale, const float *__restrict__ g0,
                                                    const float *__restrict__ g1, const float *__restrict__ g2,
                                                    float *__restrict__ disp, double *__restrict__ partial) {
   __shared__ double red[3][kBlock];
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   double s[3] = {0, 0, 0};
-  if (i < np) {
+  if (t < np) {
+    const size_t i = t;
     unsigned IX, IY, IZ;
     double DX, DY, DZ;
     cell_of((double)pos[i] * scale, N, IX, DX);
@@ -140,6 +127,26 @@ __global__ __launch_bounds__(kBlock) void k_gather(const float *__restrict__ pos
   }
 #pragma unroll
   for (int a = 0; a < 3; a++) red[a][threadIdx.x] = s[a];
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int a = 0; a < 3; a++) red[a][threadIdx.x] += red[a][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int a = 0; a < 3; a++) partial[(size_t)blockIdx.x * 3 + a] = red[a][0];
+}
+
+// per-block sums of Disp in particle order (blocks of kBlock particles), the
+// partials k_gather forms, so sumD does not depend on the gather's order
+__global__ __launch_bounds__(kBlock) void k_sum3(const float *__restrict__ disp, size_t np,
+                                                 double *__restrict__ partial) {
+  __shared__ double red[3][kBlock];
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+  for (int a = 0; a < 3; a++) red[a][threadIdx.x] = t < np ? (double)disp[(size_t)a * np + t] : 0.0;
   __syncthreads();
   for (int w = kBlock / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w)
@@ -264,11 +271,10 @@ int pm_reduce3(mgpu_ctx *ctx, int nblocks, double out[3]) {
 int pm_deposit(mgpu_ctx *ctx) {
   PhaseTimer pt(ctx, &ctx->t.ptomesh);
   const bool need_dx = ctx->p.mg_active && ctx->p.model != MGPU_BRANSDICKE && ctx->p.model != MGPU_LCDM;
-  // the force meshes (slots 3..5) are free until Forces: the accumulator uses 8 N^3 bytes of them.
   // With MG screening delta(x) goes to slot 1, where it stays for the screening
   // pass, and the r2c writes delta(k) to slot 0.
   ctx->dx_slot = need_dx ? 1 : 0;
-  return deposit_tiled(ctx, (unsigned long long *)ctx->slot[3], (float *)ctx->slot[ctx->dx_slot]);
+  return deposit_tiled(ctx, (float *)ctx->slot[ctx->dx_slot]);
 }
 
 namespace {
@@ -461,13 +467,23 @@ int pm_gather(mgpu_ctx *ctx, double sumD[3]) {
   }
   const int N = ctx->N;
   const unsigned nb = cdiv(ctx->np, kBlock);
-  {
-    // compulsory: positions and Disp (12 B each per particle) and the three force meshes once
-    KScope ks(ctx, MGPU_K_GATHER, 24.0 * ctx->np + 12.0 * (double)N * N * N);
+  const float *f0 = (const float *)ctx->slot[ctx->force_slot[0]], *f1 = (const float *)ctx->slot[ctx->force_slot[1]],
+              *f2 = (const float *)ctx->slot[ctx->force_slot[2]];
+  // compulsory: positions and Disp (12 B each per particle) and the three force meshes once
+  const double bytes = 24.0 * ctx->np + 12.0 * (double)N * N * N;
+  if (ctx->sorted_valid) {
+    // by the deposit's tiles, then the sums in particle order
+    {
+      KScope ks(ctx, MGPU_K_GATHER, bytes);
+      int rc = gather_tiled(ctx, f0, f1, f2);
+      if (rc) return rc;
+    }
+    KScope ks(ctx, MGPU_K_MEAN, 12.0 * ctx->np);
+    hipLaunchKernelGGL(k_sum3, dim3(nb), dim3(kBlock), 0, ctx->stream, ctx->disp, ctx->np, ctx->partial);
+  } else {
+    KScope ks(ctx, MGPU_K_GATHER, bytes);
     hipLaunchKernelGGL(k_gather, dim3(nb), dim3(kBlock), 0, ctx->stream, ctx->pos, ctx->np, N, 2 * ctx->HP,
-                       (double)N / ctx->p.box, (const float *)ctx->slot[ctx->force_slot[0]],
-                       (const float *)ctx->slot[ctx->force_slot[1]], (const float *)ctx->slot[ctx->force_slot[2]],
-                       ctx->disp, ctx->partial);
+                       (double)N / ctx->p.box, f0, f1, f2, ctx->disp, ctx->partial);
   }
   MGPU_HIP(ctx, hipGetLastError());
   double s[3];
@@ -502,6 +518,7 @@ int pm_kick(mgpu_ctx *ctx, double dda, double A, double s1, double s2, const dou
 
 int pm_drift(mgpu_ctx *ctx, double dyyy, double s1, double s2, const double sumV[3]) {
   PhaseTimer pt(ctx, &ctx->t.drift);
+  ctx->sorted_valid = false;
   if (ctx->np == 0) return MGPU_OK;
   const int sd = ctx->p.scale_dependent ? 2 : 0;
   {

@@ -75,6 +75,17 @@ static int lpt_fields(sim *s, int slot, int field, double A, double AFF) {
   return 0;
 }
 
+/* both LPT slots at once: slot 0 gets field f0, slot 1 field f1 */
+static int lpt_fields2(sim *s, int f0, int f1, double A, double AFF) {
+  mgpu_kslice a1, a2, b1, b2;
+  if (mgp_lpt_kslices(s->c, f0, A, AFF, &a1, &a2, s->g1) || mgp_lpt_kslices(s->c, f1, A, AFF, &b1, &b2, s->g2)) {
+    snprintf(s->err, s->errlen, "no scale-dependent growth tables");
+    return -1;
+  }
+  GPU(mgpu_lpt_fields_slices2(s->g, &a1, &a2, &b1, &b2));
+  return 0;
+}
+
 /* Kick, src/main.c:688-742 */
 static int kick(sim *s, double AI, double AF, double A) {
   double dda;
@@ -217,7 +228,7 @@ mgp_sim *mgp_sim_create(mgp_params *p, int verbose, char *err, size_t errlen) {
   /* g1 also holds the 8 growth slices of 1000 ln k nodes (mgp_lpt_kslices) */
   const size_t n2len = mgpu_n2_table_len(p->Nmesh);
   s->g1 = malloc(sizeof(double) * (n2len > 8000 ? n2len : 8000));
-  s->g2 = malloc(sizeof(double) * n2len);
+  s->g2 = malloc(sizeof(double) * (n2len > 8000 ? n2len : 8000));
   return s;
 fail:
   snprintf(err, errlen, "%s", s->err);
@@ -260,8 +271,7 @@ int mgp_sim_ic(mgp_sim *s) {
     return -1;
   }
   if (p->scale_dependent) {
-    if (lpt_fields(s, 0, MGP_FIELD_D, s->A, s->AFF)) return -1;
-    if (lpt_fields(s, 1, MGP_FIELD_dDdy, s->A, s->AFF)) return -1;
+    if (lpt_fields2(s, MGP_FIELD_D, MGP_FIELD_dDdy, s->A, s->AFF)) return -1;
   }
   GPU(mgpu_init_particles(s->g, s->Di, s->Di2));
   s->sumV[0] = s->sumV[1] = s->sumV[2] = 0.0;
@@ -309,8 +319,7 @@ int mgp_sim_advance(mgp_sim *s, int *counted) {
   }
   if (get_displacements(s, A)) return -1;
   if (p->scale_dependent) {
-    if (lpt_fields(s, 1, MGP_FIELD_deltaD, A, s->AFF)) return -1;
-    if (lpt_fields(s, 0, MGP_FIELD_ddDddy, A, s->AFF)) return -1;
+    if (lpt_fields2(s, MGP_FIELD_ddDddy, MGP_FIELD_deltaD, A, s->AFF)) return -1;
   }
   if (kick(s, s->AI, s->AF, A)) return -1;
   if (s->ts == 0 && s->i != s->NoutputStart) {

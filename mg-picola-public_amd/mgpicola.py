@@ -25,6 +25,7 @@ MGPU_SYMBOLS = [
     "mgpu_get_displacements", "mgpu_kick", "mgpu_drift", "mgpu_ic_displacement_fields", "mgpu_init_particles",
     "mgpu_lpt_fields", "mgpu_get_mesh", "mgpu_set_mesh", "mgpu_get_timers", "mgpu_enable_timers",
     "mgpu_kernel_name", "mgpu_enable_kernel_timing", "mgpu_kernel_timing", "mgpu_lpt_fields_slices",
+    "mgpu_lpt_fields_slices2",
 ]
 
 
@@ -449,8 +450,9 @@ def run(hp, verbose=False):
     return {"step_seconds": st.step_seconds, "steps": st.steps, "final_A": st.final_A}
 
 
-KERNELS = ["deposit", "density", "fft_r2c", "fft_c2r", "fft_col_fwd", "fft_col_inv", "pofk", "spectral", "real",
-           "gather", "kick", "drift", "lpt", "readout", "fft_green", "fft_dual", "fft_screen", "fft_epi"]
+KERNELS = ["deposit", "mean", "fft_r2c", "fft_c2r", "fft_col_fwd", "fft_col_inv", "pofk", "spectral", "real",
+           "gather", "kick", "drift", "lpt", "readout", "fft_green", "fft_dual", "fft_screen", "fft_epi", "dep_sort",
+           "fold", "fft_pot", "fft_pk"]
 
 
 def _kt_setup():
@@ -569,3 +571,11 @@ def lpt_fields_slices(g, slot, o1, o2):
     L = lib()
     L.mgpu_lpt_fields_slices.argtypes = [C.c_void_p, C.c_int, C.POINTER(KSlice), C.POINTER(KSlice)]
     g._ck(L.mgpu_lpt_fields_slices(g.ctx, slot, C.byref(o1), C.byref(o2)), "lpt_fields_slices")
+
+
+def lpt_fields_slices2(g, s0, s1):
+    """Both LPT slots in one call: s0 = (order 1, order 2) slices of slot 0, s1 of slot 1."""
+    L = lib()
+    L.mgpu_lpt_fields_slices2.argtypes = [C.c_void_p] + [C.POINTER(KSlice)] * 4
+    g._ck(L.mgpu_lpt_fields_slices2(g.ctx, C.byref(s0[0]), C.byref(s0[1]), C.byref(s1[0]), C.byref(s1[1])),
+          "lpt_fields_slices2")

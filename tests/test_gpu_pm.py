@@ -55,6 +55,63 @@ def test_deposit_matches_oracle(n, ns):
     assert abs(dx.sum(dtype=np.float64)) < 1e-6 * n ** 3
 
 
+def clustered_particles(n, ns, box, seed):
+    """Half the particles in a tight clump straddling a corner shared by 8
+    tiles, some exactly on the box edges, the rest uniform."""
+    rng = np.random.default_rng(seed)
+    npart = ns ** 3
+    t = 16 if n % 16 == 0 else 8 if n % 8 == 0 else 4
+    h = box / n
+    p = rng.uniform(0.0, box, size=(3, npart))
+    k = npart // 2
+    p[:, :k] = t * h + rng.normal(0.0, 0.7 * h, size=(3, k))
+    p[:, k:k + 8] = 0.0
+    p[0, k + 8:k + 16] = np.nextafter(np.float32(box), np.float32(0))
+    p = np.mod(p, box).astype(np.float32)
+    p[p >= np.float32(box)] = np.nextafter(np.float32(box), np.float32(0))
+    return p
+
+
+@pytest.mark.parametrize("n,ns", [(12, 6), (16, 8), (24, 12), (32, 16)])
+def test_deposit_clustered_matches_oracle(n, ns):
+    """Tiles of 4, 8 and 16 cells (one, two or three per side), a clump over a
+    tile corner and particles on the box edges: the bins' face classes and the
+    periodic lower neighbours."""
+    p = clustered_particles(n, ns, BOX, seed=n)
+    print("n, ns =", n, ns, flush=True)
+    g = M.MGPU(n, ns, BOX, OM)
+    g.set_particles(p)
+    g.pm_deposit()
+    dx = g.get_mesh(M.MESH_DENSITY_X)
+    c = O.cfg(n, ns, BOX, OM)
+    ref = O.real_view(O.deposit(c, p), n)
+    err = np.abs(dx - ref) / np.maximum(1.0, np.abs(1.0 + ref))
+    assert err.max() < 1e-6, err.max()
+
+
+@pytest.mark.parametrize("n,ns", [(12, 6), (24, 12), (32, 16)])
+def test_tiled_gather_equals_direct_gather(n, ns):
+    """The gather by the deposit's tiles gives the same Disp and the same mean
+    (bit for bit) as the direct gather in particle order."""
+    p = clustered_particles(n, ns, BOX, seed=n + 1)
+    print("n, ns =", n, ns, flush=True)
+    g = M.MGPU(n, ns, BOX, OM)
+    g.set_particles(p)
+    g.pm_density()
+    g.forces()
+    s_tiled = g.mesh_to_particles()
+    d_tiled = g.get_disp()
+    g.set_particles(p)  # new positions: the bins no longer apply, the direct gather runs
+    s_direct = g.mesh_to_particles()
+    d_direct = g.get_disp()
+    assert np.array_equal(d_tiled, d_direct)
+    assert np.array_equal(s_tiled, s_direct)
+    c = O.cfg(n, ns, BOX, OM)
+    d, _, _ = oracle_density(c, p)
+    rdisp, _ = O.gather(c, p, O.forces(c, d, None))
+    assert rel_err(d_tiled, rdisp) < 2e-5
+
+
 def test_deposit_is_deterministic(pos):
     g = M.MGPU(N, NS, BOX, OM)
     g.set_particles(pos)

@@ -108,3 +108,27 @@ def test_lpt_slices_match_host_tables(tmp_path):
         M.lpt_fields_slices(g, 1, o1, o2)
         assert rel_err(g.get_lpt(1), ref) < 1e-6
     del rng
+
+
+@pytest.mark.parametrize("n,ns", [(32, 16), (24, 24), (48, 16), (32, 12)])
+def test_lpt_two_slot_fold_matches_single_slot(tmp_path, n, ns):
+    """The two-slot call (one pass over the stored spectra for both LPT slots)
+    equals two one-slot calls: lattice fold with m = 2, 1, 3 and the full-mesh
+    path (N not a multiple of Nsample)."""
+    hp = make_params(tmp_path, "lcdm_64_128", 4, Nmesh=n, Nsample=ns, Box=2.0 * n, modified_gravity_active=1,
+                     include_screening=1)
+    hp.scale_dependent = 1
+    cosmo = M.Cosmo(hp)
+    g = M.MGPU(n, ns, 2.0 * n, 0.3, model=M.FOFR, mg_active=1, include_screening=1, scale_dependent=1)
+    k = np.arange(M.n2_table_len(n))
+    g.ic_displacement_fields(5001, np.where(k > 0, 1e4 / (1.0 + 0.01 * k), 0.0))
+    a1, a2, ba = M.lpt_kslices(cosmo, M.FIELD_ddDddy, 0.5, 0.55)
+    b1, b2, bb = M.lpt_kslices(cosmo, M.FIELD_deltaD, 0.5, 0.55)
+    M.lpt_fields_slices(g, 0, a1, a2)
+    M.lpt_fields_slices(g, 1, b1, b2)
+    r0, r1 = g.get_lpt(0), g.get_lpt(1)
+    M.lpt_fields_slices2(g, (a1, a2), (b1, b2))
+    assert rel_err(g.get_lpt(0), r0) < 1e-6
+    assert rel_err(g.get_lpt(1), r1) < 1e-6
+    assert rel_err(r0, r1) > 1e-3  # the two slots hold different fields
+    del ba, bb
