@@ -44,14 +44,17 @@ __device__ __forceinline__ unsigned tile_of(const TileGeom &g, unsigned IX, unsi
 __device__ __forceinline__ float to_delta(unsigned long long v) { return (float)((double)v * kInvFix - 1.0); }
 
 // 1. bin key of every particle, 8 tile + face class (class bits: x, y, z local
-// coordinate = T - 1), and its rank inside the bin (wave-aggregated atomics)
+// coordinate = T - 1), and its rank inside the bin.  The lanes of a wave with
+// equal keys are grouped first (ballots only); then the group leaders add
+// their group sizes in one atomic instruction, so a wave waits for one memory
+// round trip however many bins it touches.
 __global__ __launch_bounds__(kTB) void k_dep_bin(const float *__restrict__ pos, size_t np, TileGeom g, double scale,
-                                                 unsigned *__restrict__ count, unsigned *__restrict__ tile,
+                                                 unsigned *__restrict__ count, unsigned *__restrict__ key,
                                                  unsigned *__restrict__ rank) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool active = i < np;
+  const bool valid = i < np;
   unsigned t = 0;
-  if (active) {
+  if (valid) {
     unsigned IX, IY, IZ;
     double D;
     cell_of((double)pos[i] * scale, g.N, IX, D);
@@ -62,25 +65,27 @@ __global__ __launch_bounds__(kTB) void k_dep_bin(const float *__restrict__ pos, 
     t = 8u * tile_of(g, IX, IY, IZ) + cls;
   }
   const int lane = threadIdx.x % warpSize;
-  unsigned r = 0;
-  const bool valid = active;
-  while (true) {
-    const unsigned long long amask = __ballot(active);
-    if (amask == 0) break;
-    const int leader = __ffsll((long long)amask) - 1;
+  bool pending = valid;
+  int leader_of = lane;
+  unsigned off_in = 0, gsize = 0;
+  for (unsigned long long rem = __ballot(pending); rem;) {
+    const int leader = __ffsll((long long)rem) - 1;
     const unsigned tl = __shfl(t, leader);
-    const unsigned long long same = __ballot(active && t == tl);
-    unsigned base = 0;
-    if (lane == leader) base = atomicAdd(&count[tl], (unsigned)__popcll(same));
-    base = __shfl(base, leader);
-    if (active && t == tl) {
-      r = base + (unsigned)__popcll(same & ((1ull << lane) - 1ull));
-      active = false;
+    const unsigned long long same = __ballot(pending && t == tl);
+    if (pending && t == tl) {
+      leader_of = leader;
+      off_in = (unsigned)__popcll(same & ((1ull << lane) - 1ull));
+      pending = false;
     }
+    if (lane == leader) gsize = (unsigned)__popcll(same);
+    rem &= ~same;
   }
+  unsigned base = 0;
+  if (valid && lane == leader_of) base = atomicAdd(&count[t], gsize);
+  base = __shfl(base, leader_of);
   if (valid) {
-    tile[i] = t;
-    rank[i] = r;
+    key[i] = t;
+    rank[i] = base + off_in;
   }
 }
 
@@ -130,14 +135,14 @@ __global__ __launch_bounds__(1024) void k_scan_add(unsigned *__restrict__ out, s
   if (i == 0) out[ntail] = total_tail;
 }
 
-// 3. particle indices in tile order
-__global__ __launch_bounds__(kTB) void k_dep_scatter(const unsigned *__restrict__ tile,
+// 3. particle records in bin order: (x, y, z, index as bits)
+__global__ __launch_bounds__(kTB) void k_dep_scatter(const unsigned *__restrict__ key,
                                                      const unsigned *__restrict__ rank,
-                                                     const unsigned *__restrict__ off, size_t np,
-                                                     unsigned *__restrict__ idx) {
+                                                     const unsigned *__restrict__ off, const float *__restrict__ pos,
+                                                     size_t np, float4 *__restrict__ rec) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
-  idx[off[tile[i]] + rank[i]] = (unsigned)i;
+  rec[off[key[i]] + rank[i]] = make_float4(pos[i], pos[np + i], pos[2 * np + i], __uint_as_float((unsigned)i));
 }
 
 // 4. one workgroup per tile.  The particle lists it reads are up to 64 bin
@@ -146,8 +151,7 @@ __global__ __launch_bounds__(kTB) void k_dep_scatter(const unsigned *__restrict_
 // read through shift s has local coordinate (I mod T) - s T, so its corners at
 // local coordinate -1 (s = 1) or T (s = 0) fall outside the tile and are
 // skipped; every corner is added by exactly one tile.
-__global__ __launch_bounds__(kTB) void k_dep_tile(const float *__restrict__ pos, size_t np,
-                                                  const unsigned *__restrict__ idx, const unsigned *__restrict__ off,
+__global__ __launch_bounds__(kTB) void k_dep_tile(const float4 *__restrict__ rec, const unsigned *__restrict__ off,
                                                   TileGeom g, double scale, double wpar, int HP2,
                                                   float *__restrict__ d) {
   extern __shared__ unsigned long long blk[];
@@ -181,13 +185,12 @@ __global__ __launch_bounds__(kTB) void k_dep_tile(const float *__restrict__ pos,
   __syncthreads();
   const unsigned total = rpre[64];
   const int tm = T - 1;
-  // U virtual indices per thread and round: all index loads, then all position
-  // loads, are in flight before the first deposit
+  // U virtual indices per thread and round: all record loads are in flight
+  // before the first deposit
   constexpr int U = 4;
   for (unsigned base = 0; base < total; base += U * blockDim.x) {
-    unsigned ii[U];
     int sh[U];
-    float px[U], py[U], pz[U];
+    float4 pr[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const unsigned v = min(base + u * blockDim.x + threadIdx.x, total - 1);
@@ -197,22 +200,16 @@ __global__ __launch_bounds__(kTB) void k_dep_tile(const float *__restrict__ pos,
       for (int stp = 32; stp > 0; stp >>= 1)
         if (rpre[q + stp] <= v) q += stp;
       sh[u] = q >> 3;
-      ii[u] = idx[rstart[q] + (v - rpre[q])];
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      px[u] = pos[ii[u]];
-      py[u] = pos[np + ii[u]];
-      pz[u] = pos[2 * np + ii[u]];
+      pr[u] = rec[rstart[q] + (v - rpre[q])];
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       if (base + u * blockDim.x + threadIdx.x >= total) break;
       unsigned IX, IY, IZ;
       double DX, DY, DZ;
-      cell_of((double)px[u] * scale, g.N, IX, DX);
-      cell_of((double)py[u] * scale, g.N, IY, DY);
-      cell_of((double)pz[u] * scale, g.N, IZ, DZ);
+      cell_of((double)pr[u].x * scale, g.N, IX, DX);
+      cell_of((double)pr[u].y * scale, g.N, IY, DY);
+      cell_of((double)pr[u].z * scale, g.N, IZ, DZ);
       const double TX = 1.0 - DX, TZ = 1.0 - DZ;
       const double TY = (1.0 - DY) * wpar;
       DY *= wpar;
@@ -246,8 +243,7 @@ __global__ __launch_bounds__(kTB) void k_dep_tile(const float *__restrict__ pos,
 // dealt to the 8 XCDs round robin), so the +1 halo a tile reads was mostly
 // fetched into the same L2 by its neighbour.
 template <int T>
-__global__ __launch_bounds__(512) void k_gather_tile(const float *__restrict__ pos, size_t np,
-                                                     const unsigned *__restrict__ idx,
+__global__ __launch_bounds__(512) void k_gather_tile(const float4 *__restrict__ rec, size_t np,
                                                      const unsigned *__restrict__ off, int N, int nt1, int HP2,
                                                      double scale, const float *__restrict__ g0,
                                                      const float *__restrict__ g1, const float *__restrict__ g2,
@@ -258,6 +254,10 @@ __global__ __launch_bounds__(512) void k_gather_tile(const float *__restrict__ p
   const unsigned t = nt % 8 == 0 ? (blockIdx.x & 7) * (nt >> 3) + (blockIdx.x >> 3) : blockIdx.x;
   const int ti = t / (nt1 * nt1), tj = (t / nt1) % nt1, tk = t % nt1;
   const int X0 = ti * T, Y0 = tj * T, Z0 = tk * T;
+  // the thread's first particle record is requested before the mesh fill
+  const unsigned p1 = off[8 * t + 8];
+  unsigned q = off[8 * t] + threadIdx.x;
+  float4 nxt = q < p1 ? rec[q] : make_float4(0.f, 0.f, 0.f, 0.f);
   // rows (lx, ly) of each mesh: T contiguous cells, then the z = T column.
   // Every load of the thread is issued before the first LDS store, so the
   // fill costs one memory latency, not one per row group.
@@ -300,14 +300,15 @@ __global__ __launch_bounds__(512) void k_gather_tile(const float *__restrict__ p
     }
   }
   __syncthreads();
-  const unsigned p1 = off[8 * t + 8];
-  for (unsigned q = off[8 * t] + threadIdx.x; q < p1; q += blockDim.x) {
-    const size_t i = idx[q];
+  for (; q < p1; q += NT) {
+    const float4 r = nxt;
+    if (q + NT < p1) nxt = rec[q + NT];
+    const size_t i = __float_as_uint(r.w);
     unsigned IX, IY, IZ;
     double DX, DY, DZ;
-    cell_of((double)pos[i] * scale, N, IX, DX);
-    cell_of((double)pos[np + i] * scale, N, IY, DY);
-    cell_of((double)pos[2 * np + i] * scale, N, IZ, DZ);
+    cell_of((double)r.x * scale, N, IX, DX);
+    cell_of((double)r.y * scale, N, IY, DY);
+    cell_of((double)r.z * scale, N, IZ, DZ);
     const double TX = 1.0 - DX, TY = 1.0 - DY, TZ = 1.0 - DZ;
     // the particle was binned into this tile, so 0 <= local index < T
     const int b00 = ((int)(IX - X0) * B + (int)(IY - Y0)) * B + (int)(IZ - Z0), b01 = b00 + B, b10 = b00 + B2,
@@ -337,12 +338,12 @@ TileGeom make_geom(int N) {
 
 }  // namespace
 
-// scratch sizes for the tiled deposit: bin counters + offsets, scan sums, bin keys, ranks, indices
+// scratch sizes for the tiled deposit: bin counters + offsets, scan sums, bin keys, ranks, records
 size_t deposit_scratch_bytes(int N, size_t np) {
   TileGeom g = make_geom(N);
   const size_t nb = 8 * (size_t)g.ntx * g.nty * g.ntz;
   const size_t nsum = (nb + 1023) / 1024;
-  return sizeof(unsigned) * (2 * (nb + 1) + nsum + 3 * np) + 256 * 8;
+  return sizeof(unsigned) * (2 * (nb + 1) + nsum + 2 * np) + sizeof(float4) * np + 256 * 8;
 }
 
 int deposit_tiled(mgpu_ctx *ctx, float *d) {
@@ -352,6 +353,7 @@ int deposit_tiled(mgpu_ctx *ctx, float *d) {
   const size_t nt = (size_t)g.ntx * g.nty * g.ntz, nb = 8 * nt;
   const size_t nsum = (nb + 1023) / 1024;
   if (nsum > 1024 * 1024 || nb > 0xffffffffull) return mgpu_fail(ctx, MGPU_E_INVALID, "too many deposit tiles");
+  if (np > 0xffffffffull) return mgpu_fail(ctx, MGPU_E_INVALID, "more than 2^32 - 1 particles");
   // carve the scratch buffer (256-byte aligned pieces)
   char *b = (char *)ctx->dep_scratch;
   auto take = [&](size_t bytes) {
@@ -364,11 +366,11 @@ int deposit_tiled(mgpu_ctx *ctx, float *d) {
   unsigned *sums = (unsigned *)take(sizeof(unsigned) * nsum);
   unsigned *key = (unsigned *)take(sizeof(unsigned) * np);
   unsigned *rank = (unsigned *)take(sizeof(unsigned) * np);
-  unsigned *idx = (unsigned *)take(sizeof(unsigned) * np);
+  float4 *rec = (float4 *)take(sizeof(float4) * np);
   if ((size_t)(b - (char *)ctx->dep_scratch) > ctx->dep_scratch_bytes)
     return mgpu_fail(ctx, MGPU_E_INVALID, "deposit scratch too small");
-  // the sorted indices stay valid for the gather until the positions change
-  ctx->dep_idx = idx;
+  // the records stay valid for the gather until the positions change
+  ctx->dep_rec = rec;
   ctx->dep_off = off;
   ctx->sorted_valid = np > 0;
   const double scale = (double)N / ctx->p.box;
@@ -386,16 +388,17 @@ int deposit_tiled(mgpu_ctx *ctx, float *d) {
   MGPU_HIP(ctx, hipGetLastError());
   if (np) {
     KScope ks(ctx, MGPU_K_DEP_SORT, 12.0 * np);
-    hipLaunchKernelGGL(k_dep_scatter, dim3(cdiv(np, kTB)), dim3(kTB), 0, ctx->stream, key, rank, off, np, idx);
+    hipLaunchKernelGGL(k_dep_scatter, dim3(cdiv(np, kTB)), dim3(kTB), 0, ctx->stream, key, rank, off, ctx->pos, np,
+                       rec);
   }
   MGPU_HIP(ctx, hipGetLastError());
   const size_t shmem = sizeof(unsigned long long) * g.tx * g.ty * g.tz;
   const double cells = (double)N * N * N;
   {
-    // compulsory: indices + positions read once, delta written
+    // compulsory: the particle records read once, delta written
     KScope ks(ctx, MGPU_K_DEPOSIT, 16.0 * np + 4.0 * cells);
-    hipLaunchKernelGGL(k_dep_tile, dim3(nt), dim3(kTB), shmem, ctx->stream, ctx->pos, np, idx, off, g, scale, wpar,
-                       2 * ctx->HP, d);
+    hipLaunchKernelGGL(k_dep_tile, dim3(nt), dim3(kTB), shmem, ctx->stream, rec, off, g, scale, wpar, 2 * ctx->HP,
+                       d);
   }
   MGPU_HIP(ctx, hipGetLastError());
   return MGPU_OK;
@@ -410,8 +413,8 @@ int gather_tiled(mgpu_ctx *ctx, const float *f0, const float *f1, const float *f
   const unsigned nt = (unsigned)nt1 * nt1 * nt1;
   const double scale = (double)N / ctx->p.box;
   auto kern = T == 16 ? k_gather_tile<16> : T == 8 ? k_gather_tile<8> : k_gather_tile<4>;
-  hipLaunchKernelGGL(kern, dim3(nt), dim3(512), 0, ctx->stream, ctx->pos, ctx->np, ctx->dep_idx, ctx->dep_off, N,
-                     nt1, 2 * ctx->HP, scale, f0, f1, f2, ctx->disp);
+  hipLaunchKernelGGL(kern, dim3(nt), dim3(512), 0, ctx->stream, ctx->dep_rec, ctx->np, ctx->dep_off, N, nt1,
+                     2 * ctx->HP, scale, f0, f1, f2, ctx->disp);
   MGPU_HIP(ctx, hipGetLastError());
   return MGPU_OK;
 }

@@ -75,9 +75,9 @@ struct mgpu_ctx {
   hipEvent_t slice_ev = nullptr;   // the last upload from slice_host
   void *dep_scratch = nullptr;     // tiled deposit: bin keys, ranks, sorted indices, bin offsets
   size_t dep_scratch_bytes = 0;
-  unsigned *dep_idx = nullptr;     // particle indices in tile order (inside dep_scratch)
-  unsigned *dep_off = nullptr;     // bin offsets of dep_idx, 8 bins (face classes) per tile
-  bool sorted_valid = false;       // dep_idx sorts the current positions
+  float4 *dep_rec = nullptr;       // particle records (x, y, z, index bits) in bin order (inside dep_scratch)
+  unsigned *dep_off = nullptr;     // bin offsets of dep_rec, 8 bins (face classes) per tile
+  bool sorted_valid = false;       // dep_rec holds the current positions
 
   // state
   bool have_deposit = false, have_density = false, have_forces = false, have_disp = false;
