@@ -132,7 +132,11 @@ int mgpu_forces(mgpu_ctx *ctx);
 /* MtoParticles: CIC gather into Disp, mean over particles.  src/auxPM.c:560-644 */
 int mgpu_mesh_to_particles(mgpu_ctx *ctx, double sumD[3]);
 /* GetDisplacements = PtoMesh + ComputeFifthForce + Forces + MtoParticles.
- * mg may be NULL when modified gravity is off.  src/auxPM.c:37-103 */
+ * mg may be NULL when modified gravity is off.  src/auxPM.c:37-103
+ * For LCDM and the screened f(R), m-beta and DGP solvers the x passes run in
+ * pairs (forward x, multiplier, inverse x in one kernel), so phi(k) is never
+ * stored: MGPU_MESH_PHI_K is only defined after mgpu_fifth_force.  The other
+ * meshes, Disp and the P(k) are those of the four separate calls. */
 int mgpu_get_displacements(mgpu_ctx *ctx, const mgpu_mg *mg, mgpu_pofk *pk, double sumD[3]);
 /* Kick: v += dda [-1.5 Omega (Disp - sumD) - UseCOLA lpt/A]; returns the new
  * mean velocity.  Not scale-dependent lpt = slot0*s1 + slot1*s2 with
@@ -203,13 +207,15 @@ int mgpu_enable_timers(mgpu_ctx *ctx, int on);
  * inverse of the Nmesh transforms (the Nsample-lattice transforms count as
  * MGPU_K_LPT), P(k), ..., the FFT passes with fused multipliers (Green's force
  * x pass, dual y pass, z screening pass, x pass with the MG epilogue), the
- * deposit's binning and sort, the lattice fold, the potential/smoothing x pass
- * and the density x pass with P(k) binning. */
+ * deposit's binning and sort, the lattice fold, the potential/smoothing x pass,
+ * the density x pass with P(k) binning, and the two x-pass pairs of the fused
+ * step (density forward x + potential/smoothing or force inverse x; effective
+ * density forward x + MG epilogue + force inverse x). */
 enum {
   MGPU_K_DEPOSIT = 0, MGPU_K_MEAN, MGPU_K_FFT_R2C, MGPU_K_FFT_C2R, MGPU_K_FFT_COL_FWD, MGPU_K_FFT_COL_INV,
   MGPU_K_POFK, MGPU_K_SPECTRAL, MGPU_K_REAL, MGPU_K_GATHER, MGPU_K_KICK, MGPU_K_DRIFT, MGPU_K_LPT, MGPU_K_READOUT,
   MGPU_K_FFT_GREEN, MGPU_K_FFT_DUAL, MGPU_K_FFT_SCREEN, MGPU_K_FFT_EPI, MGPU_K_DEP_SORT, MGPU_K_FOLD,
-  MGPU_K_FFT_POT, MGPU_K_FFT_PK, MGPU_K_COUNT
+  MGPU_K_FFT_POT, MGPU_K_FFT_PK, MGPU_K_FFT_XDENS, MGPU_K_FFT_XFORCE, MGPU_K_COUNT
 };
 const char *mgpu_kernel_name(int kernel);
 int mgpu_enable_kernel_timing(mgpu_ctx *ctx, int on);

@@ -88,7 +88,7 @@ const char *mgpu_kernel_name(int k) {
                                             "fft_col_inv", "pofk",      "spectral", "real",    "gather",
                                             "kick",       "drift",      "lpt",      "readout", "fft_green",
                                             "fft_dual",   "fft_screen", "fft_epi",  "dep_sort", "fold",
-                                            "fft_pot",    "fft_pk"};
+                                            "fft_pot",    "fft_pk",     "fft_xdens", "fft_xforce"};
   return (k >= 0 && k < MGPU_K_COUNT) ? names[k] : "?";
 }
 
@@ -214,8 +214,28 @@ int mgpu_create(const mgpu_params *p, mgpu_ctx **out) {
     if ((rc = alloc(ctx, (void **)&ctx->winv, winv.size() * sizeof(double), "P(k) window"))) return fail(rc);
     if (hipMemcpy(ctx->winv, winv.data(), winv.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
       return fail(mgpu_fail(ctx, MGPU_E_HIP, "window upload failed"));
+    std::vector<float> winvf(winv.begin(), winv.end());
+    if ((rc = alloc(ctx, (void **)&ctx->winvf, winvf.size() * sizeof(float), "P(k) window"))) return fail(rc);
+    if (hipMemcpy(ctx->winvf, winvf.data(), winvf.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(mgpu_fail(ctx, MGPU_E_HIP, "window upload failed"));
   }
   ctx->n2len = mgpu_n2_table_len(ctx->N);
+  if (p->pofk_nbins > 0 && p->pofk_nbins <= 32767) {
+    // P(k) bin of every |n|^2, by compute_power_spectrum's formula in FP64
+    // (src/compute_pofk.c:112-126); -1 outside the bins
+    std::vector<int16_t> tab(ctx->n2len);
+    const double kmin = p->pofk_kmin, kmax = p->pofk_kmax, lnrange = p->pofk_bintype == 1 ? std::log(kmax / kmin) : 1.0;
+    for (size_t n2 = 0; n2 < ctx->n2len; n2++) {
+      const double kmag = std::sqrt((double)n2);
+      int nk;
+      if (p->pofk_bintype == 0) nk = (int)((kmag - kmin) / (kmax - kmin) * p->pofk_nbins + 0.5);
+      else nk = kmag <= 0.0 ? -1 : (int)(std::log(kmag / kmin) / lnrange * p->pofk_nbins + 0.5);
+      tab[n2] = (int16_t)(nk < 0 || nk >= p->pofk_nbins ? -1 : nk);
+    }
+    if ((rc = alloc(ctx, (void **)&ctx->pk_bin, tab.size() * sizeof(int16_t), "P(k) bin table"))) return fail(rc);
+    if (hipMemcpy(ctx->pk_bin, tab.data(), tab.size() * sizeof(int16_t), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(mgpu_fail(ctx, MGPU_E_HIP, "bin table upload failed"));
+  }
   for (int s = 0; s < 2; s++) {
     if ((rc = alloc(ctx, (void **)&ctx->n2tab[s], ctx->n2len * sizeof(double), "n2 table"))) return fail(rc);
     if ((rc = alloc(ctx, (void **)&ctx->n2tabf[s], ctx->n2len * sizeof(float), "n2 table"))) return fail(rc);
@@ -250,7 +270,7 @@ void mgpu_destroy(mgpu_ctx *ctx) {
                   ctx->slot[0],  ctx->slot[1],   ctx->slot[2],   ctx->slot[3],   ctx->cdelta[0],
                   ctx->cdelta[1], ctx->partial,  ctx->dsum,      ctx->pk,        ctx->n2tab[0],
                   ctx->n2tab[1], ctx->n2tabf[0], ctx->n2tabf[1], ctx->lattice_slot, ctx->slice_buf,
-                  ctx->dep_scratch, ctx->winv};
+                  ctx->dep_scratch, ctx->winv,    ctx->winvf,     ctx->pk_bin};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (ctx->slice_host) (void)hipHostFree(ctx->slice_host);
@@ -374,6 +394,16 @@ int mgpu_mesh_to_particles(mgpu_ctx *ctx, double sumD[3]) {
 
 int mgpu_get_displacements(mgpu_ctx *ctx, const mgpu_mg *mg, mgpu_pofk *pk, double sumD[3]) {
   int rc;
+  if (check_ctx(ctx) || !sumD) return MGPU_E_INVALID;
+  if (ctx->p.mg_active && ctx->p.model != MGPU_LCDM && !mg)
+    return mgpu_fail(ctx, MGPU_E_INVALID, "modified gravity active but no mgpu_mg coefficients given");
+  if (pm_step_fusable(ctx, mg, pk)) {
+    ctx->have_deposit = ctx->have_density = ctx->have_forces = false;
+    ctx->have_phik = false;
+    if ((rc = pm_step_fused(ctx, mg, pk, sumD))) return rc;
+    ctx->have_density = ctx->have_forces = ctx->have_disp = true;
+    return MGPU_OK;
+  }
   if ((rc = mgpu_pm_density(ctx, pk))) return rc;
   if (ctx->p.mg_active && (rc = mgpu_fifth_force(ctx, mg))) return rc;
   if ((rc = mgpu_forces(ctx))) return rc;

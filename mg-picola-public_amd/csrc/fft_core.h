@@ -140,7 +140,10 @@ constexpr int elems_per_thread(int L) { return (L % 3 == 0) ? (L < 24 ? L : 24) 
 // barrier separates those reads from the first exchange write.  With pre !=
 // nullptr the first stage's operands are already in registers, pre[q*R + r]
 // holding what ld(b, j, r*M) would return for butterfly q (see first_loads).
-template <int L, int B, int NT, int SIGN, int BS, int ES, bool BFAST, bool IN_LDS, int Ns, class LD, class ST>
+// With SYNC_ST a barrier precedes the last stage's stores, so st may write the
+// same LDS buffer (every thread has read its last-stage operands by then).
+template <int L, int B, int NT, int SIGN, int BS, int ES, bool BFAST, bool IN_LDS, int Ns, bool SYNC_ST = false,
+          class LD, class ST>
 __device__ __forceinline__ void fft_stages(float2 *lds, const float2 *__restrict__ tw, int twstride, int tid,
                                            const LD &ld, const ST &st, const float2 *pre = nullptr) {
   constexpr int E = L * B / NT;
@@ -194,6 +197,7 @@ __device__ __forceinline__ void fft_stages(float2 *lds, const float2 *__restrict
   for (int q = 0; q < NB; ++q) Bfly<R, SIGN>::run(v[q]);
   if constexpr (LAST) {
     // last stage: output index j + r*Ns with j < Ns
+    if constexpr (SYNC_ST) __syncthreads();
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
       const int g = tid + q * NT;
@@ -215,7 +219,7 @@ __device__ __forceinline__ void fft_stages(float2 *lds, const float2 *__restrict
       for (int r = 0; r < R; ++r) lds[b * BS + (d + r * Ns) * ES] = v[q][r];
     }
     __syncthreads();
-    fft_stages<L, B, NT, SIGN, BS, ES, BFAST, false, Ns * R>(lds, tw, twstride, tid, ld, st);
+    fft_stages<L, B, NT, SIGN, BS, ES, BFAST, false, Ns * R, SYNC_ST>(lds, tw, twstride, tid, ld, st);
   }
 }
 

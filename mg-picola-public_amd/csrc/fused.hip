@@ -46,6 +46,27 @@ __device__ __forceinline__ float window(int filter, float kR) {
   }
 }
 
+// The x pass's Green's / smoothing multiplier of S at (n_x, n^2), valid = the
+// column is a stored mode (kz <= N/2): o1 = G (or i n_x G for GREEN_FORCE,
+// o2 = G), G = f S.
+template <int MODE>
+__device__ __forceinline__ void green_mul(float2 S, int nx, int n2, bool valid, const GreenArgs &ga, float2 &o1,
+                                          float2 &o2) {
+  float f = 0.f;
+  if (valid) {
+    if (MODE == GREEN_FORCE) f = n2 > 0 ? -ga.cnorm / (float)n2 : 0.f;
+    if (MODE == GREEN_LAPLACE) f = n2 > 0 ? -ga.nf / (float)n2 : 0.f;
+    if (MODE == GREEN_SMOOTH) f = window(ga.filter, sqrtf((float)n2) * ga.kfac) * ga.nf;
+  }
+  const float2 G = cscale(S, f);
+  if (MODE == GREEN_FORCE) {
+    o1 = cscale(mul_i<+1>(G), (float)nx);
+    o2 = G;
+  } else {
+    o1 = G;
+  }
+}
+
 // x pass, inverse.  Tile: columns kz = blockIdx.x*T + c, outer index ky = blockIdx.y,
 // line index kx.  out1 is written by GREEN_FORCE only.
 template <int L, int T, int MODE>
@@ -82,20 +103,7 @@ __global__ __launch_bounds__(col_threads(L, T)) void k_col_green(const float2 *_
     for (int r = 0; r < R0; ++r) {
       const int nx = nsgn(j + r * M0, N);
       const int n2 = nx * nx + ny * ny + kz * kz;
-      const float2 S = p1[q * R0 + r];
-      float f = 0.f;
-      if (kz <= N / 2) {
-        if (MODE == GREEN_FORCE) f = n2 > 0 ? -ga.cnorm / (float)n2 : 0.f;
-        if (MODE == GREEN_LAPLACE) f = n2 > 0 ? -ga.nf / (float)n2 : 0.f;
-        if (MODE == GREEN_SMOOTH) f = window(ga.filter, sqrtf((float)n2) * ga.kfac) * ga.nf;
-      }
-      const float2 G = cscale(S, f);
-      if (MODE == GREEN_FORCE) {
-        p1[q * R0 + r] = cscale(mul_i<+1>(G), (float)nx);
-        p2[q * R0 + r] = G;
-      } else {
-        p1[q * R0 + r] = G;
-      }
+      green_mul<MODE>(p1[q * R0 + r], nx, n2, kz <= N / 2, ga, p1[q * R0 + r], p2[q * R0 + r]);
     }
   }
   auto ld = [&](int, int, int) { return make_float2(0.f, 0.f); };
@@ -169,14 +177,14 @@ __global__ __launch_bounds__(col_threads(L, T)) void k_col_epi(float2 *buf, size
 template <int L, int T>
 __global__ __launch_bounds__(col_threads(L, T)) void k_col_pofk(float2 *buf, size_t ls, size_t os,
                                                                   const float2 *__restrict__ tw,
-                                                                  const double *__restrict__ winv, int nbins,
-                                                                  int bintype, double kmin, double kmax,
+                                                                  const float *__restrict__ winv,
+                                                                  const int16_t *__restrict__ pbin, int nbins,
                                                                   double *__restrict__ pk) {
   extern __shared__ float2 lds[];
   constexpr int NT = col_threads(L, T);
   constexpr int N = L;
   double *bins = (double *)(lds + (size_t)L * T);
-  double *wv = bins + nbins;
+  float *wv = (float *)(bins + nbins);
   const int tid = threadIdx.x;
   for (int b = tid; b < nbins; b += NT) bins[b] = 0.0;
   for (int d = tid; d <= N / 2; d += NT) wv[d] = winv[d];
@@ -184,25 +192,132 @@ __global__ __launch_bounds__(col_threads(L, T)) void k_col_pofk(float2 *buf, siz
   const int ky = blockIdx.y, kz0 = blockIdx.x * T;
   const size_t tile = (size_t)ky * os + kz0;
   const int d1 = ky > N / 2 ? N - ky : ky;
-  const double lnrange = bintype == 1 ? log(kmax / kmin) : 1.0;
   auto ld = [&](int c, int j, int o) { return buf[tile + (size_t)j * ls + c + (size_t)o * ls]; };
   auto st = [&](int c, int j, int o, float2 v) {
     buf[tile + (size_t)j * ls + c + (size_t)o * ls] = v;
     const int kz = kz0 + c, kx = j + o;
     if (kz > N / 2) return;
     const int d0 = kx > N / 2 ? N - kx : kx;
-    const double kmag = sqrt((double)(d0 * d0 + d1 * d1 + kz * kz));
-    int nk;
-    if (bintype == 0) nk = (int)((kmag - kmin) / (kmax - kmin) * nbins + 0.5);
-    else nk = kmag <= 0.0 ? -1 : (int)(log(kmag / kmin) / lnrange * nbins + 0.5);
-    if (nk < 0 || nk >= nbins) return;
-    const double w = (kz == 0 || kz == N / 2) ? 1.0 : 2.0;
-    atomicAdd(&bins[nk], w * ((double)v.x * v.x + (double)v.y * v.y) * wv[d0] * wv[d1] * wv[kz]);
+    // the bin of |n|^2 comes from the table the host fills with the FP64 formula
+    const int nk = pbin[d0 * d0 + d1 * d1 + kz * kz];
+    if (nk < 0) return;
+    const float w = (kz == 0 || kz == N / 2) ? 1.f : 2.f;
+    atomicAdd(&bins[nk], (double)((v.x * v.x + v.y * v.y) * (w * wv[d0] * wv[d1] * wv[kz])));
   };
   fft_stages<L, T, NT, -1, 1, T, true, false, 1>(lds, tw, 1, tid, ld, st);
   __syncthreads();
   for (int b = tid; b < nbins; b += NT)
     if (bins[b] != 0.0) atomicAdd(&pk[b], bins[b]);
+}
+
+// x pass pair: a forward x transform and, from its output in LDS, an inverse
+// x transform with a Green's / smoothing multiplier, so the spectrum between
+// them is never written and read back.
+//   FWD_DENSITY  the density's forward x pass: delta(k) stored to kout (when
+//                not null) and its power binned into pk (when not null) as
+//                k_col_pofk does; S = delta(k).
+//   FWD_EPI      forward x pass of the screened effective density, the
+//                EffDensitykToPhiofk epilogue (as k_col_epi), S = delta(k) + phi(k)
+//                with delta(k) read from dk (as k_col_green adds them).
+//   FWD_PLAIN    the same without the epilogue (DGP: phi(k) is the transform).
+// Then S times the MODE multiplier and the inverse x transform(s), as
+// k_col_green.  The forward's last stage stores its outputs into LDS in the
+// tile layout (after a barrier), from where each thread takes its first-stage
+// operands of the inverse.
+enum { FWD_DENSITY = 0, FWD_EPI = 1, FWD_PLAIN = 2 };
+template <int L, int T, int FWD, int MODE>
+__global__ __launch_bounds__(col_threads(L, T)) void k_col_fused(
+    const float2 *in, float2 *kout, const float2 *__restrict__ dk, float2 *out0, float2 *out1, size_t ls, size_t os,
+    const float2 *__restrict__ tw, GreenArgs ga, float coupling, float m2, const float *__restrict__ winv,
+    const int16_t *__restrict__ pbin, int nbins, double *__restrict__ pk) {
+  extern __shared__ float2 lds[];
+  constexpr int NT = col_threads(L, T), E = col_ept(L, T);
+  constexpr int R0 = pick_radix(L, E), NB0 = E / R0, M0 = L / R0;
+  constexpr int N = L;
+  const int tid = threadIdx.x;
+  // blocks are dealt to the 8 XCDs round robin: give each XCD a contiguous
+  // range of tiles, so tiles that share 128-byte lines (T = 8) meet in one L2
+  const unsigned nbx = gridDim.x, nblk = gridDim.x * gridDim.y, id = blockIdx.x + blockIdx.y * nbx;
+  const unsigned lin = nblk % 8 == 0 ? (id % 8) * (nblk / 8) + id / 8 : id;
+  const int ky = (int)(lin / nbx), kz0 = (int)(lin % nbx) * T;
+  const size_t tile = (size_t)ky * os + kz0;
+  const int ny = nsgn(ky, N), d1 = ky > N / 2 ? N - ky : ky;
+  const bool bin = FWD == FWD_DENSITY && pk != nullptr;
+  double *bins = (double *)(lds + (size_t)L * T);
+  float *wv = (float *)(bins + nbins);
+  if (bin) {
+    for (int b = tid; b < nbins; b += NT) bins[b] = 0.0;
+    for (int d = tid; d <= N / 2; d += NT) wv[d] = winv[d];
+    __syncthreads();
+  }
+  // forward transform; its outputs (row kx of column c) go to lds[c + kx T]
+  auto ld = [&](int c, int j, int o) { return in[tile + (size_t)j * ls + c + (size_t)o * ls]; };
+  auto st = [&](int c, int j, int o, float2 v) {
+    const int kx = j + o, kz = kz0 + c;
+    if (FWD == FWD_DENSITY) {
+      if (kout) kout[tile + (size_t)kx * ls + c] = v;
+      if (bin && kz <= N / 2) {
+        const int d0 = kx > N / 2 ? N - kx : kx;
+        const int nk = pbin[d0 * d0 + d1 * d1 + kz * kz];
+        if (nk >= 0) {
+          const float w = (kz == 0 || kz == N / 2) ? 1.f : 2.f;
+          atomicAdd(&bins[nk], (double)((v.x * v.x + v.y * v.y) * (w * wv[d0] * wv[d1] * wv[kz])));
+        }
+      }
+    }
+    if (FWD == FWD_EPI) {
+      const int nx = nsgn(kx, N), n2 = nx * nx + ny * ny + kz * kz;
+      const float f = (kz <= N / 2 && n2 > 0) ? coupling * ((float)n2 / ((float)n2 + m2)) : 0.f;
+      v = cscale(v, f);
+    }
+    lds[c + kx * T] = v;
+  };
+  fft_stages<L, T, NT, -1, 1, T, true, false, 1, true>(lds, tw, 1, tid, ld, st);
+  // delta(k) to add, in the inverse's first-stage layout
+  float2 p1[E], p2[E];
+  if (FWD != FWD_DENSITY) {
+#pragma unroll
+    for (int q = 0; q < NB0; ++q) {
+      const int g = tid + q * NT, c = g % T, j = g / T;
+#pragma unroll
+      for (int r = 0; r < R0; ++r) p2[q * R0 + r] = dk[tile + (size_t)(j + r * M0) * ls + c];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NB0; ++q) {
+    const int g = tid + q * NT, c = g % T, j = g / T;
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+      const float2 v = lds[c + (j + r * M0) * T];
+      p1[q * R0 + r] = FWD == FWD_DENSITY ? v : cadd(p2[q * R0 + r], v);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NB0; ++q) {
+    const int g = tid + q * NT, c = g % T, j = g / T;
+    const int kz = kz0 + c;
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+      const int nx = nsgn(j + r * M0, N);
+      const int n2 = nx * nx + ny * ny + kz * kz;
+      green_mul<MODE>(p1[q * R0 + r], nx, n2, kz <= N / 2, ga, p1[q * R0 + r], p2[q * R0 + r]);
+    }
+  }
+  auto ldz = [&](int, int, int) { return make_float2(0.f, 0.f); };
+  auto st0 = [&](int c, int j, int o, float2 v) { out0[tile + (size_t)j * ls + c + (size_t)o * ls] = v; };
+  fft_stages<L, T, NT, +1, 1, T, true, false, 1>(lds, tw, 1, tid, ldz, st0, p1);
+  if (MODE == GREEN_FORCE) {
+    __syncthreads();
+    auto st1 = [&](int c, int j, int o, float2 v) { out1[tile + (size_t)j * ls + c + (size_t)o * ls] = v; };
+    fft_stages<L, T, NT, +1, 1, T, true, false, 1>(lds, tw, 1, tid, ldz, st1, p2);
+  }
+  if (bin) {
+    __syncthreads();
+    for (int b = tid; b < nbins; b += NT)
+      if (bins[b] != 0.0) atomicAdd(&pk[b], bins[b]);
+  }
 }
 
 enum { SCREEN_POTENTIAL = 0, SCREEN_DENSITY = 1 };
@@ -292,6 +407,43 @@ int green_launch(mgpu_ctx *ctx, int mode, const float2 *a, const float2 *b, floa
   return MGPU_OK;
 }
 
+struct FusedXArgs {
+  int fwd, mode;
+  const float2 *in;
+  float2 *kout;
+  const float2 *dk;
+  float2 *o0, *o1;
+  GreenArgs ga;
+  float coupling, m2;
+  double *pk;
+};
+
+template <int N, int T, int FWD, int MODE>
+int fused_x_launch_k(mgpu_ctx *ctx, const FusedXArgs &a) {
+  const size_t HP = ctx->fft.hp, ls = (size_t)N * HP, os = HP;
+  const int nbins = a.pk ? ctx->p.pofk_nbins : 0;
+  const size_t lds = (size_t)N * T * sizeof(float2) + (a.pk ? sizeof(double) * nbins + sizeof(float) * (N / 2 + 1) : 0);
+  if (lds > ctx->lds_max) return mgpu_fail(ctx, MGPU_E_INVALID, "P(k) bins do not fit the LDS of the x pass");
+  if (a.pk && !ctx->pk_bin) return mgpu_fail(ctx, MGPU_E_INVALID, "no P(k) bin table");
+  int rc;
+  auto kern = k_col_fused<N, T, FWD, MODE>;
+  if ((rc = set_lds(ctx, kern, lds))) return rc;
+  hipLaunchKernelGGL(kern, dim3(HP / T, N), dim3(col_threads(N, T)), lds, ctx->stream, a.in, a.kout, a.dk, a.o0, a.o1,
+                     ls, os, ctx->fft.tw, a.ga, a.coupling, a.m2, ctx->winvf, ctx->pk_bin, nbins, a.pk);
+  MGPU_HIP(ctx, hipGetLastError());
+  return MGPU_OK;
+}
+
+template <int N, int T>
+int fused_x_launch(mgpu_ctx *ctx, const FusedXArgs &a) {
+  if (a.fwd == FWD_DENSITY && a.mode == GREEN_LAPLACE) return fused_x_launch_k<N, T, FWD_DENSITY, GREEN_LAPLACE>(ctx, a);
+  if (a.fwd == FWD_DENSITY && a.mode == GREEN_SMOOTH) return fused_x_launch_k<N, T, FWD_DENSITY, GREEN_SMOOTH>(ctx, a);
+  if (a.fwd == FWD_DENSITY && a.mode == GREEN_FORCE) return fused_x_launch_k<N, T, FWD_DENSITY, GREEN_FORCE>(ctx, a);
+  if (a.fwd == FWD_EPI && a.mode == GREEN_FORCE) return fused_x_launch_k<N, T, FWD_EPI, GREEN_FORCE>(ctx, a);
+  if (a.fwd == FWD_PLAIN && a.mode == GREEN_FORCE) return fused_x_launch_k<N, T, FWD_PLAIN, GREEN_FORCE>(ctx, a);
+  return mgpu_fail(ctx, MGPU_E_INVALID, "unsupported fused x pass");
+}
+
 template <int N, int T>
 int dual_launch(mgpu_ctx *ctx, float2 *buf, float2 *oy) {
   const size_t HP = ctx->fft.hp, ls = HP, os = (size_t)N * HP;
@@ -317,15 +469,16 @@ int epi_launch(mgpu_ctx *ctx, float2 *buf, float coupling, float m2) {
 }
 
 template <int N, int T>
-int pofk_launch(mgpu_ctx *ctx, float2 *buf, const double *winv, int nbins, int bintype, double kmin, double kmax,
-                double *pk) {
+int pofk_launch(mgpu_ctx *ctx, float2 *buf, double *pk) {
   const size_t HP = ctx->fft.hp, ls = (size_t)N * HP, os = HP;
-  const size_t lds = (size_t)N * T * sizeof(float2) + sizeof(double) * ((size_t)nbins + N / 2 + 1);
+  const int nbins = ctx->p.pofk_nbins;
+  const size_t lds = (size_t)N * T * sizeof(float2) + sizeof(double) * nbins + sizeof(float) * (N / 2 + 1);
   if (lds > ctx->lds_max) return mgpu_fail(ctx, MGPU_E_INVALID, "P(k) bins do not fit the LDS of the x pass");
+  if (!ctx->pk_bin) return mgpu_fail(ctx, MGPU_E_INVALID, "no P(k) bin table");
   int rc;
   if ((rc = set_lds(ctx, k_col_pofk<N, T>, lds))) return rc;
   hipLaunchKernelGGL((k_col_pofk<N, T>), dim3(HP / T, N), dim3(col_threads(N, T)), lds, ctx->stream, buf, ls, os,
-                     ctx->fft.tw, winv, nbins, bintype, kmin, kmax, pk);
+                     ctx->fft.tw, ctx->winvf, ctx->pk_bin, nbins, pk);
   MGPU_HIP(ctx, hipGetLastError());
   return MGPU_OK;
 }
@@ -370,9 +523,11 @@ struct DualF {
 };
 template <int N, int T>
 struct PofkF {
-  static int run(mgpu_ctx *c, float2 *buf, const double *w, int nb, int bt, double k0, double k1, double *pk) {
-    return pofk_launch<N, T>(c, buf, w, nb, bt, k0, k1, pk);
-  }
+  static int run(mgpu_ctx *c, float2 *buf, double *pk) { return pofk_launch<N, T>(c, buf, pk); }
+};
+template <int N, int T>
+struct FusedXF {
+  static int run(mgpu_ctx *c, const FusedXArgs &a) { return fused_x_launch<N, T>(c, a); }
 };
 template <int N, int T>
 struct EpiF {
@@ -396,6 +551,40 @@ int fused_green_x(mgpu_ctx *ctx, int mode, const float2 *a, const float2 *b, flo
 #define X(S) \
   case S:    \
     return by_tile<S, GreenF>(ctx, mode, a, b, o0, o1, ga);
+    MGPU_FFT_SIZES(X)
+#undef X
+    default:
+      return mgpu_fail(ctx, MGPU_E_INVALID, "unsupported FFT size");
+  }
+}
+
+int fused_x_pair(mgpu_ctx *ctx, int fwd, int mode, const float2 *in, float2 *kout, const float2 *dk, float2 *o0,
+                 float2 *o1, double p0, double p1, int filter, double coupling, double massterm2, double *pk) {
+  FusedXArgs a;
+  a.fwd = fwd;
+  a.mode = mode;
+  a.in = in;
+  a.kout = kout;
+  a.dk = dk;
+  a.o0 = o0;
+  a.o1 = o1;
+  a.ga = GreenArgs{0.f, 0.f, 0.f, filter};
+  if (mode == GREEN_FORCE) a.ga.cnorm = (float)p0;
+  else a.ga.nf = (float)p0;
+  a.ga.kfac = (float)p1;
+  a.coupling = (float)coupling;
+  a.m2 = (float)massterm2;
+  a.pk = pk;
+  // compulsory: the input (and delta(k)) read, the outputs (and delta(k)) written
+  const double modes = (double)ctx->N * ctx->N * (ctx->N / 2 + 1);
+  const int nio = 1 + (dk ? 1 : 0) + (kout ? 1 : 0) + (mode == GREEN_FORCE ? 2 : 1);
+  KScope ks(ctx, fwd == FWD_DENSITY ? MGPU_K_FFT_XDENS : MGPU_K_FFT_XFORCE, 8.0 * modes * nio);
+  switch (ctx->N) {
+#define X(S)                                                               \
+  case S:                                                                  \
+    /* the force pairs hold G through the first inverse: 8-column tiles */ \
+    if (mode == GREEN_FORCE) return FusedXF<S, 8>::run(ctx, a);            \
+    return by_tile<S, FusedXF>(ctx, a);
     MGPU_FFT_SIZES(X)
 #undef X
     default:
@@ -446,14 +635,13 @@ int fused_z_screen(mgpu_ctx *ctx, int density_kind, float2 *buf, const float *dx
   }
 }
 
-int fused_pofk_x(mgpu_ctx *ctx, float2 *buf, const double *winv, int nbins, int bintype, double kmin, double kmax,
-                 double *pk) {
+int fused_pofk_x(mgpu_ctx *ctx, float2 *buf, double *pk) {
   const double modes = (double)ctx->N * ctx->N * (ctx->N / 2 + 1);
   KScope ks(ctx, MGPU_K_FFT_PK, 16.0 * modes);
   switch (ctx->N) {
 #define X(S) \
   case S:    \
-    return by_tile<S, PofkF>(ctx, buf, winv, nbins, bintype, kmin, kmax, pk);
+    return by_tile<S, PofkF>(ctx, buf, pk);
     MGPU_FFT_SIZES(X)
 #undef X
     default:

@@ -60,6 +60,8 @@ struct mgpu_ctx {
   double *dsum = nullptr;     // final sums
   double *pk = nullptr;       // 3 * nbins
   double *winv = nullptr;     // P(k) deconvolution per axis: 1/sinc(pi d/N)^4 N^-2, d = 0..N/2
+  float *winvf = nullptr;     // FP32 copy of winv
+  int16_t *pk_bin = nullptr;  // P(k) bin of each |n|^2 (-1: outside the bins); null when nbins > 32767
   std::vector<double> pk_geom;  // mode-weighted |n| sums and mode counts per bin (2 * nbins), fixed per context
   double *n2tab[2] = {nullptr, nullptr};  // per-|n|^2 device tables
   float *n2tabf[2] = {nullptr, nullptr};  // FP32 copies of n2tab (lattice fold)
@@ -158,8 +160,15 @@ int fused_dual_y(mgpu_ctx *ctx, float2 *buf, float2 *oy);
 // forward x pass then times coupling n^2/(n^2 + massterm2)
 int fused_epi_phi_x(mgpu_ctx *ctx, float2 *buf, double coupling, double massterm2);
 // forward x pass of the density r2c, accumulating the binned P(k) power into pk[nbins]
-int fused_pofk_x(mgpu_ctx *ctx, float2 *buf, const double *winv, int nbins, int bintype, double kmin, double kmax,
-                 double *pk);
+int fused_pofk_x(mgpu_ctx *ctx, float2 *buf, double *pk);
+// x pass pair (forward x transform, multiplier, inverse x transform(s)) in one kernel:
+// fwd FUSED_FWD_DENSITY (in -> delta(k) to kout if not null, P(k) into pk if not null),
+// FUSED_FWD_EPI (in = effective density; phi = coupling n^2/(n^2+m^2) X, S = dk + phi) or
+// FUSED_FWD_PLAIN (S = dk + X); then mode FUSED_GREEN_* of S (p0, p1, filter as for
+// fused_green_x) into o0 (and o1 for FUSED_GREEN_FORCE)
+enum { FUSED_FWD_DENSITY = 0, FUSED_FWD_EPI = 1, FUSED_FWD_PLAIN = 2 };
+int fused_x_pair(mgpu_ctx *ctx, int fwd, int mode, const float2 *in, float2 *kout, const float2 *dk, float2 *o0,
+                 float2 *o1, double p0, double p1, int filter, double coupling, double massterm2, double *pk);
 // z pass: c2r, screening against delta(x) (dx, real layout), r2c; in place on buf
 int fused_z_screen(mgpu_ctx *ctx, int density_kind, float2 *buf, const float *dx, double p0, double p1);
 
@@ -176,6 +185,9 @@ int pm_pofk(mgpu_ctx *ctx, const float2 *dk, mgpu_pofk *out);
 int pm_transform(mgpu_ctx *ctx, mgpu_pofk *out);  // r2c of delta(x) into slot 0 (+ P(k))
 int pm_fifth_force(mgpu_ctx *ctx, const mgpu_mg *mg);
 int pm_forces(mgpu_ctx *ctx);
+// the whole of GetDisplacements with paired x passes (pm_step_fusable says when it applies)
+bool pm_step_fusable(const mgpu_ctx *ctx, const mgpu_mg *mg, const mgpu_pofk *out);
+int pm_step_fused(mgpu_ctx *ctx, const mgpu_mg *mg, mgpu_pofk *out, double sumD[3]);
 int pm_gather(mgpu_ctx *ctx, double sumD[3]);
 int pm_kick(mgpu_ctx *ctx, double dda, double A, double s1, double s2, const double sumD[3], double sumV[3]);
 int pm_drift(mgpu_ctx *ctx, double dyyy, double s1, double s2, const double sumV[3]);

@@ -17,8 +17,9 @@ constexpr int kBlock = 256;
 // spectrum with weight 1 (kz = 0, N/2) or 2, |n| binned linearly or
 // logarithmically, CIC deconvolution 1/(W_x W_y W_z)^4 and 1/N^6.
 __global__ __launch_bounds__(kBlock) void k_pofk(const float2 *__restrict__ dk, int N, int HP,
-                                                 const double *__restrict__ winv, int nbins, int bintype,
-                                                 double kmin, double kmax, double *__restrict__ out) {
+                                                 const double *__restrict__ winv, const int16_t *__restrict__ pbin,
+                                                 int nbins, int bintype, double kmin, double kmax,
+                                                 double *__restrict__ out) {
   extern __shared__ double sh[];  // 3*nbins
   for (int b = threadIdx.x; b < 3 * nbins; b += blockDim.x) sh[b] = 0.0;
   __syncthreads();
@@ -30,9 +31,11 @@ __global__ __launch_bounds__(kBlock) void k_pofk(const float2 *__restrict__ dk, 
     const size_t rowi = m / H;
     const int j = (int)(rowi % N), i = (int)(rowi / N);
     const int d0 = i > N / 2 ? N - i : i, d1 = j > N / 2 ? N - j : j, d2 = k;
-    const double kmag = sqrt((double)(d0 * d0 + d1 * d1 + d2 * d2));
+    const int n2 = d0 * d0 + d1 * d1 + d2 * d2;
+    const double kmag = sqrt((double)n2);
     int nk;
-    if (bintype == 0) nk = (int)((kmag - kmin) / (kmax - kmin) * nbins + 0.5);
+    if (pbin) nk = pbin[n2];  // the same formula, evaluated by the host
+    else if (bintype == 0) nk = (int)((kmag - kmin) / (kmax - kmin) * nbins + 0.5);
     else nk = kmag <= 0.0 ? -1 : (int)(log(kmag / kmin) / lnrange * nbins + 0.5);
     if (nk < 0 || nk >= nbins) continue;
     const double w = (k == 0 || k == N / 2) ? 1.0 : 2.0;
@@ -87,76 +90,82 @@ __global__ __launch_bounds__(kBlock) void k_scale(const float2 *__restrict__ in,
 }
 
 // ------------------------------------------------------------------ gather
+// Sums of 3 per-particle values in a fixed order: each block takes kChunk
+// consecutive particles, each thread kRed of them at stride kBlock (in order),
+// then a shuffle tree per wave and the waves in order.  The partials
+// [block][3] are reduced by k_reduce3, so the means do not depend on launch
+// order or timing.
+constexpr int kRed = 8, kChunk = kBlock * kRed;
+
+__device__ __forceinline__ void block_sum3(double s[3], double *__restrict__ partial) {
+  __shared__ double red[3][kBlock / 64];
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s[a] += __shfl_down(s[a], o);
+    if (lane == 0) red[a][w] = s[a];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      double t = red[a][0];
+      for (int q = 1; q < kBlock / 64; q++) t += red[a][q];
+      partial[(size_t)blockIdx.x * 3 + a] = t;
+    }
+}
+
 // MtoParticles (src/auxPM.c:560-644): CIC interpolation of the three force
-// meshes (WPAR = 1), Disp stored as float; per-block sums of the stored values
-// for the mean, reduced later in a fixed order.  This direct form serves when
-// the deposit's bins do not describe the current positions; otherwise the
-// gather goes by tiles (gather_tiled) and k_sum3 forms the same partials.
+// meshes (WPAR = 1), Disp stored as float.  This direct form serves when the
+// deposit's bins do not describe the current positions; otherwise the gather
+// goes by tiles (gather_tiled).  Either way k_sum3 forms the sums for the mean.
 __global__ __launch_bounds__(kBlock) void k_gather(const float *__restrict__ pos, size_t np, int N, int HP2,
                                                    double scale, const float *__restrict__ g0,
                                                    const float *__restrict__ g1, const float *__restrict__ g2,
-                                                   float *__restrict__ disp, double *__restrict__ partial) {
-  __shared__ double red[3][kBlock];
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double s[3] = {0, 0, 0};
-  if (t < np) {
-    const size_t i = t;
-    unsigned IX, IY, IZ;
-    double DX, DY, DZ;
-    cell_of((double)pos[i] * scale, N, IX, DX);
-    cell_of((double)pos[np + i] * scale, N, IY, DY);
-    cell_of((double)pos[2 * np + i] * scale, N, IZ, DZ);
-    const double TX = 1.0 - DX, TY = 1.0 - DY, TZ = 1.0 - DZ;
-    const unsigned IXn = IX + 1 >= (unsigned)N ? 0 : IX + 1;
-    const unsigned IYn = IY + 1 >= (unsigned)N ? 0 : IY + 1;
-    const unsigned IZn = IZ + 1 >= (unsigned)N ? 0 : IZ + 1;
-    const size_t r00 = ((size_t)IX * N + IY) * HP2, r01 = ((size_t)IX * N + IYn) * HP2;
-    const size_t r10 = ((size_t)IXn * N + IY) * HP2, r11 = ((size_t)IXn * N + IYn) * HP2;
-    const double w000 = TX * TY * TZ, w001 = TX * TY * DZ, w010 = TX * DY * TZ, w011 = TX * DY * DZ;
-    const double w100 = DX * TY * TZ, w101 = DX * TY * DZ, w110 = DX * DY * TZ, w111 = DX * DY * DZ;
-    const float *g[3] = {g0, g1, g2};
+                                                   float *__restrict__ disp) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  unsigned IX, IY, IZ;
+  double DX, DY, DZ;
+  cell_of((double)pos[i] * scale, N, IX, DX);
+  cell_of((double)pos[np + i] * scale, N, IY, DY);
+  cell_of((double)pos[2 * np + i] * scale, N, IZ, DZ);
+  const double TX = 1.0 - DX, TY = 1.0 - DY, TZ = 1.0 - DZ;
+  const unsigned IXn = IX + 1 >= (unsigned)N ? 0 : IX + 1;
+  const unsigned IYn = IY + 1 >= (unsigned)N ? 0 : IY + 1;
+  const unsigned IZn = IZ + 1 >= (unsigned)N ? 0 : IZ + 1;
+  const size_t r00 = ((size_t)IX * N + IY) * HP2, r01 = ((size_t)IX * N + IYn) * HP2;
+  const size_t r10 = ((size_t)IXn * N + IY) * HP2, r11 = ((size_t)IXn * N + IYn) * HP2;
+  const double w000 = TX * TY * TZ, w001 = TX * TY * DZ, w010 = TX * DY * TZ, w011 = TX * DY * DZ;
+  const double w100 = DX * TY * TZ, w101 = DX * TY * DZ, w110 = DX * DY * TZ, w111 = DX * DY * DZ;
+  const float *g[3] = {g0, g1, g2};
 #pragma unroll
-    for (int a = 0; a < 3; a++) {
-      const float *m = g[a];
-      const double d = m[r00 + IZ] * w000 + m[r00 + IZn] * w001 + m[r01 + IZ] * w010 + m[r01 + IZn] * w011 +
-                       m[r10 + IZ] * w100 + m[r10 + IZn] * w101 + m[r11 + IZ] * w110 + m[r11 + IZn] * w111;
-      const float df = (float)d;
-      disp[(size_t)a * np + i] = df;
-      s[a] = (double)df;
-    }
+  for (int a = 0; a < 3; a++) {
+    const float *m = g[a];
+    const double d = m[r00 + IZ] * w000 + m[r00 + IZn] * w001 + m[r01 + IZ] * w010 + m[r01 + IZn] * w011 +
+                     m[r10 + IZ] * w100 + m[r10 + IZn] * w101 + m[r11 + IZ] * w110 + m[r11 + IZn] * w111;
+    disp[(size_t)a * np + i] = (float)d;
   }
-#pragma unroll
-  for (int a = 0; a < 3; a++) red[a][threadIdx.x] = s[a];
-  __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
-#pragma unroll
-      for (int a = 0; a < 3; a++) red[a][threadIdx.x] += red[a][threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0)
-#pragma unroll
-    for (int a = 0; a < 3; a++) partial[(size_t)blockIdx.x * 3 + a] = red[a][0];
 }
 
-// per-block sums of Disp in particle order (blocks of kBlock particles), the
-// partials k_gather forms, so sumD does not depend on the gather's order
+// sums of Disp for the mean, in particle order (see block_sum3)
 __global__ __launch_bounds__(kBlock) void k_sum3(const float *__restrict__ disp, size_t np,
                                                  double *__restrict__ partial) {
-  __shared__ double red[3][kBlock];
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t base = (size_t)blockIdx.x * kChunk + threadIdx.x;
+  double s[3] = {0, 0, 0};
 #pragma unroll
-  for (int a = 0; a < 3; a++) red[a][threadIdx.x] = t < np ? (double)disp[(size_t)a * np + t] : 0.0;
-  __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
+  for (int a = 0; a < 3; a++) {
+    float v[kRed];
 #pragma unroll
-      for (int a = 0; a < 3; a++) red[a][threadIdx.x] += red[a][threadIdx.x + w];
-    __syncthreads();
+    for (int r = 0; r < kRed; r++) {
+      const size_t i = base + (size_t)r * kBlock;
+      v[r] = i < np ? disp[(size_t)a * np + i] : 0.f;
+    }
+#pragma unroll
+    for (int r = 0; r < kRed; r++) s[a] += (double)v[r];
   }
-  if (threadIdx.x == 0)
-#pragma unroll
-    for (int a = 0; a < 3; a++) partial[(size_t)blockIdx.x * 3 + a] = red[a][0];
+  block_sum3(s, partial);
 }
 
 // fixed-order reduction of per-block partials [nb][3] -> out[3]
@@ -194,11 +203,13 @@ __global__ __launch_bounds__(kBlock) void k_kick(float *__restrict__ vel, float 
                                                  double A, int use_cola, double sD0, double sD1, double sD2,
                                                  double *__restrict__ partial) {
 #pragma clang fp contract(off)  // same rounding sequence as the CPU reference
-  __shared__ double red[3][kBlock];
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t base = (size_t)blockIdx.x * kChunk + threadIdx.x;
   double s[3] = {0, 0, 0};
-  if (i < np) {
-    const double sD[3] = {sD0, sD1, sD2};
+  const double sD[3] = {sD0, sD1, sD2};
+#pragma unroll 2
+  for (int r = 0; r < kRed; r++) {
+    const size_t i = base + (size_t)r * kBlock;
+    if (i >= np) break;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
       const size_t idx = (size_t)a * np + i;
@@ -208,21 +219,10 @@ __global__ __launch_bounds__(kBlock) void k_kick(float *__restrict__ vel, float 
       const double force = -1.5 * omega * (double)dv - use_cola * lp / A;
       const float v = (float)((double)vel[idx] + force * dda);
       vel[idx] = v;
-      s[a] = (double)v;
+      s[a] += (double)v;
     }
   }
-#pragma unroll
-  for (int a = 0; a < 3; a++) red[a][threadIdx.x] = s[a];
-  __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
-#pragma unroll
-      for (int a = 0; a < 3; a++) red[a][threadIdx.x] += red[a][threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0)
-#pragma unroll
-    for (int a = 0; a < 3; a++) partial[(size_t)blockIdx.x * 3 + a] = red[a][0];
+  block_sum3(s, partial);
 }
 
 // periodic_wrap on float (src/auxPM.c:649-655) with a bounded loop
@@ -329,8 +329,8 @@ int pm_pofk(mgpu_ctx *ctx, const float2 *dk, mgpu_pofk *out) {
   unsigned grid = std::min<size_t>(cdiv(modes, kBlock), 4096);
   {
     KScope ks(ctx, MGPU_K_POFK, 8.0 * modes);
-    hipLaunchKernelGGL(k_pofk, dim3(grid), dim3(kBlock), shmem, ctx->stream, dk, N, ctx->HP, dwin, nbins,
-                       ctx->p.pofk_bintype, ctx->p.pofk_kmin, ctx->p.pofk_kmax, ctx->pk);
+    hipLaunchKernelGGL(k_pofk, dim3(grid), dim3(kBlock), shmem, ctx->stream, dk, N, ctx->HP, dwin, ctx->pk_bin,
+                       nbins, ctx->p.pofk_bintype, ctx->p.pofk_kmin, ctx->p.pofk_kmax, ctx->pk);
   }
   MGPU_HIP(ctx, hipGetLastError());
   return pofk_finish(ctx, out, false);
@@ -342,7 +342,7 @@ int pm_transform(mgpu_ctx *ctx, mgpu_pofk *out) {
   float2 *dk = ctx->slot[0];
   const float *dx = (const float *)ctx->slot[ctx->dx_slot];
   int rc;
-  if (!out || ctx->pk_geom.size() != 2 * (size_t)ctx->p.pofk_nbins) {
+  if (!out || !ctx->pk_bin || ctx->pk_geom.size() != 2 * (size_t)ctx->p.pofk_nbins) {
     {
       PhaseTimer pt(ctx, &ctx->t.ptomesh);
       if ((rc = fft_r2c_3d(ctx, &ctx->fft, dk, dx))) return rc;
@@ -356,12 +356,23 @@ int pm_transform(mgpu_ctx *ctx, mgpu_pofk *out) {
     PhaseTimer pf(ctx, &ctx->t.fft);
     if ((rc = fft_pass_z(ctx, &ctx->fft, dk, 1, nullptr, dx))) return rc;
     if ((rc = fft_pass_col(ctx, &ctx->fft, dk, 1, 1))) return rc;
-    if ((rc = fused_pofk_x(ctx, dk, dwin, ctx->p.pofk_nbins, ctx->p.pofk_bintype, ctx->p.pofk_kmin, ctx->p.pofk_kmax,
-                           ctx->pk)))
-      return rc;
+    if ((rc = fused_pofk_x(ctx, dk, ctx->pk))) return rc;
   }
   PhaseTimer pt(ctx, &ctx->t.pofk);
   return pofk_finish(ctx, out, true);
+}
+
+static double force_norm(const mgpu_ctx *ctx) {
+  return 1.0 / (std::pow((double)ctx->N, 3) * (2. * M_PI / ctx->p.box));
+}
+
+// The f(R)/m-beta mass term and DivideByLaplacian normalisation (src/mg.h:21-116)
+static double fofr_massterm2(const mgpu_ctx *ctx, const mgpu_mg *mg) {
+  return mg->a * mg->a * mg->mass2 / std::pow(2.0 * MGPU_PI * MGPU_INVERSE_H0_MPCH / ctx->p.box, 2);
+}
+static double laplace_norm(const mgpu_ctx *ctx, const mgpu_mg *mg) {
+  double nf = 1.0 / std::pow((double)ctx->N, 3);
+  return nf * 1.5 * ctx->p.omega / mg->a * std::pow(ctx->p.box / MGPU_INVERSE_H0_MPCH / (2.0 * MGPU_PI), 2);
 }
 
 // ComputeFifthForce (src/user_defined_functions.h:687-719).  With screening the
@@ -391,12 +402,11 @@ int pm_fifth_force(mgpu_ctx *ctx, const mgpu_mg *mg) {
     }
     case MGPU_FOFR:
     case MGPU_MBETA: {
-      const double massterm2 = mg->a * mg->a * mg->mass2 / std::pow(2.0 * MGPU_PI * MGPU_INVERSE_H0_MPCH / ctx->p.box, 2);
+      const double massterm2 = fofr_massterm2(ctx, mg);
       if (ctx->p.include_screening) {
         // _PotentialScreening (src/mg.h:147-189) with DivideByLaplacian (src/mg.h:21-64)
-        double nf = 1.0 / std::pow((double)N, 3);
-        nf *= 1.5 * ctx->p.omega / mg->a * std::pow(ctx->p.box / MGPU_INVERSE_H0_MPCH / (2.0 * MGPU_PI), 2);
-        if ((rc = fused_green_x(ctx, FUSED_GREEN_LAPLACE, dk, nullptr, phi, nullptr, nf, 0.0, 0))) return rc;
+        if ((rc = fused_green_x(ctx, FUSED_GREEN_LAPLACE, dk, nullptr, phi, nullptr, laplace_norm(ctx, mg), 0.0, 0)))
+          return rc;
         if ((rc = fft_pass_col(ctx, &ctx->fft, phi, 1, 0))) return rc;
         if ((rc = fused_z_screen(ctx, 0, phi, dx, mg->phi_crit, 0.0))) return rc;
         if ((rc = fft_pass_col(ctx, &ctx->fft, phi, 1, 1))) return rc;
@@ -436,18 +446,11 @@ int pm_fifth_force(mgpu_ctx *ctx, const mgpu_mg *mg) {
   }
 }
 
-// Forces (src/auxPM.c:437-555): F_a = c2r(i n_a (-1/n^2) P / (N^3 2pi/Box)), P = delta(k) (+ phi(k)).
-// G = -P/(n^2 N^3 2pi/Box) is formed once in the x pass, which also writes the x
-// transform of i n_x G; one y pass yields the y transforms of G and i n_y G, and
-// the z component's i n_z is applied in its c2r.  Meshes: F_x slot 3, F_y slot 5, F_z slot 4.
-int pm_forces(mgpu_ctx *ctx) {
-  PhaseTimer pt(ctx, &ctx->t.forces);
-  const int N = ctx->N;
-  const double cnorm = 1.0 / (std::pow((double)N, 3) * (2. * M_PI / ctx->p.box));
+// The force tail shared by pm_forces and the fused step: slot 3 holds the x
+// transform of i n_x G and slot 4 that of G; one dual y pass, the y pass of
+// slot 3 and the three z c2r give F_x (slot 3), F_y (slot 5), F_z (slot 4).
+static int forces_tail(mgpu_ctx *ctx) {
   int rc;
-  if ((rc = fused_green_x(ctx, FUSED_GREEN_FORCE, ctx->slot[0], ctx->have_phik ? ctx->slot[ctx->phi_slot] : nullptr,
-                          ctx->slot[3], ctx->slot[4], cnorm, 0.0, 0)))
-    return rc;
   if ((rc = fused_dual_y(ctx, ctx->slot[4], ctx->slot[5]))) return rc;
   if ((rc = fft_pass_col(ctx, &ctx->fft, ctx->slot[3], 1, 0))) return rc;
   if ((rc = fft_pass_z(ctx, &ctx->fft, ctx->slot[3], 0, nullptr))) return rc;
@@ -459,6 +462,89 @@ int pm_forces(mgpu_ctx *ctx) {
   return MGPU_OK;
 }
 
+bool pm_step_fusable(const mgpu_ctx *ctx, const mgpu_mg *mg, const mgpu_pofk *out) {
+  if (out && (!ctx->pk_bin || ctx->pk_geom.size() != 2 * (size_t)ctx->p.pofk_nbins)) return false;
+  if (!ctx->p.mg_active || ctx->p.model == MGPU_LCDM) return true;
+  if (!mg || !ctx->p.include_screening) return false;
+  return ctx->p.model == MGPU_FOFR || ctx->p.model == MGPU_MBETA || ctx->p.model == MGPU_DGP;
+}
+
+// GetDisplacements (src/auxPM.c:37-103) with the x passes in pairs: the
+// density's forward x pass (with P(k)) carries the inverse x pass of the
+// Green's force (LCDM) or of the Laplacian / smoothing (screened MG), and the
+// screened effective density's forward x pass carries the MG epilogue and the
+// inverse x pass of the force.  Same arithmetic as pm_transform, pm_fifth_force
+// and pm_forces; phi(k) is not stored.
+int pm_step_fused(mgpu_ctx *ctx, const mgpu_mg *mg, mgpu_pofk *out, double sumD[3]) {
+  int rc;
+  if ((rc = pm_deposit(ctx))) return rc;
+  float2 *dk = ctx->slot[0];
+  double *pk = nullptr;
+  if (out) {
+    double *dwin;
+    if ((rc = pofk_setup(ctx, &dwin))) return rc;
+    pk = ctx->pk;
+  }
+  {
+    PhaseTimer pt(ctx, &ctx->t.ptomesh);
+    PhaseTimer pf(ctx, &ctx->t.fft);
+    if ((rc = fft_pass_z(ctx, &ctx->fft, dk, 1, nullptr, (const float *)ctx->slot[ctx->dx_slot]))) return rc;
+    if ((rc = fft_pass_col(ctx, &ctx->fft, dk, 1, 1))) return rc;
+  }
+  ctx->have_phik = false;
+  const double cnorm = force_norm(ctx);
+  if (!ctx->p.mg_active || ctx->p.model == MGPU_LCDM) {
+    PhaseTimer pt(ctx, &ctx->t.forces);
+    if ((rc = fused_x_pair(ctx, FUSED_FWD_DENSITY, FUSED_GREEN_FORCE, dk, dk, nullptr, ctx->slot[3], ctx->slot[4], cnorm,
+                           0.0, 0, 0.0, 0.0, pk)))
+      return rc;
+  } else {
+    PhaseTimer pt(ctx, &ctx->t.modified_gravity);
+    float2 *phi = ctx->slot[2];
+    const float *dx = (const float *)ctx->slot[1];
+    const bool dgp = ctx->p.model == MGPU_DGP;
+    if (dgp) {
+      if ((rc = fused_x_pair(ctx, FUSED_FWD_DENSITY, FUSED_GREEN_SMOOTH, dk, dk, nullptr, phi, nullptr,
+                             1.0 / std::pow((double)ctx->N, 3), 2.0 * MGPU_PI / ctx->p.box * mg->rsmooth,
+                             ctx->p.filter, 0.0, 0.0, pk)))
+        return rc;
+    } else if ((rc = fused_x_pair(ctx, FUSED_FWD_DENSITY, FUSED_GREEN_LAPLACE, dk, dk, nullptr, phi, nullptr,
+                                  laplace_norm(ctx, mg), 0.0, 0, 0.0, 0.0, pk))) {
+      return rc;
+    }
+    if ((rc = fft_pass_col(ctx, &ctx->fft, phi, 1, 0))) return rc;
+    if ((rc = fused_z_screen(ctx, dgp ? 1 : 0, phi, dx, dgp ? mg->coupling : mg->phi_crit, dgp ? mg->dgp_fac : 0.0)))
+      return rc;
+    if ((rc = fft_pass_col(ctx, &ctx->fft, phi, 1, 1))) return rc;
+    if ((rc = fused_x_pair(ctx, dgp ? FUSED_FWD_PLAIN : FUSED_FWD_EPI, FUSED_GREEN_FORCE, phi, nullptr, dk,
+                           ctx->slot[3], ctx->slot[4], cnorm, 0.0, 0, dgp ? 0.0 : mg->coupling,
+                           dgp ? 0.0 : fofr_massterm2(ctx, mg), nullptr)))
+      return rc;
+  }
+  {
+    PhaseTimer pt(ctx, &ctx->t.forces);
+    if ((rc = forces_tail(ctx))) return rc;
+  }
+  if (out) {
+    PhaseTimer pt(ctx, &ctx->t.pofk);
+    if ((rc = pofk_finish(ctx, out, true))) return rc;
+  }
+  return pm_gather(ctx, sumD);
+}
+
+// Forces (src/auxPM.c:437-555): F_a = c2r(i n_a (-1/n^2) P / (N^3 2pi/Box)), P = delta(k) (+ phi(k)).
+// G = -P/(n^2 N^3 2pi/Box) is formed once in the x pass, which also writes the x
+// transform of i n_x G; one y pass yields the y transforms of G and i n_y G, and
+// the z component's i n_z is applied in its c2r.  Meshes: F_x slot 3, F_y slot 5, F_z slot 4.
+int pm_forces(mgpu_ctx *ctx) {
+  PhaseTimer pt(ctx, &ctx->t.forces);
+  int rc;
+  if ((rc = fused_green_x(ctx, FUSED_GREEN_FORCE, ctx->slot[0], ctx->have_phik ? ctx->slot[ctx->phi_slot] : nullptr,
+                          ctx->slot[3], ctx->slot[4], force_norm(ctx), 0.0, 0)))
+    return rc;
+  return forces_tail(ctx);
+}
+
 int pm_gather(mgpu_ctx *ctx, double sumD[3]) {
   PhaseTimer pt(ctx, &ctx->t.mtoparticles);
   if (ctx->np == 0) {
@@ -466,7 +552,7 @@ int pm_gather(mgpu_ctx *ctx, double sumD[3]) {
     return MGPU_OK;
   }
   const int N = ctx->N;
-  const unsigned nb = cdiv(ctx->np, kBlock);
+  const unsigned nb = cdiv(ctx->np, kChunk);
   const float *f0 = (const float *)ctx->slot[ctx->force_slot[0]], *f1 = (const float *)ctx->slot[ctx->force_slot[1]],
               *f2 = (const float *)ctx->slot[ctx->force_slot[2]];
   // compulsory: positions and Disp (12 B each per particle) and the three force meshes once
@@ -478,12 +564,15 @@ int pm_gather(mgpu_ctx *ctx, double sumD[3]) {
       int rc = gather_tiled(ctx, f0, f1, f2);
       if (rc) return rc;
     }
-    KScope ks(ctx, MGPU_K_MEAN, 12.0 * ctx->np);
-    hipLaunchKernelGGL(k_sum3, dim3(nb), dim3(kBlock), 0, ctx->stream, ctx->disp, ctx->np, ctx->partial);
   } else {
     KScope ks(ctx, MGPU_K_GATHER, bytes);
-    hipLaunchKernelGGL(k_gather, dim3(nb), dim3(kBlock), 0, ctx->stream, ctx->pos, ctx->np, N, 2 * ctx->HP,
-                       (double)N / ctx->p.box, f0, f1, f2, ctx->disp, ctx->partial);
+    hipLaunchKernelGGL(k_gather, dim3(cdiv(ctx->np, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->pos, ctx->np, N,
+                       2 * ctx->HP, (double)N / ctx->p.box, f0, f1, f2, ctx->disp);
+  }
+  MGPU_HIP(ctx, hipGetLastError());
+  {
+    KScope ks(ctx, MGPU_K_MEAN, 12.0 * ctx->np);
+    hipLaunchKernelGGL(k_sum3, dim3(nb), dim3(kBlock), 0, ctx->stream, ctx->disp, ctx->np, ctx->partial);
   }
   MGPU_HIP(ctx, hipGetLastError());
   double s[3];
@@ -499,7 +588,7 @@ int pm_kick(mgpu_ctx *ctx, double dda, double A, double s1, double s2, const dou
     sumV[0] = sumV[1] = sumV[2] = 0;
     return MGPU_OK;
   }
-  const unsigned nb = cdiv(ctx->np, kBlock);
+  const unsigned nb = cdiv(ctx->np, kChunk);
   const int sd = ctx->p.scale_dependent ? 1 : 0;
   {
     // vel, Disp read + written, one (scale-dependent) or two LPT vectors read

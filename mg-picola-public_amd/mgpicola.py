@@ -452,7 +452,7 @@ def run(hp, verbose=False):
 
 KERNELS = ["deposit", "mean", "fft_r2c", "fft_c2r", "fft_col_fwd", "fft_col_inv", "pofk", "spectral", "real",
            "gather", "kick", "drift", "lpt", "readout", "fft_green", "fft_dual", "fft_screen", "fft_epi", "dep_sort",
-           "fold", "fft_pot", "fft_pk"]
+           "fold", "fft_pot", "fft_pk", "fft_xdens", "fft_xforce"]
 
 
 def _kt_setup():

@@ -216,6 +216,49 @@ def test_fifth_force_matches_oracle(pos, case):
         assert rel_err(g.get_mesh(mid), O.real_view(f[a], N)) < 3e-5
 
 
+@pytest.mark.parametrize("case", ["lcdm", "fofr_screen", "dgp_screen", "mbeta_screen"])
+@pytest.mark.parametrize("n", [32, 48])
+def test_fused_step_equals_separate_calls(pos, case, n):
+    """GetDisplacements with paired x passes (phi(k) never stored) against
+    PtoMesh, ComputeFifthForce, Forces and MtoParticles called one by one:
+    force meshes, Disp, the mean and P(k).  The first density call of each
+    context takes the unpaired path (it records the P(k) mode counts), so the
+    paired path runs from the second call on."""
+    ns = n // 2
+    p = lattice_particles(ns, BOX, amp=1.5, seed=n)
+    if case == "lcdm":
+        kw, mg = dict(), None
+    else:
+        cs = MG_CASES[case]
+        kw = dict(model=cs["model"], mg_active=1, include_screening=cs["scr"], filter=cs.get("filt", O.FILTER_NONE))
+        mgd = dict(a=1.0, coupling=0.0, mass2=0.0, phi_crit=0.0, dgp_fac=0.0, rsmooth=1.0, geff=1.0)
+        mgd.update(cs["mg"])
+        mg = M.MG(**mgd)
+    print("case, n =", case, n, flush=True)
+    g = M.MGPU(n, ns, BOX, OM, pofk_nbins=n // 2, **kw)
+    g.set_particles(p)
+    g.get_displacements(mg=mg, pofk=True)  # unpaired: records the mode counts
+    s_fused, (pk_f, k_f, n_f) = g.get_displacements(mg=mg, pofk=True)
+    f_fused = [g.get_mesh(m) for m in (M.MESH_FORCE_X, M.MESH_FORCE_Y, M.MESH_FORCE_Z)]
+    d_fused = g.get_disp()
+    dk_fused = g.get_mesh(M.MESH_DENSITY_K)
+    pk_s, k_s, n_s = g.pm_density(pofk=True)
+    dk_sep = g.get_mesh(M.MESH_DENSITY_K)
+    if mg is not None:
+        g.fifth_force(mg)
+    g.forces()
+    s_sep = g.mesh_to_particles()
+    f_sep = [g.get_mesh(m) for m in (M.MESH_FORCE_X, M.MESH_FORCE_Y, M.MESH_FORCE_Z)]
+    d_sep = g.get_disp()
+    assert rel_err(dk_fused, dk_sep) < 1e-6
+    for a in range(3):
+        assert rel_err(f_fused[a], f_sep[a]) < 1e-6, a
+    assert rel_err(d_fused, d_sep) < 1e-6
+    assert np.abs(s_fused - s_sep).max() <= 1e-6 * np.abs(d_sep).max()
+    assert np.array_equal(n_f, n_s) and np.array_equal(k_f, k_s)
+    np.testing.assert_allclose(pk_f, pk_s, rtol=1e-5, atol=1e-9 * np.abs(pk_s).max())
+
+
 def test_fofr_screening_is_active(pos):
     """The f(R) case above must actually screen some cells (else it pins nothing)."""
     c = O.cfg(N, NS, BOX, OM, model=O.FOFR, mg_active=1, include_screening=1)
