@@ -38,7 +38,9 @@ __global__ __launch_bounds__(col_threads(L, T)) void k_fft_col(float2 *__restric
   constexpr int E = col_ept(L, T);
   static_assert(NT <= 1024, "column tile too large");
   const int tid = threadIdx.x;
-  auto tile = [&](int i) { return buf + (size_t)(blockIdx.y * TPB + i) * outer_stride + (size_t)blockIdx.x * T; };
+  int bx, by;
+  xcd_tile(bx, by);
+  auto tile = [&](int i) { return buf + (size_t)(by * TPB + i) * outer_stride + (size_t)bx * T; };
   if constexpr (TPB == 1) {
     float2 *base = tile(0);
     auto ld = [&](int c, int j, int o) { return (base + (size_t)j * line_stride + c)[(size_t)o * line_stride]; };

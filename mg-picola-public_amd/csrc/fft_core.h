@@ -240,6 +240,16 @@ __device__ __forceinline__ void first_loads(float2 *pre, int tid, const LD &ld) 
   }
 }
 
+// Block -> tile order for 2D grids: workgroups are dealt to the 8 XCDs round
+// robin, so give each XCD a contiguous range of tiles (x fastest); tiles that
+// share cache lines or rows then meet in one L2.
+__device__ __forceinline__ void xcd_tile(int &bx, int &by) {
+  const unsigned nbx = gridDim.x, nblk = gridDim.x * gridDim.y, id = blockIdx.x + blockIdx.y * nbx;
+  const unsigned lin = nblk % 8 == 0 ? (id % 8) * (nblk / 8) + id / 8 : id;
+  bx = (int)(lin % nbx);
+  by = (int)(lin / nbx);
+}
+
 // ---------------------------------------------------------------- strided column pass
 constexpr int col_ept(int L, int T) { return elems_per_thread(L); }
 constexpr int col_threads(int L, int T) { return L * T / col_ept(L, T); }

@@ -67,7 +67,7 @@ __device__ __forceinline__ void green_mul(float2 S, int nx, int n2, bool valid, 
   }
 }
 
-// x pass, inverse.  Tile: columns kz = blockIdx.x*T + c, outer index ky = blockIdx.y,
+// x pass, inverse.  Tile (bx, by) of xcd_tile: columns kz = bx*T + c, outer index ky = by,
 // line index kx.  out1 is written by GREEN_FORCE only.
 template <int L, int T, int MODE>
 __global__ __launch_bounds__(col_threads(L, T)) void k_col_green(const float2 *__restrict__ a,
@@ -79,7 +79,9 @@ __global__ __launch_bounds__(col_threads(L, T)) void k_col_green(const float2 *_
   constexpr int R0 = pick_radix(L, E), NB0 = E / R0, M0 = L / R0;
   constexpr int N = L;
   const int tid = threadIdx.x;
-  const int ky = blockIdx.y, kz0 = blockIdx.x * T;
+  int bx, ky;
+  xcd_tile(bx, ky);
+  const int kz0 = bx * T;
   const size_t tile = (size_t)ky * os + kz0;
   float2 p1[E], p2[E];
 #pragma unroll
@@ -117,7 +119,7 @@ __global__ __launch_bounds__(col_threads(L, T)) void k_col_green(const float2 *_
 }
 
 // y pass, inverse, two outputs from one read: buf <- Y[buf], oy <- Y[i n_y buf].
-// Tile: columns kz, outer index kx = blockIdx.y, line index ky.
+// Tile (bx, by) of xcd_tile: columns kz, outer index kx = by, line index ky.
 template <int L, int T>
 __global__ __launch_bounds__(col_threads(L, T)) void k_col_dual(float2 *buf, float2 *__restrict__ oy, size_t ls,
                                                                   size_t os, const float2 *__restrict__ tw) {
@@ -125,7 +127,9 @@ __global__ __launch_bounds__(col_threads(L, T)) void k_col_dual(float2 *buf, flo
   constexpr int NT = col_threads(L, T), E = col_ept(L, T);
   constexpr int R0 = pick_radix(L, E), NB0 = E / R0, M0 = L / R0;
   const int tid = threadIdx.x;
-  const size_t tile = (size_t)blockIdx.y * os + (size_t)blockIdx.x * T;
+  int bx, by;
+  xcd_tile(bx, by);
+  const size_t tile = (size_t)by * os + (size_t)bx * T;
   float2 p1[E], p2[E];
 #pragma unroll
   for (int q = 0; q < NB0; ++q) {
@@ -156,7 +160,9 @@ __global__ __launch_bounds__(col_threads(L, T)) void k_col_epi(float2 *buf, size
   constexpr int NT = col_threads(L, T);
   constexpr int N = L;
   const int tid = threadIdx.x;
-  const int ky = blockIdx.y, kz0 = blockIdx.x * T;
+  int bx, ky;
+  xcd_tile(bx, ky);
+  const int kz0 = bx * T;
   const size_t tile = (size_t)ky * os + kz0;
   const int ny = nsgn(ky, N);
   auto ld = [&](int c, int j, int o) { return buf[tile + (size_t)j * ls + c + (size_t)o * ls]; };
@@ -189,7 +195,9 @@ __global__ __launch_bounds__(col_threads(L, T)) void k_col_pofk(float2 *buf, siz
   for (int b = tid; b < nbins; b += NT) bins[b] = 0.0;
   for (int d = tid; d <= N / 2; d += NT) wv[d] = winv[d];
   __syncthreads();
-  const int ky = blockIdx.y, kz0 = blockIdx.x * T;
+  int bx, ky;
+  xcd_tile(bx, ky);
+  const int kz0 = bx * T;
   const size_t tile = (size_t)ky * os + kz0;
   const int d1 = ky > N / 2 ? N - ky : ky;
   auto ld = [&](int c, int j, int o) { return buf[tile + (size_t)j * ls + c + (size_t)o * ls]; };
@@ -235,11 +243,9 @@ __global__ __launch_bounds__(col_threads(L, T)) void k_col_fused(
   constexpr int R0 = pick_radix(L, E), NB0 = E / R0, M0 = L / R0;
   constexpr int N = L;
   const int tid = threadIdx.x;
-  // blocks are dealt to the 8 XCDs round robin: give each XCD a contiguous
-  // range of tiles, so tiles that share 128-byte lines (T = 8) meet in one L2
-  const unsigned nbx = gridDim.x, nblk = gridDim.x * gridDim.y, id = blockIdx.x + blockIdx.y * nbx;
-  const unsigned lin = nblk % 8 == 0 ? (id % 8) * (nblk / 8) + id / 8 : id;
-  const int ky = (int)(lin / nbx), kz0 = (int)(lin % nbx) * T;
+  int bx, ky;
+  xcd_tile(bx, ky);
+  const int kz0 = bx * T;
   const size_t tile = (size_t)ky * os + kz0;
   const int ny = nsgn(ky, N), d1 = ky > N / 2 ? N - ky : ky;
   const bool bin = FWD == FWD_DENSITY && pk != nullptr;
@@ -438,9 +444,12 @@ template <int N, int T>
 int fused_x_launch(mgpu_ctx *ctx, const FusedXArgs &a) {
   if (a.fwd == FWD_DENSITY && a.mode == GREEN_LAPLACE) return fused_x_launch_k<N, T, FWD_DENSITY, GREEN_LAPLACE>(ctx, a);
   if (a.fwd == FWD_DENSITY && a.mode == GREEN_SMOOTH) return fused_x_launch_k<N, T, FWD_DENSITY, GREEN_SMOOTH>(ctx, a);
-  if (a.fwd == FWD_DENSITY && a.mode == GREEN_FORCE) return fused_x_launch_k<N, T, FWD_DENSITY, GREEN_FORCE>(ctx, a);
-  if (a.fwd == FWD_EPI && a.mode == GREEN_FORCE) return fused_x_launch_k<N, T, FWD_EPI, GREEN_FORCE>(ctx, a);
-  if (a.fwd == FWD_PLAIN && a.mode == GREEN_FORCE) return fused_x_launch_k<N, T, FWD_PLAIN, GREEN_FORCE>(ctx, a);
+  // the force pair keeps G and i n_x G in registers: only with 8-column tiles
+  if constexpr (T == 8) {
+    if (a.fwd == FWD_DENSITY && a.mode == GREEN_FORCE) return fused_x_launch_k<N, T, FWD_DENSITY, GREEN_FORCE>(ctx, a);
+    if (a.fwd == FWD_EPI && a.mode == GREEN_FORCE) return fused_x_launch_k<N, T, FWD_EPI, GREEN_FORCE>(ctx, a);
+    if (a.fwd == FWD_PLAIN && a.mode == GREEN_FORCE) return fused_x_launch_k<N, T, FWD_PLAIN, GREEN_FORCE>(ctx, a);
+  }
   return mgpu_fail(ctx, MGPU_E_INVALID, "unsupported fused x pass");
 }
 
@@ -533,6 +542,7 @@ template <int N, int T>
 struct EpiF {
   static int run(mgpu_ctx *c, float2 *buf, float cp, float m2) { return epi_launch<N, T>(c, buf, cp, m2); }
 };
+
 
 }  // namespace
 
