@@ -41,6 +41,7 @@ KERNEL_SYMBOL = {
     "fft_epi": "k_col_epi<{N}, 16>", "fft_pk": "k_col_pofk<{N}, 16>", "fold": "k_lpt_fold_gs<{m}, 2>",
     "gather": "k_gather_tile<16>", "kick": "k_kick", "drift": "k_drift", "pofk": "k_pofk",
     "lpt": "k_fft_col<{Ns}, ...> + k_fft_z_c2r<{Ns}, false>",
+    "fft_xdens": "k_col_fused<{N}, 16, 0, 1>", "fft_xforce": "k_col_fused<{N}, 8, 1, 0>",
 }
 
 
@@ -102,6 +103,8 @@ def gpu_bench(steps, warmup, workload):
     while not sim.finished:
         sim.advance()
         nrun += 1
+        if nrun % 10 == 0:
+            print(f"[bench] kernel-timed run: {nrun} steps", file=sys.stderr, flush=True)
     sim.gpu.sync()
     kt = M.kernel_timing(sim.gpu)
     M.enable_kernel_timing(sim.gpu, False)
@@ -205,8 +208,8 @@ def main():
     symbol = KERNEL_SYMBOL.get(dom, dom).format(N=r["nmesh"], Ns=r["nsample"], m=r["nmesh"] // r["nsample"])
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_of(args.workload, symbol),
-            "kernel": symbol, "class": dom, "launches_per_step": round(n, 2), "ms_per_launch": round(ms / max(n, 1), 4),
-            "algorithmic_bytes_per_launch": round(by / max(n, 1))}
+            "kernel": symbol, "class": dom, "launches_per_step": round(n, 2), "ms_per_launch": round(ms / n, 4),
+            "algorithmic_bytes_per_launch": round(by / n)}
     per_kernel = {k: {"launches": round(v[0], 2), "ms": round(v[1], 3),
                       "GBps": round(v[2] / (v[1] * 1e-3) / 1e9, 1) if v[1] > 0 else None}
                   for k, v in kt.items() if v[0]}
