@@ -131,6 +131,19 @@ int mgpu_fifth_force(mgpu_ctx *ctx, const mgpu_mg *mg);
 int mgpu_forces(mgpu_ctx *ctx);
 /* MtoParticles: CIC gather into Disp, mean over particles.  src/auxPM.c:560-644 */
 int mgpu_mesh_to_particles(mgpu_ctx *ctx, double sumD[3]);
+/* RSD multipoles of one line of sight (axis 1 = y, 2 = z): PtoMesh_RSD, the r2c and
+ * bin_up_RSD_power_spectrum of compute_RSD_powerspectrum (src/compute_pofk.c:279-395,
+ * :403-455, :496-753).  Positions move along the line of sight by vnorm * V, vnorm =
+ * (Hubble/A)/(100 A hubble(A)) * Nmesh/Box; V = Vel plus, with COLA, the LPT velocity:
+ * scale-dependent, LPT slot 1 must hold dD/dy (both orders, FIELD_dDdy); otherwise
+ * slot0 * s1 + slot1 * s2 with s1 = dDdy(A), s2 = dD2dy(A).  The binning is the P(k)'s
+ * (pofk_* parameters of the context).  P0, P2, P4 (shot noise subtracted when
+ * pofk_subtract_shotnoise), k = the bin's nominal k, n = modes; each array nbins long,
+ * any may be NULL.  Overwrites the force meshes. */
+typedef struct {
+  double *P0, *P2, *P4, *k, *n;
+} mgpu_rsd_pofk;
+int mgpu_rsd_power_spectrum(mgpu_ctx *ctx, int axis, double vnorm, double s1, double s2, mgpu_rsd_pofk *out);
 /* GetDisplacements = PtoMesh + ComputeFifthForce + Forces + MtoParticles.
  * mg may be NULL when modified gravity is off.  src/auxPM.c:37-103
  * For LCDM and the screened f(R), m-beta and DGP solvers the x passes run in

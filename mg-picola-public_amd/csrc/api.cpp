@@ -201,7 +201,7 @@ int mgpu_create(const mgpu_params *p, mgpu_ctx **out) {
   if ((rc = alloc(ctx, (void **)&ctx->partial, ctx->partial_len * sizeof(double), "reduction scratch"))) return fail(rc);
   if ((rc = alloc(ctx, (void **)&ctx->dsum, 8 * sizeof(double), "sums"))) return fail(rc);
   const int nb = std::max(1, p->pofk_nbins);
-  if ((rc = alloc(ctx, (void **)&ctx->pk, 3 * (size_t)nb * sizeof(double), "P(k) bins"))) return fail(rc);
+  if ((rc = alloc(ctx, (void **)&ctx->pk, 4 * (size_t)nb * sizeof(double), "P(k) bins"))) return fail(rc);
   {
     // compute_power_spectrum's CIC deconvolution 1/(W_x W_y W_z)^4 with the 1/N^6 of the
     // transform split over the three axes (src/compute_pofk.c:128-134)
@@ -270,7 +270,7 @@ void mgpu_destroy(mgpu_ctx *ctx) {
                   ctx->slot[0],  ctx->slot[1],   ctx->slot[2],   ctx->slot[3],   ctx->cdelta[0],
                   ctx->cdelta[1], ctx->partial,  ctx->dsum,      ctx->pk,        ctx->n2tab[0],
                   ctx->n2tab[1], ctx->n2tabf[0], ctx->n2tabf[1], ctx->lattice_slot, ctx->slice_buf,
-                  ctx->dep_scratch, ctx->winv,    ctx->winvf,     ctx->pk_bin};
+                  ctx->dep_scratch, ctx->winv,    ctx->winvf,     ctx->pk_bin,    ctx->rsd_crd};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (ctx->slice_host) (void)hipHostFree(ctx->slice_host);
@@ -408,6 +408,11 @@ int mgpu_get_displacements(mgpu_ctx *ctx, const mgpu_mg *mg, mgpu_pofk *pk, doub
   if (ctx->p.mg_active && (rc = mgpu_fifth_force(ctx, mg))) return rc;
   if ((rc = mgpu_forces(ctx))) return rc;
   return mgpu_mesh_to_particles(ctx, sumD);
+}
+
+int mgpu_rsd_power_spectrum(mgpu_ctx *ctx, int axis, double vnorm, double s1, double s2, mgpu_rsd_pofk *out) {
+  if (check_ctx(ctx) || !out) return MGPU_E_INVALID;
+  return pm_rsd_pofk(ctx, axis, vnorm, s1, s2, out);
 }
 
 int mgpu_kick(mgpu_ctx *ctx, double dda, double A, double s1, double s2, const double sumD[3], double sumV[3]) {

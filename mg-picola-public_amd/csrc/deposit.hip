@@ -48,18 +48,41 @@ __device__ __forceinline__ float to_delta(unsigned long long v) { return (float)
 // equal keys are grouped first (ballots only); then the group leaders add
 // their group sizes in one atomic instruction, so a wave waits for one memory
 // round trip however many bins it touches.
-__global__ __launch_bounds__(kTB) void k_dep_bin(const float *__restrict__ pos, size_t np, TileGeom g, double scale,
-                                                 unsigned *__restrict__ count, unsigned *__restrict__ key,
-                                                 unsigned *__restrict__ rank) {
+// Mesh coordinates of a particle.  Plain: from the SoA positions.  Redshift
+// space (RSD): from a record (x, y in box units, Z in mesh units as FP64 in
+// the last two slots), written by k_rsd_coords.
+__device__ __forceinline__ double rec_z(const float4 &r) {
+  return __hiloint2double(__float_as_int(r.w), __float_as_int(r.z));
+}
+template <bool RSD>
+__device__ __forceinline__ void particle_xyz(const float *__restrict__ pos, const float4 *__restrict__ crd, size_t np,
+                                             size_t i, double scale, double &X, double &Y, double &Z) {
+  if (RSD) {
+    const float4 r = crd[i];
+    X = (double)r.x * scale;
+    Y = (double)r.y * scale;
+    Z = rec_z(r);
+  } else {
+    X = (double)pos[i] * scale;
+    Y = (double)pos[np + i] * scale;
+    Z = (double)pos[2 * np + i] * scale;
+  }
+}
+
+template <bool RSD>
+__global__ __launch_bounds__(kTB) void k_dep_bin(const float *__restrict__ pos, const float4 *__restrict__ crd,
+                                                 size_t np, TileGeom g, double scale, unsigned *__restrict__ count,
+                                                 unsigned *__restrict__ key, unsigned *__restrict__ rank) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < np;
   unsigned t = 0;
   if (valid) {
     unsigned IX, IY, IZ;
-    double D;
-    cell_of((double)pos[i] * scale, g.N, IX, D);
-    cell_of((double)pos[np + i] * scale, g.N, IY, D);
-    cell_of((double)pos[2 * np + i] * scale, g.N, IZ, D);
+    double D, X, Y, Z;
+    particle_xyz<RSD>(pos, crd, np, i, scale, X, Y, Z);
+    cell_of(X, g.N, IX, D);
+    cell_of(Y, g.N, IY, D);
+    cell_of(Z, g.N, IZ, D);
     const unsigned m = (unsigned)g.tx - 1;
     const unsigned cls = ((IX & m) == m ? 4u : 0u) | ((IY & m) == m ? 2u : 0u) | ((IZ & m) == m ? 1u : 0u);
     t = 8u * tile_of(g, IX, IY, IZ) + cls;
@@ -135,14 +158,17 @@ __global__ __launch_bounds__(1024) void k_scan_add(unsigned *__restrict__ out, s
   if (i == 0) out[ntail] = total_tail;
 }
 
-// 3. particle records in bin order: (x, y, z, index as bits)
+// 3. particle records in bin order: (x, y, z, index as bits), or the RSD record
+template <bool RSD>
 __global__ __launch_bounds__(kTB) void k_dep_scatter(const unsigned *__restrict__ key,
                                                      const unsigned *__restrict__ rank,
                                                      const unsigned *__restrict__ off, const float *__restrict__ pos,
-                                                     size_t np, float4 *__restrict__ rec) {
+                                                     const float4 *__restrict__ crd, size_t np,
+                                                     float4 *__restrict__ rec) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
-  rec[off[key[i]] + rank[i]] = make_float4(pos[i], pos[np + i], pos[2 * np + i], __uint_as_float((unsigned)i));
+  rec[off[key[i]] + rank[i]] =
+      RSD ? crd[i] : make_float4(pos[i], pos[np + i], pos[2 * np + i], __uint_as_float((unsigned)i));
 }
 
 // 4. one workgroup per tile.  The particle lists it reads are up to 64 bin
@@ -151,6 +177,7 @@ __global__ __launch_bounds__(kTB) void k_dep_scatter(const unsigned *__restrict_
 // read through shift s has local coordinate (I mod T) - s T, so its corners at
 // local coordinate -1 (s = 1) or T (s = 0) fall outside the tile and are
 // skipped; every corner is added by exactly one tile.
+template <bool RSD>
 __global__ __launch_bounds__(kTB) void k_dep_tile(const float4 *__restrict__ rec, const unsigned *__restrict__ off,
                                                   TileGeom g, double scale, double wpar, int HP2,
                                                   float *__restrict__ d) {
@@ -209,7 +236,7 @@ __global__ __launch_bounds__(kTB) void k_dep_tile(const float4 *__restrict__ rec
       double DX, DY, DZ;
       cell_of((double)pr[u].x * scale, g.N, IX, DX);
       cell_of((double)pr[u].y * scale, g.N, IY, DY);
-      cell_of((double)pr[u].z * scale, g.N, IZ, DZ);
+      cell_of(RSD ? rec_z(pr[u]) : (double)pr[u].z * scale, g.N, IZ, DZ);
       const double TX = 1.0 - DX, TZ = 1.0 - DZ;
       const double TY = (1.0 - DY) * wpar;
       DY *= wpar;
@@ -346,7 +373,7 @@ size_t deposit_scratch_bytes(int N, size_t np) {
   return sizeof(unsigned) * (2 * (nb + 1) + nsum + 2 * np) + sizeof(float4) * np + 256 * 8;
 }
 
-int deposit_tiled(mgpu_ctx *ctx, float *d) {
+int deposit_tiled(mgpu_ctx *ctx, float *d, const float4 *rsd) {
   const int N = ctx->N;
   const size_t np = ctx->np;
   TileGeom g = make_geom(N);
@@ -369,17 +396,18 @@ int deposit_tiled(mgpu_ctx *ctx, float *d) {
   float4 *rec = (float4 *)take(sizeof(float4) * np);
   if ((size_t)(b - (char *)ctx->dep_scratch) > ctx->dep_scratch_bytes)
     return mgpu_fail(ctx, MGPU_E_INVALID, "deposit scratch too small");
-  // the records stay valid for the gather until the positions change
+  // the records stay valid for the gather until the positions change (never for
+  // redshift-space positions)
   ctx->dep_rec = rec;
   ctx->dep_off = off;
-  ctx->sorted_valid = np > 0;
+  ctx->sorted_valid = np > 0 && !rsd;
   const double scale = (double)N / ctx->p.box;
   const double wpar = std::pow((double)N / (double)ctx->Np, 3);
   MGPU_HIP(ctx, hipMemsetAsync(count, 0, sizeof(unsigned) * (nb + 1), ctx->stream));
   if (np) {
     KScope ks(ctx, MGPU_K_DEP_SORT, 20.0 * np);
-    hipLaunchKernelGGL(k_dep_bin, dim3(cdiv(np, kTB)), dim3(kTB), 0, ctx->stream, ctx->pos, np, g, scale, count, key,
-                       rank);
+    hipLaunchKernelGGL(rsd ? k_dep_bin<true> : k_dep_bin<false>, dim3(cdiv(np, kTB)), dim3(kTB), 0, ctx->stream,
+                       ctx->pos, rsd, np, g, scale, count, key, rank);
   }
   MGPU_HIP(ctx, hipGetLastError());
   hipLaunchKernelGGL(k_scan_block, dim3(nsum), dim3(1024), 0, ctx->stream, count, nb, off, sums);
@@ -388,8 +416,8 @@ int deposit_tiled(mgpu_ctx *ctx, float *d) {
   MGPU_HIP(ctx, hipGetLastError());
   if (np) {
     KScope ks(ctx, MGPU_K_DEP_SORT, 12.0 * np);
-    hipLaunchKernelGGL(k_dep_scatter, dim3(cdiv(np, kTB)), dim3(kTB), 0, ctx->stream, key, rank, off, ctx->pos, np,
-                       rec);
+    hipLaunchKernelGGL(rsd ? k_dep_scatter<true> : k_dep_scatter<false>, dim3(cdiv(np, kTB)), dim3(kTB), 0,
+                       ctx->stream, key, rank, off, ctx->pos, rsd, np, rec);
   }
   MGPU_HIP(ctx, hipGetLastError());
   const size_t shmem = sizeof(unsigned long long) * g.tx * g.ty * g.tz;
@@ -397,8 +425,8 @@ int deposit_tiled(mgpu_ctx *ctx, float *d) {
   {
     // compulsory: the particle records read once, delta written
     KScope ks(ctx, MGPU_K_DEPOSIT, 16.0 * np + 4.0 * cells);
-    hipLaunchKernelGGL(k_dep_tile, dim3(nt), dim3(kTB), shmem, ctx->stream, rec, off, g, scale, wpar, 2 * ctx->HP,
-                       d);
+    hipLaunchKernelGGL(rsd ? k_dep_tile<true> : k_dep_tile<false>, dim3(nt), dim3(kTB), shmem, ctx->stream, rec, off,
+                       g, scale, wpar, 2 * ctx->HP, d);
   }
   MGPU_HIP(ctx, hipGetLastError());
   return MGPU_OK;

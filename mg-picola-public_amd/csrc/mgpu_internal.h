@@ -62,6 +62,7 @@ struct mgpu_ctx {
   double *winv = nullptr;     // P(k) deconvolution per axis: 1/sinc(pi d/N)^4 N^-2, d = 0..N/2
   float *winvf = nullptr;     // FP32 copy of winv
   int16_t *pk_bin = nullptr;  // P(k) bin of each |n|^2 (-1: outside the bins); null when nbins > 32767
+  float4 *rsd_crd = nullptr;  // redshift-space particle records (allocated on the first RSD call)
   std::vector<double> pk_geom;  // mode-weighted |n| sums and mode counts per bin (2 * nbins), fixed per context
   double *n2tab[2] = {nullptr, nullptr};  // per-|n|^2 device tables
   float *n2tabf[2] = {nullptr, nullptr};  // FP32 copies of n2tab (lattice fold)
@@ -175,7 +176,8 @@ int fused_z_screen(mgpu_ctx *ctx, int density_kind, float2 *buf, const float *dx
 // ---------------------------------------------------------------- tiled deposit (deposit.hip)
 size_t deposit_scratch_bytes(int N, size_t np);
 // particles -> delta(x) in real layout in d (sums in 64-bit fixed point in LDS)
-int deposit_tiled(mgpu_ctx *ctx, float *d);
+// (with rsd != NULL: particles at the redshift-space records of k_rsd_coords instead)
+int deposit_tiled(mgpu_ctx *ctx, float *d, const float4 *rsd = nullptr);
 // Disp from the three force meshes by the last deposit's bins (sorted_valid must hold)
 int gather_tiled(mgpu_ctx *ctx, const float *f0, const float *f1, const float *f2);
 
@@ -186,6 +188,7 @@ int pm_transform(mgpu_ctx *ctx, mgpu_pofk *out);  // r2c of delta(x) into slot 0
 int pm_fifth_force(mgpu_ctx *ctx, const mgpu_mg *mg);
 int pm_forces(mgpu_ctx *ctx);
 // the whole of GetDisplacements with paired x passes (pm_step_fusable says when it applies)
+int pm_rsd_pofk(mgpu_ctx *ctx, int axis, double vnorm, double s1, double s2, mgpu_rsd_pofk *out);
 bool pm_step_fusable(const mgpu_ctx *ctx, const mgpu_mg *mg, const mgpu_pofk *out);
 int pm_step_fused(mgpu_ctx *ctx, const mgpu_mg *mg, mgpu_pofk *out, double sumD[3]);
 int pm_gather(mgpu_ctx *ctx, double sumD[3]);

@@ -50,6 +50,72 @@ __global__ __launch_bounds__(kBlock) void k_pofk(const float2 *__restrict__ dk, 
     if (sh[b] != 0.0) atomicAdd(&out[b], sh[b]);
 }
 
+// ------------------------------------------------------------------ RSD multipoles
+// PtoMesh_RSD's positions (src/compute_pofk.c:279-335): mesh coordinates
+// (Pos0, Pos1, Pos2) for the z line of sight and (Pos0, Pos2, Pos1) for y; the
+// line-of-sight one moved by vnorm V, V = Vel (+ the LPT velocity with COLA:
+// the dD/dy field summed over orders as a float sum when scale-dependent, else
+// D dDdy(A) + D2 dD2dy(A)), then one periodic wrap.  Record: x, y in box units,
+// Z in mesh units as FP64 (low, high words), the layout the deposit reads.
+__global__ __launch_bounds__(kBlock) void k_rsd_coords(const float *__restrict__ pos, const float *__restrict__ vel,
+                                                       const float *__restrict__ l0, const float *__restrict__ l1,
+                                                       size_t np, int axis, double vnorm, double scale, int sd,
+                                                       int use_cola, double s1, double s2, int N,
+                                                       float4 *__restrict__ crd) {
+#pragma clang fp contract(off)  // the rounding sequence of the CPU reference
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const int ya = axis == 1 ? 2 : 1;
+  const size_t ia = (size_t)axis * np + i;
+  const float va = vel[ia];
+  double V;
+  if (!use_cola) V = va;
+  else if (sd) V = (double)(va + l1[ia]);
+  else V = (double)va + ((double)l0[ia] * s1 + (double)l1[ia] * s2);
+  V *= vnorm;
+  double Z = (double)pos[ia] * scale;
+  Z += V;
+  if (Z >= (double)N) Z -= (double)N;
+  if (Z < 0.0) Z += (double)N;
+  crd[i] = make_float4(pos[i], pos[(size_t)ya * np + i], __int_as_float(__double2loint(Z)),
+                       __int_as_float(__double2hiint(Z)));
+}
+
+// bin_up_RSD_power_spectrum's sums (src/compute_pofk.c:496-704): per bin the
+// weighted |delta|^2 (deconvolved, 1/N^6), its mu^2 and mu^4 moments with
+// mu^2 = n_z^2/n^2, and the mode count; weights 1 (kz = 0, N/2) or 2.
+__global__ __launch_bounds__(kBlock) void k_rsd_bins(const float2 *__restrict__ dk, int N, int HP,
+                                                     const double *__restrict__ winv,
+                                                     const int16_t *__restrict__ pbin, int nbins,
+                                                     double *__restrict__ out) {
+  extern __shared__ double sh[];  // 4*nbins
+  for (int b = threadIdx.x; b < 4 * nbins; b += blockDim.x) sh[b] = 0.0;
+  __syncthreads();
+  const int H = N / 2 + 1;
+  const size_t total = (size_t)N * N * H;
+  for (size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x; m < total; m += (size_t)gridDim.x * blockDim.x) {
+    const int k = (int)(m % H);
+    const size_t rowi = m / H;
+    const int j = (int)(rowi % N), i = (int)(rowi / N);
+    const int d0 = i > N / 2 ? N - i : i, d1 = j > N / 2 ? N - j : j;
+    const int n2 = d0 * d0 + d1 * d1 + k * k;
+    const int nk = pbin[n2];
+    if (nk < 0) continue;
+    const double kmag = sqrt((double)n2);
+    const double mu2 = k == 0 ? 0.0 : (double)(k * k) / kmag / kmag;
+    const double w = (k == 0 || k == N / 2) ? 1.0 : 2.0;
+    const float2 v = dk[rowi * HP + k];
+    const double p = ((double)v.x * v.x + (double)v.y * v.y) * winv[d0] * winv[d1] * winv[k];
+    atomicAdd(&sh[nk], w * p);
+    atomicAdd(&sh[nbins + nk], w * p * mu2);
+    atomicAdd(&sh[2 * nbins + nk], w * p * mu2 * mu2);
+    atomicAdd(&sh[3 * nbins + nk], w);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < 4 * nbins; b += blockDim.x)
+    if (sh[b] != 0.0) atomicAdd(&out[b], sh[b]);
+}
+
 // ------------------------------------------------------------------ spectral helpers
 // |n|^2 of a half-spectrum entry (identical under the Forces, mg.h and 2LPT.c
 // index conventions)
@@ -618,5 +684,72 @@ int pm_drift(mgpu_ctx *ctx, double dyyy, double s1, double s2, const double sumV
                        sumV[0], sumV[1], sumV[2]);
   }
   MGPU_HIP(ctx, hipGetLastError());
+  return MGPU_OK;
+}
+
+// compute_RSD_powerspectrum for one line of sight (src/compute_pofk.c:403-455,
+// :496-753): redshift-space deposit, r2c and the multipoles, normalised as
+// the reference does (Legendre combination P4 before P2, shot noise off all
+// three, k of the bin from k_from_index).  Uses slot 3 and the deposit scratch.
+int pm_rsd_pofk(mgpu_ctx *ctx, int axis, double vnorm, double s1, double s2, mgpu_rsd_pofk *out) {
+  PhaseTimer pt(ctx, &ctx->t.pofk);
+  const int N = ctx->N, nbins = ctx->p.pofk_nbins;
+  if (axis != 1 && axis != 2) return mgpu_fail(ctx, MGPU_E_INVALID, "RSD line of sight must be y (1) or z (2)");
+  if (!ctx->pk_bin) return mgpu_fail(ctx, MGPU_E_INVALID, "RSD multipoles need 0 < pofk_nbins <= 32767");
+  if (!ctx->rsd_crd) {
+    MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    MGPU_HIP(ctx, hipMalloc((void **)&ctx->rsd_crd, sizeof(float4) * std::max<size_t>(ctx->np, 1)));
+  }
+  ctx->have_forces = false;
+  float2 *dk = ctx->slot[3];
+  if (ctx->np) {
+    KScope ks(ctx, MGPU_K_POFK, 52.0 * ctx->np);
+    hipLaunchKernelGGL(k_rsd_coords, dim3(cdiv(ctx->np, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->pos, ctx->vel,
+                       ctx->lpt[0], ctx->lpt[1], ctx->np, axis, vnorm, (double)N / ctx->p.box,
+                       ctx->p.scale_dependent ? 1 : 0, ctx->p.use_cola, s1, s2, N, ctx->rsd_crd);
+    MGPU_HIP(ctx, hipGetLastError());
+  }
+  int rc;
+  if ((rc = deposit_tiled(ctx, (float *)dk, ctx->rsd_crd))) return rc;
+  if ((rc = fft_r2c_3d(ctx, &ctx->fft, dk, (const float *)dk))) return rc;
+  MGPU_HIP(ctx, hipMemsetAsync(ctx->pk, 0, 4 * (size_t)nbins * sizeof(double), ctx->stream));
+  const size_t shmem = 4 * (size_t)nbins * sizeof(double);
+  if (shmem > 65536)
+    MGPU_HIP(ctx, hipFuncSetAttribute((const void *)k_rsd_bins, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+  const size_t modes = (size_t)N * N * (N / 2 + 1);
+  {
+    KScope ks(ctx, MGPU_K_POFK, 8.0 * modes);
+    hipLaunchKernelGGL(k_rsd_bins, dim3(std::min<size_t>(cdiv(modes, kBlock), 4096)), dim3(kBlock), shmem, ctx->stream,
+                       dk, N, ctx->HP, ctx->winv, ctx->pk_bin, nbins, ctx->pk);
+  }
+  MGPU_HIP(ctx, hipGetLastError());
+  std::vector<double> h(4 * (size_t)nbins);
+  MGPU_HIP(ctx, hipMemcpyAsync(h.data(), ctx->pk, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const double box3 = std::pow(ctx->p.box, 3), shot = std::pow(ctx->p.box / (double)ctx->Np, 3);
+  const double kmin = ctx->p.pofk_kmin, kmax = ctx->p.pofk_kmax;
+  for (int b = 0; b < nbins; b++) {
+    double p0 = h[b], p2 = h[nbins + b], p4 = h[2 * nbins + b], n = h[3 * nbins + b], k = 0.0;
+    if (n > 0) {
+      p0 = (p0 / n) * box3;
+      p2 = (p2 / n) * box3;
+      p4 = (p4 / n) * box3;
+      p4 = 9.0 * (35.0 * p4 - 30.0 * p2 + 3.0 * p0) / 8.0;
+      p2 = 5.0 * (3.0 * p2 - 1.0 * p0) / 2.0;
+      if (ctx->p.pofk_subtract_shotnoise) {
+        p0 -= shot;
+        p2 -= shot;
+        p4 -= shot;
+      }
+      // k_from_index (src/compute_pofk.c:51-64)
+      if (ctx->p.pofk_bintype == 0) k = (kmin + (kmax - kmin) / (double)nbins * b) * 2.0 * M_PI / ctx->p.box;
+      else k = std::exp(std::log(kmin) + std::log(kmax / kmin) / (double)nbins * b) * 2.0 * M_PI / ctx->p.box;
+    }
+    if (out->P0) out->P0[b] = p0;
+    if (out->P2) out->P2[b] = p2;
+    if (out->P4) out->P4[b] = p4;
+    if (out->k) out->k[b] = k;
+    if (out->n) out->n[b] = n;
+  }
   return MGPU_OK;
 }

@@ -120,6 +120,37 @@ static int drift(sim *s, double A, double AFF, double AF, double Di, double Di2)
   return 0;
 }
 
+/* compute_RSD_powerspectrum (src/compute_pofk.c:403-500): P0, P2, P4 of the y and z lines
+ * of sight, averaged, to pofk_RSD_*.  Scale-dependent COLA needs dD/dy in LPT slot 1. */
+static int rsd_pofk(sim *s, double A) {
+  const mgp_params *p = s->p;
+  const double UnitTime_in_s = p->UnitLength_in_cm / p->UnitVelocity_in_cm_per_s;
+  const double Hubble = MGP_HUBBLE * UnitTime_in_s;
+  const double vnorm = (Hubble / A) / (100.0 * A * mgp_hubble(s->c, A)) * ((double)p->Nmesh / p->Box);
+  const double s1 = p->scale_dependent ? 0.0 : mgp_growth(s->c, MGP_G_dDdy, A);
+  const double s2 = p->scale_dependent ? 0.0 : mgp_growth(s->c, MGP_G_dD2dy, A);
+  const int nb = s->nbins;
+  double *buf = calloc(10 * (size_t)nb, sizeof(double));
+  const double *y[5], *z[5];
+  for (int l = 0; l < 5; l++) {
+    y[l] = buf + l * nb;
+    z[l] = buf + (5 + l) * nb;
+  }
+  mgpu_rsd_pofk ry = {buf, buf + nb, buf + 2 * nb, buf + 3 * nb, buf + 4 * nb};
+  mgpu_rsd_pofk rz = {buf + 5 * nb, buf + 6 * nb, buf + 7 * nb, buf + 8 * nb, buf + 9 * nb};
+  int rc = mgpu_rsd_power_spectrum(s->g, 1, vnorm, s1, s2, &ry);
+  if (!rc) rc = mgpu_rsd_power_spectrum(s->g, 2, vnorm, s1, s2, &rz);
+  if (rc) {
+    snprintf(s->err, s->errlen, "RSD power spectrum: %s", mgpu_last_error(s->g));
+    free(buf);
+    return -1;
+  }
+  if (mgp_write_rsd_pofk(p, A, y, z, nb) && s->verbose)
+    fprintf(stderr, "warning: could not write the RSD P(k) file in %s\n", p->OutputDir);
+  free(buf);
+  return 0;
+}
+
 /* Output, src/main.c:792-1062 */
 static int output(sim *s, double A, double AF, double AFF, double z_label) {
   const mgp_params *p = s->p;
@@ -127,6 +158,9 @@ static int output(sim *s, double A, double AF, double AFF, double z_label) {
   if (p->scale_dependent) {
     if (lpt_fields(s, 1, MGP_FIELD_dDdy, A, AFF)) return -1;
   }
+  /* RSD multipoles at the output, with the velocities synchronised (src/main.c:875-879) */
+  const int first_step = fabs(p->Init_Redshift - z_label) / p->Init_Redshift <= 1.0E-6;
+  if (p->compute_pofk && p->pofk_compute_rsd_pofk >= 1 && !first_step && rsd_pofk(s, A)) return -1;
   float *pos = malloc(sizeof(float) * 3 * np), *vel = malloc(sizeof(float) * 3 * np);
   float *l0 = malloc(sizeof(float) * 3 * np), *l1 = malloc(sizeof(float) * 3 * np);
   int rc = mgpu_get_particles(s->g, pos, vel) || mgpu_get_lpt(s->g, 0, l0) || mgpu_get_lpt(s->g, 1, l1);
@@ -166,6 +200,15 @@ static int get_displacements(sim *s, double A) {
   if (want_pk && s->pk &&
       mgp_write_pofk(p, A, "CDM", s->pk, s->pkk, s->pkn, s->nbins, s->bintype, s->kmin, s->kmax) && s->verbose)
     fprintf(stderr, "warning: could not write P(k) file in %s\n", p->OutputDir);
+  /* RSD multipoles every step (src/auxPM.c:373-380).  The reference skips the steps it
+   * takes to be output steps with (timeStep == 0) && (i != NoutputStart), where i is
+   * PtoMesh's slab loop counter (2 alloc_slice by then), so in effect every first step
+   * of an output interval.  Scale-dependent: dD/dy at A goes to LPT slot 1, which the
+   * step refills before Kick. */
+  if (p->compute_pofk && p->pofk_compute_rsd_pofk == 1 && s->ts != 0) {
+    if (p->scale_dependent && p->UseCOLA && lpt_fields(s, 1, MGP_FIELD_dDdy, A, A)) return -1;
+    if (rsd_pofk(s, A)) return -1;
+  }
   return 0;
 }
 

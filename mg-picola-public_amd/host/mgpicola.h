@@ -135,6 +135,7 @@ const char *mgp_sim_error(const mgp_sim *s);
 double mgp_sim_a(const mgp_sim *s);
 
 /* ---- output (src/main.c:792-1149, src/compute_pofk.c:239-261) ---- */
+int mgp_write_rsd_pofk(const mgp_params *p, double a, const double *const y[5], const double *const z[5], int nbins);
 int mgp_write_pofk(const mgp_params *p, double a, const char *label, const double *pofk, const double *kmean,
                    const double *nmodes, int nbins, int bintype, double kmin, double kmax);
 int mgp_write_gadget(const mgp_params *p, double A, double z_label, const float *pos, const float *vel, size_t np);

@@ -36,6 +36,32 @@ int mgp_write_pofk(const mgp_params *p, double a, const char *label, const doubl
   return 0;
 }
 
+/* compute_RSD_powerspectrum's file (src/compute_pofk.c:457-485): the multipoles of the y
+ * and z lines of sight averaged, |y - z|/sqrt(2) as the error, for bins with modes and k > 0.
+ * Each of y and z is {P0, P2, P4, k, n}. */
+int mgp_write_rsd_pofk(const mgp_params *p, double a, const double *const y[5], const double *const z[5], int nbins) {
+  double znow = 1.0 / a - 1.0;
+  int zint = (int)znow, zfrac = (int)((znow - zint) * 1000);
+  char fn[1200];
+  snprintf(fn, sizeof fn, "%s/pofk_RSD_%s_z%d.%03d.txt", p->OutputDir, p->FileBase, zint, zfrac);
+  FILE *fp = fopen(fn, "w");
+  if (!fp) return -1;
+  fprintf(fp, "#  k (h/Mpc)      P0 (Mpc/h)^3      P2 (Mpc/h)^3      P4 (Mpc/h)^3     sigma0     sigma2     sigma4\n");
+  for (int i = 0; i < nbins; i++) {
+    if (y[4][i] > 0 && y[3][i] > 0.0) {
+      double P[3], e[3];
+      for (int l = 0; l < 3; l++) {
+        P[l] = (y[l][i] + z[l][i]) / 2.0;
+        e[l] = fabs(y[l][i] - z[l][i]) / sqrt(2.0);
+      }
+      fprintf(fp, "%10.5f   %10.5f   %10.5f   %10.5f   %10.5f   %10.5f   %10.5f\n", y[3][i], P[0], P[1], P[2], e[0],
+              e[1], e[2]);
+    }
+  }
+  fclose(fp);
+  return 0;
+}
+
 #pragma pack(push, 4)
 typedef struct {
   uint32_t npart[6];

@@ -21,7 +21,7 @@ MESH_DENSITY_X, MESH_DENSITY_K, MESH_PHI_K, MESH_FORCE_X, MESH_FORCE_Y, MESH_FOR
 MGPU_SYMBOLS = [
     "mgpu_n2_table_len", "mgpu_last_error", "mgpu_device_info", "mgpu_create", "mgpu_destroy", "mgpu_sync",
     "mgpu_set_particles", "mgpu_get_particles", "mgpu_set_lpt", "mgpu_get_lpt", "mgpu_get_disp",
-    "mgpu_pm_density", "mgpu_pm_deposit", "mgpu_pm_transform", "mgpu_power_spectrum", "mgpu_fifth_force", "mgpu_forces", "mgpu_mesh_to_particles",
+    "mgpu_pm_density", "mgpu_pm_deposit", "mgpu_pm_transform", "mgpu_power_spectrum", "mgpu_rsd_power_spectrum", "mgpu_fifth_force", "mgpu_forces", "mgpu_mesh_to_particles",
     "mgpu_get_displacements", "mgpu_kick", "mgpu_drift", "mgpu_ic_displacement_fields", "mgpu_init_particles",
     "mgpu_lpt_fields", "mgpu_get_mesh", "mgpu_set_mesh", "mgpu_get_timers", "mgpu_enable_timers",
     "mgpu_kernel_name", "mgpu_enable_kernel_timing", "mgpu_kernel_timing", "mgpu_lpt_fields_slices",
@@ -45,6 +45,10 @@ class MG(C.Structure):
 class Pofk(C.Structure):
     _fields_ = [("pofk", C.POINTER(C.c_double)), ("kmean", C.POINTER(C.c_double)),
                 ("nmodes", C.POINTER(C.c_double))]
+
+
+class RsdPofk(C.Structure):
+    _fields_ = [(n, C.POINTER(C.c_double)) for n in ("P0", "P2", "P4", "k", "n")]
 
 
 class Timers(C.Structure):
@@ -215,6 +219,17 @@ class MGPU:
     def power_spectrum(self):
         s, bufs = self._pofk_struct()
         self._ck(lib().mgpu_power_spectrum(self.ctx, C.byref(s)), "power_spectrum")
+        return tuple(bufs)
+
+    def rsd_power_spectrum(self, axis, vnorm, s1=0.0, s2=0.0):
+        """P0, P2, P4, k, n of one line of sight (axis 1 = y, 2 = z)."""
+        nb = self.p.pofk_nbins
+        bufs = [np.zeros(nb) for _ in range(5)]
+        r = RsdPofk(*[b.ctypes.data_as(C.POINTER(C.c_double)) for b in bufs])
+        L = lib()
+        L.mgpu_rsd_power_spectrum.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double,
+                                              C.POINTER(RsdPofk)]
+        self._ck(L.mgpu_rsd_power_spectrum(self.ctx, axis, vnorm, s1, s2, C.byref(r)), "rsd_power_spectrum")
         return tuple(bufs)
 
     def fifth_force(self, mg):

@@ -64,6 +64,12 @@ void or_pm_deposit(const or_cfg *c, size_t np, const float *const pos[3], double
 void or_pofk(const or_cfg *c, const double *dk, int nbins, int bintype, double kmin, double kmax,
              int subtract_shotnoise, double *pofk, double *kmean, double *nmodes);
 void or_adjust_pofk(int nmesh, int *nbins, int *bintype, int *subtract, double *kmin, double *kmax);
+/* PtoMesh_RSD (axis 1 = y, 2 = z) and bin_up_RSD_power_spectrum */
+void or_pm_deposit_rsd(const or_cfg *c, size_t np, const float *const pos[3], const float *const vel[3],
+                       const float *const l0[3], const float *const l1[3], int axis, double vnorm,
+                       int scale_dependent, int use_cola, double s1, double s2, double *density);
+void or_rsd_multipoles(const or_cfg *c, const double *dk, int nbins, int bintype, double kmin, double kmax,
+                       int subtract_shotnoise, double *P0, double *P2, double *P4, double *k, double *n);
 /* ComputeFifthForce: dk = delta(k) (P3D), dx = copy of delta(x) (mgarray_two before the r2c).
  * phik receives P3D_mgarray_two (to be added to P3D in Forces); for JBD dk is scaled in place. */
 int  or_mg_fifth_force(const or_cfg *c, const or_mg *m, double *dk, const double *dx, double *phik);

@@ -57,6 +57,10 @@ def lib():
         L.or_pofk.argtypes = [C.POINTER(Cfg), _D, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _D, _D, _D]
         L.or_adjust_pofk.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                      C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.or_pm_deposit_rsd.argtypes = [C.POINTER(Cfg), C.c_size_t, _P3, _P3, _P3, _P3, C.c_int, C.c_double, C.c_int,
+                                        C.c_int, C.c_double, C.c_double, _D]
+        L.or_rsd_multipoles.argtypes = [C.POINTER(Cfg), _D, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _D,
+                                        _D, _D, _D, _D]
         L.or_mg_fifth_force.restype = C.c_int
         L.or_mg_fifth_force.argtypes = [C.POINTER(Cfg), C.POINTER(MG), _D, _D, _D]
         L.or_pm_forces.argtypes = [C.POINTER(Cfg), _D, C.c_void_p, _D, _D, _D]
@@ -135,6 +139,21 @@ def pofk(c, dk, nbins, bintype, kmin, kmax, subtract):
     p, k, n = np.zeros(nbins), np.zeros(nbins), np.zeros(nbins)
     lib().or_pofk(C.byref(c), dk, nbins, bintype, kmin, kmax, subtract, p, k, n)
     return p, k, n
+
+
+def deposit_rsd(c, pos, vel, l0, l1, axis, vnorm, scale_dependent, use_cola, s1=0.0, s2=0.0):
+    """PtoMesh_RSD: axis 1 = y, 2 = z; vnorm in mesh cells per velocity unit."""
+    d = new_mesh(c.nmesh)
+    arrs = [_soa(x) for x in (pos, vel, l0, l1)]
+    lib().or_pm_deposit_rsd(C.byref(c), pos.shape[1], *[_p3(a) for a in arrs], axis, vnorm, scale_dependent,
+                            use_cola, s1, s2, d)
+    return d
+
+
+def rsd_multipoles(c, dk, nbins, bintype, kmin, kmax, subtract):
+    out = [np.zeros(nbins) for _ in range(5)]
+    lib().or_rsd_multipoles(C.byref(c), dk, nbins, bintype, kmin, kmax, subtract, *out)
+    return tuple(out)  # P0, P2, P4, k, n
 
 
 def fifth_force(c, m, dk, dx):

@@ -375,3 +375,140 @@ void or_drift(size_t np, float *const pos[3], const float *const vel[3], const f
       pos[a][n] = wrapf((float)(pos[a][n] + use_cola * lpt_term(d1, d2, s1, s2, sd, a, n)), box);
     }
 }
+
+/* ------------------------------------------------------------------------
+ * PtoMesh_RSD, src/compute_pofk.c:279-395: the CIC deposit with axis
+ * [axis] (1 = y, 2 = z) moved to redshift space.  Mesh coordinates are
+ * (Pos0, Pos1, Pos2) for axis z and (Pos0, Pos2, Pos1) for axis y; the third
+ * gets + vnorm * V with one periodic wrap, vnorm = (Hubble/A)/(100 A hubble(A)) * N/Box.
+ * V = Vel[axis] (+ LPT velocity with COLA): scale-dependent, l1 holds dD/dy summed
+ * over both orders (P.dDdy + P.dD2dy, a float sum); otherwise D dDdy(A) + D2 dD2dy(A)
+ * with s1 = dDdy(A), s2 = dD2dy(A), l0 = D, l1 = D2.
+ * ---------------------------------------------------------------------- */
+void or_pm_deposit_rsd(const or_cfg *c, size_t np, const float *const pos[3], const float *const vel[3],
+                       const float *const l0[3], const float *const l1[3], int axis, double vnorm,
+                       int scale_dependent, int use_cola, double s1, double s2, double *density) {
+  const int N = c->nmesh;
+  const double scaleBox = (double)N / c->box;
+  const double WPAR = pow((double)N / (double)c->nsample, 3);
+  if (axis != 1 && axis != 2) axis = 2;
+  const int ya = axis == 1 ? 2 : 1;
+  const size_t total = or_real_len(N);
+  for (size_t i = 0; i < total; i++) density[i] = -1.0;
+  for (size_t i = 0; i < np; i++) {
+    double X = pos[0][i] * scaleBox, Y = pos[ya][i] * scaleBox, Z = pos[axis][i] * scaleBox;
+    double V;
+    if (!use_cola) V = vel[axis][i];
+    else if (scale_dependent) V = (float)(vel[axis][i] + l1[axis][i]);
+    else V = vel[axis][i] + (l0[axis][i] * s1 + l1[axis][i] * s2);
+    V *= vnorm;
+    Z += V;
+    if (Z >= (double)N) Z -= (double)N;
+    if (Z < 0) Z += (double)N;
+    unsigned IX = (unsigned)X, IY = (unsigned)Y, IZ = (unsigned)Z;
+    double DX = X - (double)IX, DY = Y - (double)IY, DZ = Z - (double)IZ;
+    double TX = 1.0 - DX, TY = 1.0 - DY, TZ = 1.0 - DZ;
+    DY *= WPAR;
+    TY *= WPAR;
+    if (IY >= (unsigned)N) IY = 0;
+    if (IZ >= (unsigned)N) IZ = 0;
+    unsigned IXn = IX + 1, IYn = IY + 1, IZn = IZ + 1;
+    if (IYn >= (unsigned)N) IYn = 0;
+    if (IZn >= (unsigned)N) IZn = 0;
+    density[ridx(N, IX, IY, IZ)] += TX * TY * TZ;
+    density[ridx(N, IX, IY, IZn)] += TX * TY * DZ;
+    density[ridx(N, IX, IYn, IZ)] += TX * DY * TZ;
+    density[ridx(N, IX, IYn, IZn)] += TX * DY * DZ;
+    density[ridx(N, IXn, IY, IZ)] += DX * TY * TZ;
+    density[ridx(N, IXn, IY, IZn)] += DX * TY * DZ;
+    density[ridx(N, IXn, IYn, IZ)] += DX * DY * TZ;
+    density[ridx(N, IXn, IYn, IZn)] += DX * DY * DZ;
+  }
+  const size_t slice = (size_t)N * 2 * (N / 2 + 1);
+  if (np != 0)
+    for (size_t i = 0; i < slice; i++) density[i] += density[(size_t)N * slice + i] + 1.0;
+}
+
+/* k_from_index, src/compute_pofk.c:51-64 (h/Mpc) */
+static double k_from_index(int index, double kmin, double kmax, int nbins, int bintype, double box) {
+  if (bintype == 0) return (kmin + (kmax - kmin) / (double)nbins * index) * 2.0 * OR_PI / box;
+  if (bintype == 1) return exp(log(kmin) + log(kmax / kmin) / (double)nbins * index) * 2.0 * OR_PI / box;
+  return 0.0;
+}
+
+/* one mode of bin_up_RSD_power_spectrum: weight w, mu^2 = d2^2/k^2 */
+static void rsd_add(const double *v, double gc, double w, double mu2, double kmag, int nk, double *p0, double *p2,
+                    double *p4, double *kb, double *nb) {
+  double pofk = (v[0] * v[0] + v[1] * v[1]) * gc;
+  p0[nk] += w * pofk;
+  p2[nk] += w * pofk * mu2;
+  p4[nk] += w * pofk * mu2 * mu2;
+  kb[nk] += w * kmag;
+  nb[nk] += w;
+}
+
+/* ------------------------------------------------------------------------
+ * bin_up_RSD_power_spectrum, src/compute_pofk.c:496-753: sums of |delta|^2,
+ * mu^2 |delta|^2, mu^4 |delta|^2 (line of sight = mesh z) with the P(k) mode
+ * weights and deconvolution, normalised by the mode counts and Box^3, turned
+ * into the Legendre multipoles P0, P2, P4 (in that order of assignment), shot
+ * noise subtracted from all three; k = k_from_index.  kmin/kmax in integer
+ * wavenumbers as after adjust_pofk_parameters.
+ * ---------------------------------------------------------------------- */
+void or_rsd_multipoles(const or_cfg *c, const double *dk, int nbins, int bintype, double kmin, double kmax,
+                       int subtract_shotnoise, double *P0, double *P2, double *P4, double *k, double *n) {
+  const int N = c->nmesh;
+  double *kb = calloc(nbins, sizeof(double));
+  for (int i = 0; i < nbins; i++) P0[i] = P2[i] = P4[i] = n[i] = 0.0;
+  const double norm = 1.0 / pow((double)N, 6);
+  for (int i = 0; i < N; i++) {
+    int d0 = i > N / 2 ? N - i : i;
+    double gx = d0 == 0 ? 1.0 : sin((OR_PI * d0) / (double)N) / ((OR_PI * d0) / (double)N);
+    for (int j = 0; j < N / 2 + 1; j++) {
+      int d1 = j;
+      double gy = d1 == 0 ? 1.0 : sin((OR_PI * d1) / (double)N) / ((OR_PI * d1) / (double)N);
+      for (int pass = 0; pass < 2; pass++) {
+        int d2 = pass == 0 ? 0 : N / 2;
+        double gz = pass == 0 ? 1.0 : 2.0 / OR_PI;
+        double kmag = sqrt((double)(d0 * d0 + d1 * d1 + d2 * d2));
+        double mu2 = d2 * d2 / (kmag * kmag);
+        if (pass == 0 && i == 0 && j == 0) mu2 = 0.0;
+        int nk = bin_index(kmag, kmin, kmax, nbins, bintype);
+        if (nk >= 0 && nk < nbins) {
+          double gc = 1.0 / pow(gx * gy * gz, 4.0) * norm;
+          rsd_add(dk + 2 * cidx(N, i, j, d2), gc, 1.0, mu2, kmag, nk, P0, P2, P4, kb, n);
+          if (j != N / 2 && j != 0) rsd_add(dk + 2 * cidx(N, i, N - j, d2), gc, 1.0, mu2, kmag, nk, P0, P2, P4, kb, n);
+        }
+      }
+      for (int kz = 1; kz < N / 2; kz++) {
+        double gz = sin((OR_PI * kz) / (double)N) / ((OR_PI * kz) / (double)N);
+        double kmag = sqrt((double)(d0 * d0 + d1 * d1 + kz * kz));
+        double mu2 = kz * kz / kmag / kmag;
+        int nk = bin_index(kmag, kmin, kmax, nbins, bintype);
+        if (nk >= 0 && nk < nbins) {
+          double gc = 1.0 / pow(gx * gy * gz, 4.0) * norm;
+          rsd_add(dk + 2 * cidx(N, i, j, kz), gc, 2.0, mu2, kmag, nk, P0, P2, P4, kb, n);
+          if (j != N / 2 && j != 0) rsd_add(dk + 2 * cidx(N, i, N - j, kz), gc, 2.0, mu2, kmag, nk, P0, P2, P4, kb, n);
+        }
+      }
+    }
+  }
+  const double shot = pow(c->box / (double)c->nsample, 3), box3 = pow(c->box, 3);
+  for (int i = 0; i < nbins; i++) {
+    k[i] = 0.0;
+    if (n[i] > 0) {
+      P0[i] = (P0[i] / n[i]) * box3;
+      P2[i] = (P2[i] / n[i]) * box3;
+      P4[i] = (P4[i] / n[i]) * box3;
+      P4[i] = 9.0 * (35.0 * P4[i] - 30.0 * P2[i] + 3.0 * P0[i]) / 8.0;
+      P2[i] = 5.0 * (3.0 * P2[i] - 1.0 * P0[i]) / 2.0;
+      if (subtract_shotnoise) {
+        P0[i] -= shot;
+        P2[i] -= shot;
+        P4[i] -= shot;
+      }
+      k[i] = k_from_index(i, kmin, kmax, nbins, bintype, c->box);
+    }
+  }
+  free(kb);
+}

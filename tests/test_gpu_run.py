@@ -132,3 +132,24 @@ def test_lpt_two_slot_fold_matches_single_slot(tmp_path, n, ns):
     assert rel_err(g.get_lpt(1), r1) < 1e-6
     assert rel_err(r0, r1) > 1e-3  # the two slots hold different fields
     del ba, bb
+
+
+def test_rsd_files_written_every_step_and_at_output(tmp_path):
+    """pofk_compute_rsd_pofk = 1 (scale-dependent f(R)): the RSD multipole file is
+    written at every step but the first of each output interval and at the
+    output, in the reference's format (7 columns, k > 0)."""
+    hp = make_params(tmp_path, "fofr_sd_512_1024", 3, Nmesh=32, Nsample=16, Box=64.0, pofk_nbins=16,
+                     pofk_compute_rsd_pofk=1)
+    sim = M.Sim(hp)
+    sim.ic()
+    while not sim.finished:
+        sim.advance()
+    sim.close()
+    files = sorted(f for f in os.listdir(str(tmp_path)) if f.startswith("pofk_RSD_"))
+    # 3 steps from z = 19: steps 2 and 3 in the loop, and the output at z = 0
+    assert len(files) == 3, files
+    for f in files:
+        rows = np.loadtxt(os.path.join(str(tmp_path), f), ndmin=2)
+        assert rows.shape[1] == 7 and rows.shape[0] > 3
+        assert np.all(np.isfinite(rows)) and np.all(rows[:, 0] > 0)
+        assert np.all(rows[:, 4:] >= 0)

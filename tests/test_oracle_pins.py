@@ -284,6 +284,71 @@ def test_adjust_pofk_defaults():
     assert O.adjust_pofk(64, 10, 0, 0, 2.0, 30.0) == (10, 0, 0, 2.0, 30.0)
 
 
+# ---------------------------------------------------------------- RSD multipoles
+
+@pytest.mark.parametrize("m,mu2", [((0, 0, 3), 1.0), ((3, 0, 0), 0.0), ((0, 2, 0), 0.0), ((2, 0, 2), 0.5),
+                                   ((0, -2, 2), 0.5)])
+def test_rsd_multipoles_plane_wave(m, mu2):
+    """A plane wave with mu^2 = m_z^2/|m|^2 (line of sight = mesh z) gives
+    P2 = 5/2 (3 mu^2 - 1) P0 and P4 = 9/8 (35 mu^4 - 30 mu^2 + 3) P0 in its bin,
+    with P0 the plain P(k) of the same field; zero elsewhere."""
+    n, ns, box, eps, nbins = 16, 8, 200.0, 0.1, 16
+    d, _ = cos_mesh(n, [(eps, m, 0.4)])
+    O.fft_r2c_3d(n, d)
+    c = O.cfg(n, ns, box, 0.3)
+    p0, p2, p4, k, nm = O.rsd_multipoles(c, d, nbins, 0, 0.0, float(n), 0)
+    pk, _, nm_pk = O.pofk(c, d, nbins, 0, 0.0, float(n), 0)
+    np.testing.assert_array_equal(nm, nm_pk)
+    np.testing.assert_allclose(p0, pk, rtol=1e-12, atol=1e-14 * pk.max())
+    b = int(np.floor(np.sqrt(np.dot(m, m)) / n * nbins + 0.5))
+    np.testing.assert_allclose(p2[b], 2.5 * (3 * mu2 - 1) * p0[b], rtol=1e-10)
+    np.testing.assert_allclose(p4[b], 9.0 / 8.0 * (35 * mu2 ** 2 - 30 * mu2 + 3) * p0[b], rtol=1e-10)
+    other = np.arange(nbins) != b
+    assert np.abs(p2[other]).max() < 1e-12 * abs(p0[b]) and np.abs(p4[other]).max() < 1e-12 * abs(p0[b])
+    # k of a bin is its nominal value (k_from_index), not the mode average
+    np.testing.assert_allclose(k[nm > 0], (np.arange(nbins) * n / nbins * 2 * np.pi / box)[nm > 0], rtol=1e-14)
+    # shot noise comes off all three multipoles
+    q0, q2, q4, _, _ = O.rsd_multipoles(c, d, nbins, 0, 0.0, float(n), 1)
+    sn = (box / ns) ** 3
+    for a, b_ in ((q0, p0), (q2, p2), (q4, p4)):
+        np.testing.assert_allclose(a[nm > 0], (b_ - sn)[nm > 0], rtol=1e-12, atol=1e-12 * sn)
+
+
+def rsd_particles(ns, box, seed):
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(0, box, size=(3, ns ** 3)).astype(np.float32)
+    z = np.zeros((3, ns ** 3), np.float32)
+    return pos, z
+
+
+def test_rsd_deposit_axis_mapping_and_shifts():
+    """Zero velocity: the z line of sight is the plain deposit and the y line
+    of sight the plain deposit of (x, z, y).  A velocity of an integer number
+    of cells rolls the field along the line of sight, for each form of V:
+    plain Vel, Vel + (dD/dy summed over orders) and Vel + D dDdy + D2 dD2dy."""
+    n, ns, box = 16, 8, 32.0
+    c = O.cfg(n, ns, box, 0.3)
+    pos, z = rsd_particles(ns, box, 7)
+    ref = O.real_view(O.deposit(c, pos), n)
+    dz = O.real_view(O.deposit_rsd(c, pos, z, z, z, 2, 1.0, 0, 0), n)
+    np.testing.assert_array_equal(dz, ref)
+    swapped = np.ascontiguousarray(pos[[0, 2, 1]])
+    dy = O.real_view(O.deposit_rsd(c, pos, z, z, z, 1, 1.0, 0, 0), n)
+    np.testing.assert_array_equal(dy, O.real_view(O.deposit(c, swapped), n))
+    one = np.ones_like(pos)
+    # 2 cells along z: vnorm = 1 cell per unit, V = 2 in three ways
+    cases = [dict(vel=2 * one, l0=z, l1=z, sd=0, cola=0, s1=0.0, s2=0.0),
+             dict(vel=one, l0=z, l1=one, sd=1, cola=1, s1=0.0, s2=0.0),
+             dict(vel=one, l0=one, l1=2 * one, sd=0, cola=1, s1=0.5, s2=0.25)]
+    for cs in cases:
+        d2 = O.real_view(O.deposit_rsd(c, pos, cs["vel"], cs["l0"], cs["l1"], 2, 1.0, cs["sd"], cs["cola"], cs["s1"],
+                                       cs["s2"]), n)
+        np.testing.assert_array_equal(d2, np.roll(ref, 2, axis=2)), cs
+    # and along y for the y line of sight (mesh z after the swap)
+    d3 = O.real_view(O.deposit_rsd(c, pos, 2 * one, z, z, 1, 1.0, 0, 0), n)
+    np.testing.assert_array_equal(d3, np.roll(O.real_view(O.deposit(c, swapped), n), 2, axis=2))
+
+
 # ---------------------------------------------------------------- modified gravity
 
 def mgd(**kw):
