@@ -198,6 +198,16 @@ int mgpu_lpt_fields_slices(mgpu_ctx *ctx, int slot, const mgpu_kslice *o1, const
  * from s1o1 + s1o2).  The stored spectra are read once for both slots. */
 int mgpu_lpt_fields_slices2(mgpu_ctx *ctx, const mgpu_kslice *s0o1, const mgpu_kslice *s0o2,
                             const mgpu_kslice *s1o1, const mgpu_kslice *s1o2);
+/* Massive neutrinos (FOFRNU): PtoMesh's neutrino term, src/auxPM.c:383-420,
+ *   delta(k) <- cdmfac delta(k) + fac(k) cdelta_IC(k)   for every mode but k = 0,
+ * cdmfac = (Omega - OmegaNu)/Omega, fac = OmegaNu/Omega N^3 T_nu(k, a)/T_cb(k, 1) given as a
+ * ln k slice (norm num/den, as mgpu_kslice).  Set once per step before the density;
+ * mgpu_pm_density and mgpu_get_displacements then apply it after the CDM P(k).  fac ==
+ * NULL turns it off.  Needs a scale-dependent context (the IC spectrum is kept). */
+int mgpu_set_neutrinos(mgpu_ctx *ctx, double cdmfac, const mgpu_kslice *fac);
+/* the neutrino term alone, on the current delta(k) */
+int mgpu_neutrino_response(mgpu_ctx *ctx);
+
 
 /* ---- inspection (tests, output) ---- */
 int mgpu_get_mesh(mgpu_ctx *ctx, int mesh, double *host);

@@ -270,7 +270,8 @@ void mgpu_destroy(mgpu_ctx *ctx) {
                   ctx->slot[0],  ctx->slot[1],   ctx->slot[2],   ctx->slot[3],   ctx->cdelta[0],
                   ctx->cdelta[1], ctx->partial,  ctx->dsum,      ctx->pk,        ctx->n2tab[0],
                   ctx->n2tab[1], ctx->n2tabf[0], ctx->n2tabf[1], ctx->lattice_slot, ctx->slice_buf,
-                  ctx->dep_scratch, ctx->winv,    ctx->winvf,     ctx->pk_bin,    ctx->rsd_crd};
+                  ctx->dep_scratch, ctx->winv,    ctx->winvf,     ctx->pk_bin,    ctx->rsd_crd,
+                  ctx->nu_slice, ctx->nu_fac};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (ctx->slice_host) (void)hipHostFree(ctx->slice_host);
@@ -359,7 +360,20 @@ int mgpu_pm_density(mgpu_ctx *ctx, mgpu_pofk *pk) {
   ctx->have_deposit = true;
   ctx->have_forces = false;
   ctx->have_phik = false;
-  return mgpu_pm_transform(ctx, pk);
+  if ((rc = mgpu_pm_transform(ctx, pk))) return rc;
+  // the neutrino term follows the CDM P(k) (src/auxPM.c:367-420)
+  return ctx->nu_on ? ic_nu_apply(ctx) : MGPU_OK;
+}
+
+int mgpu_set_neutrinos(mgpu_ctx *ctx, double cdmfac, const mgpu_kslice *fac) {
+  if (check_ctx(ctx)) return MGPU_E_INVALID;
+  return ic_nu_set(ctx, cdmfac, fac);
+}
+
+int mgpu_neutrino_response(mgpu_ctx *ctx) {
+  if (check_ctx(ctx)) return MGPU_E_INVALID;
+  if (!ctx->have_density) return mgpu_fail(ctx, MGPU_E_STATE, "no density(k): call mgpu_pm_density first");
+  return ic_nu_apply(ctx);
 }
 
 int mgpu_power_spectrum(mgpu_ctx *ctx, mgpu_pofk *pk) {

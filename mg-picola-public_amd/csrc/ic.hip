@@ -487,6 +487,38 @@ __global__ __launch_bounds__(kBlock) void k_growth_n2(size_t n2len, double kf, K
   g2f[n2] = (float)b;
 }
 
+// ---------------------------------------------------------------- massive neutrinos
+// The factor of PtoMesh's neutrino term per |n|^2 from a ln k slice:
+// fac = norm num/den at ln(kf |n|) (OmegaNu/Omega N^3 T_nu(k, a)/T_cb(k, 1)).
+__global__ __launch_bounds__(kBlock) void k_nu_fac(size_t n2len, double kf, KSl s, const double *__restrict__ t,
+                                                   double *__restrict__ fac) {
+  const size_t n2 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n2 >= n2len) return;
+  double v = 0.0;
+  if (n2 > 0) {
+    const double lk = log(kf * sqrt((double)n2));
+    v = s.norm * (herm(t, t + s.ny, s, lk) / herm(t + 2 * s.ny, t + 3 * s.ny, s, lk));
+  }
+  fac[n2] = v;
+}
+
+// PtoMesh's neutrino term (src/auxPM.c:383-420) on every stored mode but k = 0:
+// delta = cdmfac delta + fac(|n|^2) cdelta_IC, in FP64 as the reference.  Grid
+// (N*N rows, kz blocks).
+__global__ __launch_bounds__(kBlock) void k_nu_add(float2 *__restrict__ dk, const float2 *__restrict__ cd, int N,
+                                                   int HP, const double *__restrict__ fac, double cdmfac) {
+  const int k = blockIdx.y * blockDim.x + threadIdx.x;
+  if (k > N / 2) return;
+  const unsigned row = blockIdx.x;
+  const int j = (int)(row % (unsigned)N), i = (int)(row / (unsigned)N);
+  if (i == 0 && j == 0 && k == 0) return;
+  const int d0 = i > N / 2 ? N - i : i, d1 = j > N / 2 ? N - j : j;
+  const double f = fac[d0 * d0 + d1 * d1 + k * k];
+  const size_t e = (size_t)row * HP + k;
+  const float2 v = dk[e], c = cd[e];
+  dk[e] = make_float2((float)(cdmfac * (double)v.x + f * (double)c.x), (float)(cdmfac * (double)v.y + f * (double)c.y));
+}
+
 // seed table on the host (a single serial stream), src/2LPT.c:259-271
 void seedtable_host(int N, unsigned long seed, std::vector<unsigned> &t) {
   Ranlxd1 r;
@@ -807,3 +839,49 @@ int ic_lpt_fields(mgpu_ctx *ctx, int slot, const double *g1_n2, const double *g2
   return MGPU_OK;
 }
 
+
+// the neutrino term's factors for the next PtoMesh (f == NULL: off)
+int ic_nu_set(mgpu_ctx *ctx, double cdmfac, const mgpu_kslice *f) {
+  if (!f) {
+    ctx->nu_on = false;
+    return MGPU_OK;
+  }
+  if (!f->num || !f->dnum || !f->den || !f->dden || f->nk < 2 || f->nk > 8192 || f->dlnk <= 0)
+    return mgpu_fail(ctx, MGPU_E_INVALID, "bad neutrino slice");
+  if (!ctx->cdelta[0]) return mgpu_fail(ctx, MGPU_E_STATE, "the neutrino term needs the stored IC spectrum (scale-dependent context)");
+  const size_t n = 4 * (size_t)f->nk;
+  std::vector<double> h(n);
+  std::copy(f->num, f->num + f->nk, h.begin());
+  std::copy(f->dnum, f->dnum + f->nk, h.begin() + f->nk);
+  std::copy(f->den, f->den + f->nk, h.begin() + 2 * f->nk);
+  std::copy(f->dden, f->dden + f->nk, h.begin() + 3 * f->nk);
+  if (ctx->nu_slice_len < n) {
+    MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->nu_slice) MGPU_HIP(ctx, hipFree(ctx->nu_slice));
+    ctx->nu_slice = nullptr;
+    MGPU_HIP(ctx, hipMalloc((void **)&ctx->nu_slice, n * sizeof(double)));
+    ctx->nu_slice_len = n;
+  }
+  if (!ctx->nu_fac) MGPU_HIP(ctx, hipMalloc((void **)&ctx->nu_fac, ctx->n2len * sizeof(double)));
+  MGPU_HIP(ctx, hipMemcpyAsync(ctx->nu_slice, h.data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  const KSl ks{f->lnk0, f->dlnk, f->norm, f->nk};
+  hipLaunchKernelGGL(k_nu_fac, dim3(cdiv(ctx->n2len, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->n2len,
+                     2 * MGPU_PI / ctx->p.box, ks, (const double *)ctx->nu_slice, ctx->nu_fac);
+  MGPU_HIP(ctx, hipGetLastError());
+  MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));  // h goes out of scope
+  ctx->nu_cdmfac = cdmfac;
+  ctx->nu_on = true;
+  return MGPU_OK;
+}
+
+// the neutrino term on delta(k) in slot 0
+int ic_nu_apply(mgpu_ctx *ctx) {
+  if (!ctx->nu_on) return mgpu_fail(ctx, MGPU_E_STATE, "no neutrino factors: call mgpu_set_neutrinos first");
+  const int N = ctx->N;
+  KScope ks(ctx, MGPU_K_SPECTRAL, 24.0 * (double)N * N * (N / 2 + 1));
+  hipLaunchKernelGGL(k_nu_add, dim3((unsigned)((size_t)N * N), cdiv((size_t)(N / 2 + 1), kBlock)), dim3(kBlock), 0,
+                     ctx->stream, ctx->slot[0], ctx->cdelta[0], N, ctx->HP, (const double *)ctx->nu_fac,
+                     ctx->nu_cdmfac);
+  MGPU_HIP(ctx, hipGetLastError());
+  return MGPU_OK;
+}

@@ -63,6 +63,12 @@ struct mgpu_ctx {
   float *winvf = nullptr;     // FP32 copy of winv
   int16_t *pk_bin = nullptr;  // P(k) bin of each |n|^2 (-1: outside the bins); null when nbins > 32767
   float4 *rsd_crd = nullptr;  // redshift-space particle records (allocated on the first RSD call)
+  // massive neutrinos: PtoMesh's term delta = cdmfac delta + fac(|n|^2) cdelta_IC while nu_on
+  bool nu_on = false;
+  double nu_cdmfac = 1.0;
+  double *nu_slice = nullptr;  // the factor's ln k slice (4 ny doubles)
+  size_t nu_slice_len = 0;
+  double *nu_fac = nullptr;    // factor per |n|^2 (n2len doubles)
   std::vector<double> pk_geom;  // mode-weighted |n| sums and mode counts per bin (2 * nbins), fixed per context
   double *n2tab[2] = {nullptr, nullptr};  // per-|n|^2 device tables
   float *n2tabf[2] = {nullptr, nullptr};  // FP32 copies of n2tab (lattice fold)
@@ -188,6 +194,8 @@ int pm_transform(mgpu_ctx *ctx, mgpu_pofk *out);  // r2c of delta(x) into slot 0
 int pm_fifth_force(mgpu_ctx *ctx, const mgpu_mg *mg);
 int pm_forces(mgpu_ctx *ctx);
 // the whole of GetDisplacements with paired x passes (pm_step_fusable says when it applies)
+int ic_nu_set(mgpu_ctx *ctx, double cdmfac, const mgpu_kslice *f);
+int ic_nu_apply(mgpu_ctx *ctx);
 int pm_rsd_pofk(mgpu_ctx *ctx, int axis, double vnorm, double s1, double s2, mgpu_rsd_pofk *out);
 bool pm_step_fusable(const mgpu_ctx *ctx, const mgpu_mg *mg, const mgpu_pofk *out);
 int pm_step_fused(mgpu_ctx *ctx, const mgpu_mg *mg, mgpu_pofk *out, double sumD[3]);

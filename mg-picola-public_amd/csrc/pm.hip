@@ -529,6 +529,7 @@ static int forces_tail(mgpu_ctx *ctx) {
 }
 
 bool pm_step_fusable(const mgpu_ctx *ctx, const mgpu_mg *mg, const mgpu_pofk *out) {
+  if (ctx->nu_on) return false;  // the neutrino term sits between the density's x pass and the next
   if (out && (!ctx->pk_bin || ctx->pk_geom.size() != 2 * (size_t)ctx->p.pofk_nbins)) return false;
   if (!ctx->p.mg_active || ctx->p.model == MGPU_LCDM) return true;
   if (!mg || !ctx->p.include_screening) return false;

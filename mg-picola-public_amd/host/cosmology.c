@@ -34,10 +34,12 @@ static int growth_rhs(double x, const double *D, double *dD, void *vp) {
       modfac = mgp_Factor_2LPT(c, a);
   }
   const double betafac = 1.5 * Om * mu / (a * a * a * H * H);
+  /* massive neutrinos: effective mu of the CDM+baryon growth (src/cosmo.c:224-236, :253-265) */
+  const double nu1 = mgp_munu_1lpt(c, a, k), nu2 = mgp_munu_2lpt(c, a, k);
   dD[0] = D[1];
-  dD[1] = -alphafac * D[1] + betafac * D[0];
+  dD[1] = -alphafac * D[1] + betafac * nu1 * D[0];
   dD[2] = D[3];
-  dD[3] = -alphafac * D[3] + betafac * (D[2] - modfac * D[0] * D[0]);
+  dD[3] = -alphafac * D[3] + betafac * nu2 * (D[2] - modfac * D[0] * D[0]);
   return 0;
 }
 
@@ -96,6 +98,32 @@ int mgp_solve_growth(struct mgp_cosmo *c) {
     double *z[6];
     for (int f = 0; f < 6; f++) z[f] = malloc(sizeof(double) * (size_t)npts * nk);
     int failed = 0;
+    if (c->nu) {
+      /* with massive neutrinos LCDM growth is scale-dependent too: its table, mu = 1,
+       * without the neutrino factor in the source terms (src/cosmo.c:829-889) */
+#pragma omp parallel for schedule(dynamic, 4) reduction(| : failed)
+      for (int j = 0; j < nk; j++) {
+        const double know = exp(logk[j]);
+        growth_par g = {c, know, 1};
+        double y[4] = {1.0, 1.0, -3.0 / 7.0, -6.0 / 7.0};
+        double xs = xini, h = 1e-4;
+        for (int i = 0; i < npts; i++) {
+          const double xn = xini + i * dx, a = exp(xn);
+          if (i > 0 && mgp_ode_integrate(growth_rhs, &g, 4, &xs, xn, y, &h, ODE_ABS, ODE_REL)) failed = 1;
+          const double Q = mgp_Qfactor(c, a), Om = p->Omega;
+          const size_t o = (size_t)j * npts + i;
+          z[0][o] = y[0];
+          z[1][o] = y[1] * Q / a;
+          z[2][o] = 1.5 * Om * a * y[0];
+          z[3][o] = y[2];
+          z[4][o] = y[3] * Q / a;
+          z[5][o] = 1.5 * Om * a * (y[2] - pow2(y[0]));
+        }
+      }
+      if (failed) return -1;
+      for (int f = 0; f < 6; f++) mgp_spline2d_init(&c->sd_lcdm[f], x, logk, z[f], npts, nk);
+      c->have_sd_lcdm = 1;
+    }
 #pragma omp parallel for schedule(dynamic, 4) reduction(| : failed)
     for (int j = 0; j < nk; j++) {
       const double know = exp(logk[j]);
@@ -106,13 +134,14 @@ int mgp_solve_growth(struct mgp_cosmo *c) {
         const double xn = xini + i * dx, a = exp(xn);
         if (i > 0 && mgp_ode_integrate(growth_rhs, &g, 4, &xs, xn, y, &h, ODE_ABS, ODE_REL)) failed = 1;
         const double mu = mgp_GeffoverG(c, a, know), Q = mgp_Qfactor(c, a), Om = p->Omega;
+        const double nu1 = mgp_munu_1lpt(c, a, know), nu2 = mgp_munu_2lpt(c, a, know);
         const size_t o = (size_t)j * npts + i;
         z[0][o] = y[0];
         z[1][o] = y[1] * Q / a;
-        z[2][o] = 1.5 * mu * Om * a * y[0];
+        z[2][o] = 1.5 * mu * nu1 * Om * a * y[0];
         z[3][o] = y[2];
         z[4][o] = y[3] * Q / a;
-        z[5][o] = 1.5 * mu * Om * a * (y[2] - pow2(y[0]));
+        z[5][o] = 1.5 * mu * nu2 * Om * a * (y[2] - pow2(y[0]));
       }
     }
     if (failed) return -1;
@@ -205,6 +234,9 @@ double mgp_SphiStd(const mgp_cosmo *c, double ai, double af) {
 /* mg_pofk_ratio (src/cosmo.c:688-712): raw (unnormalised) growth ratio squared */
 double mgp_mg_pofk_ratio(const mgp_cosmo *c, double k, double a) {
   if (!c->p->modified_gravity_active) return 1.0;
+  if (c->have_sd_lcdm)
+    return pow2(mgp_spline2d_eval(&c->sd[MGP_G_D], log(a), log(k)) /
+                mgp_spline2d_eval(&c->sd_lcdm[MGP_G_D], log(a), log(k)));
   double dl = mgp_spline_eval(&c->glcdm[MGP_G_D], log(a));
   if (c->p->scale_dependent) return pow2(mgp_spline2d_eval(&c->sd[MGP_G_D], log(a), log(k)) / dl);
   return pow2(mgp_spline_eval(&c->g[MGP_G_D], log(a)) / dl);
@@ -224,7 +256,8 @@ double mgp_mg_sigma8_enhancement(const mgp_cosmo *c, double a) {
     double kR8 = know * 8.0;
     double w = 3.0 / (kR8 * kR8 * kR8) * (sin(kR8) - kR8 * cos(kR8));
     double D = mgp_spline2d_eval(&c->sd[MGP_G_D], log(a), log(know));
-    double DL = mgp_spline_eval(&c->glcdm[MGP_G_D], log(a));
+    double DL = c->have_sd_lcdm ? mgp_spline2d_eval(&c->sd_lcdm[MGP_G_D], log(a), log(know))
+                                : mgp_spline_eval(&c->glcdm[MGP_G_D], log(a));
     double pw = mgp_power_spec(c, know);
     in += pw * w * w * D * D * know * know * know * dlogk / (2.0 * M_PI * M_PI);
     inl += pw * w * w * DL * DL * know * know * know * dlogk / (2.0 * M_PI * M_PI);
@@ -347,7 +380,20 @@ int mgp_lpt_kslices(const mgp_cosmo *c, int field, double A, double AFF, mgpu_ks
 mgp_cosmo *mgp_cosmo_create(mgp_params *p, char *err, size_t errlen) {
   struct mgp_cosmo *c = calloc(1, sizeof(*c));
   c->p = p;
-  c->fofr = p->model == MGP_MODEL_FOFR || p->model == MGP_MODEL_FOFR_LCDM;
+  c->fofr = p->model == MGP_MODEL_FOFR || p->model == MGP_MODEL_FOFR_LCDM || p->model == MGP_MODEL_FOFRNU;
+  /* massive neutrinos (init_modified_version, src/user_defined_functions.h:50-78) */
+  c->nu = p->model == MGP_MODEL_FOFRNU;
+  p->OmegaNu = 0.0;
+  p->OmegaCDM = p->Omega - p->OmegaBaryon;
+  if (c->nu) {
+    p->OmegaNu = p->nu_SumMassNuEV / (93.14 * p->HubbleParam * p->HubbleParam);
+    p->OmegaCDM = p->Omega - p->OmegaBaryon - p->OmegaNu;
+    if (p->OmegaNu < 1e-6) {
+      p->OmegaNu = 0.0;
+      p->OmegaCDM = p->Omega - p->OmegaBaryon;
+      p->nu_include_massive_neutrinos = 0;
+    }
+  }
   c->dgp = p->model == MGP_MODEL_DGP;
   c->jbd = p->model == MGP_MODEL_BRANSDICKE;
   c->mbeta = p->model == MGP_MODEL_MBETA;
@@ -381,6 +427,8 @@ void mgp_cosmo_free(mgp_cosmo *c) {
     if (c->g[i].n) mgp_spline_free(&c->g[i]);
     if (c->glcdm[i].n) mgp_spline_free(&c->glcdm[i]);
     if (c->sd[i].nx) mgp_spline2d_free(&c->sd[i]);
+    if (c->sd_lcdm[i].nx) mgp_spline2d_free(&c->sd_lcdm[i]);
+    if (i < 4 && c->camb[i].nx) mgp_spline2d_free(&c->camb[i]);
   }
   if (c->jbd_H.n) mgp_spline_free(&c->jbd_H);
   if (c->jbd_dHda.n) mgp_spline_free(&c->jbd_dHda);

@@ -196,10 +196,26 @@ static int get_displacements(sim *s, double A) {
   mgp_mg_coeffs(s->c, A, &mg);
   const int want_pk = p->compute_pofk && p->pofk_compute_every_step;
   mgpu_pofk pk = {s->pk, s->pkk, s->pkn};
+  const int nu_model = p->model == MGP_MODEL_FOFRNU;
+  if (nu_model && p->nu_include_massive_neutrinos) {
+    /* the neutrino term of PtoMesh at this a (src/auxPM.c:383-420) */
+    mgpu_kslice f;
+    if (mgp_nu_slice(s->c, A, &f, s->g1)) {
+      snprintf(s->err, s->errlen, "no neutrino transfer functions");
+      return -1;
+    }
+    GPU(mgpu_set_neutrinos(s->g, (p->Omega - p->OmegaNu) / p->Omega, &f));
+  }
   GPU(mgpu_get_displacements(s->g, p->modified_gravity_active ? &mg : NULL, want_pk ? &pk : NULL, s->sumD));
   if (want_pk && s->pk &&
       mgp_write_pofk(p, A, "CDM", s->pk, s->pkk, s->pkn, s->nbins, s->bintype, s->kmin, s->kmax) && s->verbose)
     fprintf(stderr, "warning: could not write P(k) file in %s\n", p->OutputDir);
+  if (nu_model && want_pk && s->pk) {
+    /* the total matter P(k) after the neutrino term (src/auxPM.c:422-426) */
+    GPU(mgpu_power_spectrum(s->g, &pk));
+    if (mgp_write_pofk(p, A, "total", s->pk, s->pkk, s->pkn, s->nbins, s->bintype, s->kmin, s->kmax) && s->verbose)
+      fprintf(stderr, "warning: could not write P(k) file in %s\n", p->OutputDir);
+  }
   /* RSD multipoles every step (src/auxPM.c:373-380).  The reference skips the steps it
    * takes to be output steps with (timeStep == 0) && (i != NoutputStart), where i is
    * PtoMesh's slab loop counter (2 alloc_slice by then), so in effect every first step
@@ -240,6 +256,7 @@ mgp_sim *mgp_sim_create(mgp_params *p, int verbose, char *err, size_t errlen) {
   gp.omega = p->Omega;
   switch (p->model) {
     case MGP_MODEL_FOFR:
+    case MGP_MODEL_FOFRNU:
     case MGP_MODEL_FOFR_LCDM: gp.model = MGPU_FOFR; break;
     case MGP_MODEL_DGP: gp.model = MGPU_DGP; break;
     case MGP_MODEL_BRANSDICKE: gp.model = MGPU_BRANSDICKE; break;

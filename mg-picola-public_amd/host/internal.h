@@ -20,6 +20,11 @@ struct mgp_cosmo {
   mgp_spline jbd_H, jbd_dHda, jbd_Geff;
   /* m-beta screening threshold phi(a) (src/cosmo.c:118-194) */
   mgp_spline phi_of_a;
+  /* massive neutrinos: CAMB transfer functions nu, cdm, baryon, total on (ln k, z) and the
+   * scale-dependent LCDM growth table that then replaces the scale-independent one */
+  int nu, have_camb, have_sd_lcdm;
+  mgp_spline2d camb[4];
+  mgp_spline2d sd_lcdm[6];
   /* power spectrum (src/power.c) */
   double Norm;
   int ntab;
@@ -37,6 +42,8 @@ int mgp_compute_phi_of_a(struct mgp_cosmo *c);
 
 /* cosmology.c */
 int mgp_solve_growth(struct mgp_cosmo *c);
+/* camb.c */
+int mgp_camb_read(struct mgp_cosmo *c, const char *infofile, char *err, size_t errlen);
 /* power.c */
 int mgp_init_power(struct mgp_cosmo *c, char *err, size_t errlen);
 

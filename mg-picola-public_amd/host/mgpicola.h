@@ -31,7 +31,8 @@ enum {
   MGP_MODEL_FOFR_LCDM = 1,  /* FOFRGRAVITY */
   MGP_MODEL_DGP = 2,        /* DGPGRAVITY + GAUSSIANFILTER */
   MGP_MODEL_BRANSDICKE = 3, /* BRANSDICKE */
-  MGP_MODEL_MBETA = 4       /* MBETAMODEL + SCALEDEPENDENT */
+  MGP_MODEL_MBETA = 4,      /* MBETAMODEL + SCALEDEPENDENT */
+  MGP_MODEL_FOFRNU = 5      /* FOFRGRAVITY + MASSIVE_NEUTRINOS + SCALEDEPENDENT */
 };
 
 typedef struct {
@@ -63,6 +64,11 @@ typedef struct {
   int Noutputs;
   double out_z[MGP_MAXOUT];
   int out_nsteps[MGP_MAXOUT];
+  /* massive neutrinos (FOFRNU; src/user_defined_functions.h:261-272, :50-78) */
+  int nu_include_massive_neutrinos;
+  double nu_SumMassNuEV;
+  char nu_FilenameTransferInfofile[500];
+  double OmegaNu, OmegaCDM; /* derived: SumMassNuEV/(93.14 h^2), Omega - OmegaBaryon - OmegaNu */
 } mgp_params;
 
 /* ---- paramfile (src/read_param.c) ---- */
@@ -135,6 +141,19 @@ const char *mgp_sim_error(const mgp_sim *s);
 double mgp_sim_a(const mgp_sim *s);
 
 /* ---- output (src/main.c:792-1149, src/compute_pofk.c:239-261) ---- */
+/* ---- massive neutrinos (CAMB transfer functions, src/read_CAMB_data.h) ---- */
+double mgp_T_nu(const mgp_cosmo *c, double k, double a);
+double mgp_T_cdm(const mgp_cosmo *c, double k, double a);
+double mgp_T_b(const mgp_cosmo *c, double k, double a);
+double mgp_T_tot(const mgp_cosmo *c, double k, double a);
+double mgp_T_cb(const mgp_cosmo *c, double k, double a);
+double mgp_pk_total(const mgp_cosmo *c, double k);
+double mgp_pk_cb(const mgp_cosmo *c, double k);
+double mgp_munu_1lpt(const mgp_cosmo *c, double a, double k);
+double mgp_munu_2lpt(const mgp_cosmo *c, double a, double k);
+/* OmegaNu/Omega N^3 T_nu(k, a)/T_cb(k, 1) on the scale-dependent ln k grid (buf: 4 ny doubles);
+ * -1 without neutrinos */
+int mgp_nu_slice(const mgp_cosmo *c, double a, mgpu_kslice *o, double *buf);
 int mgp_write_rsd_pofk(const mgp_params *p, double a, const double *const y[5], const double *const z[5], int nbins);
 int mgp_write_pofk(const mgp_params *p, double a, const char *label, const double *pofk, const double *kmean,
                    const double *nmodes, int nbins, int bintype, double kmin, double kmax);

@@ -38,6 +38,12 @@ static int build_tags(mgp_params *p, tagdef *t) {
     ADD("pofk_kmax", T_FLOAT, &p->pofk_kmax);
   }
   switch (p->model) {
+    case MGP_MODEL_FOFRNU:
+      ADD("nu_include_massive_neutrinos", T_INT, &p->nu_include_massive_neutrinos);
+      ADD("nu_SumMassNuEV", T_FLOAT, &p->nu_SumMassNuEV);
+      ADD("nu_FilenameTransferInfofile", T_STRING, p->nu_FilenameTransferInfofile);
+      /* and the f(R) tags */
+      __attribute__((fallthrough));
     case MGP_MODEL_FOFR:
     case MGP_MODEL_FOFR_LCDM:
       ADD("fofr0", T_FLOAT, &p->fofr0);
@@ -95,13 +101,14 @@ int mgp_model_from_name(const char *name) {
   if (!strcmp(name, "DGP")) return MGP_MODEL_DGP;
   if (!strcmp(name, "BRANSDICKE")) return MGP_MODEL_BRANSDICKE;
   if (!strcmp(name, "MBETA")) return MGP_MODEL_MBETA;
+  if (!strcmp(name, "FOFRNU")) return MGP_MODEL_FOFRNU;
   return -1;
 }
 
 void mgp_params_defaults(mgp_params *p, int model) {
   memset(p, 0, sizeof(*p));
   p->model = model;
-  p->scale_dependent = model == MGP_MODEL_FOFR || model == MGP_MODEL_MBETA;
+  p->scale_dependent = model == MGP_MODEL_FOFR || model == MGP_MODEL_MBETA || model == MGP_MODEL_FOFRNU;
   p->gadget_style = 1;
 }
 

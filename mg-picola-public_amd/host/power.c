@@ -49,6 +49,8 @@ static double table_lookup(const struct mgp_cosmo *c, double k) {
 /* PowerSpec (src/power.c:373-394) */
 double mgp_power_spec(const mgp_cosmo *c, double k) {
   const mgp_params *p = c->p;
+  /* with massive neutrinos the CDM + baryon spectrum of the CAMB tables (src/power.c:376-379) */
+  if (c->nu) return c->Norm * mgp_pk_cb(c, k);
   switch (p->WhichSpectrum) {
     case 0: {
       double t = p->WhichTransfer == 1 ? table_lookup(c, k) : transfer_eh(c, k);
@@ -118,7 +120,12 @@ static int read_table(struct mgp_cosmo *c, const char *fname, int normalise_T, c
 /* initialize_powerspectrum (src/power.c:226-257) */
 int mgp_init_power(struct mgp_cosmo *c, char *err, size_t errlen) {
   const mgp_params *p = c->p;
-  if (p->WhichSpectrum == 1 && read_table(c, p->FileWithInputSpectrum, 0, err, errlen)) return -1;
+  if (c->nu) {
+    /* read_and_spline_transfer_functions replaces the P(k) input (src/power.c:232-240) */
+    if (mgp_camb_read(c, p->nu_FilenameTransferInfofile, err, errlen)) return -1;
+  } else if (p->WhichSpectrum == 1 && read_table(c, p->FileWithInputSpectrum, 0, err, errlen)) {
+    return -1;
+  }
   if (p->WhichSpectrum == 0 && p->WhichTransfer == 1 && read_table(c, p->FileWithInputTransfer, 1, err, errlen))
     return -1;
   c->R8 = 8 * (3.085678e24 / p->UnitLength_in_cm);

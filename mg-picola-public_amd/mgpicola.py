@@ -21,7 +21,8 @@ MESH_DENSITY_X, MESH_DENSITY_K, MESH_PHI_K, MESH_FORCE_X, MESH_FORCE_Y, MESH_FOR
 MGPU_SYMBOLS = [
     "mgpu_n2_table_len", "mgpu_last_error", "mgpu_device_info", "mgpu_create", "mgpu_destroy", "mgpu_sync",
     "mgpu_set_particles", "mgpu_get_particles", "mgpu_set_lpt", "mgpu_get_lpt", "mgpu_get_disp",
-    "mgpu_pm_density", "mgpu_pm_deposit", "mgpu_pm_transform", "mgpu_power_spectrum", "mgpu_rsd_power_spectrum", "mgpu_fifth_force", "mgpu_forces", "mgpu_mesh_to_particles",
+    "mgpu_pm_density", "mgpu_pm_deposit", "mgpu_pm_transform", "mgpu_power_spectrum", "mgpu_rsd_power_spectrum",
+    "mgpu_set_neutrinos", "mgpu_neutrino_response", "mgpu_fifth_force", "mgpu_forces", "mgpu_mesh_to_particles",
     "mgpu_get_displacements", "mgpu_kick", "mgpu_drift", "mgpu_ic_displacement_fields", "mgpu_init_particles",
     "mgpu_lpt_fields", "mgpu_get_mesh", "mgpu_set_mesh", "mgpu_get_timers", "mgpu_enable_timers",
     "mgpu_kernel_name", "mgpu_enable_kernel_timing", "mgpu_kernel_timing", "mgpu_lpt_fields_slices",
@@ -232,6 +233,15 @@ class MGPU:
         self._ck(L.mgpu_rsd_power_spectrum(self.ctx, axis, vnorm, s1, s2, C.byref(r)), "rsd_power_spectrum")
         return tuple(bufs)
 
+    def set_neutrinos(self, cdmfac, fac):
+        """fac: a KSlice (norm num/den) of the neutrino factor, or None to turn the term off."""
+        L = lib()
+        L.mgpu_set_neutrinos.argtypes = [C.c_void_p, C.c_double, C.POINTER(KSlice)]
+        self._ck(L.mgpu_set_neutrinos(self.ctx, cdmfac, None if fac is None else C.byref(fac)), "set_neutrinos")
+
+    def neutrino_response(self):
+        self._ck(lib().mgpu_neutrino_response(self.ctx), "neutrino_response")
+
     def fifth_force(self, mg):
         self._ck(lib().mgpu_fifth_force(self.ctx, C.byref(mg)), "fifth_force")
 
@@ -305,7 +315,7 @@ class MGPU:
 # ---------------------------------------------------------------------------
 # host library (libmgpicola.so): paramfile interface, cosmology, the run
 # ---------------------------------------------------------------------------
-MODELS = {"FOFR": 0, "FOFR_LCDM": 1, "DGP": 2, "BRANSDICKE": 3, "MBETA": 4}
+MODELS = {"FOFR": 0, "FOFR_LCDM": 1, "DGP": 2, "BRANSDICKE": 3, "MBETA": 4, "FOFRNU": 5}
 G_D, G_dDdy, G_ddDddy, G_D2, G_dD2dy, G_ddD2ddy = range(6)
 FIELD_D, FIELD_dDdy, FIELD_ddDddy, FIELD_deltaD = range(4)
 
@@ -329,6 +339,8 @@ class HostParams(C.Structure):
                                      "UnitVelocity_in_cm_per_s", "UnitLength_in_cm", "UnitMass_in_g",
                                      "InputSpectrum_UnitLength_in_cm")]
         + [("Noutputs", C.c_int), ("out_z", C.c_double * 512), ("out_nsteps", C.c_int * 512)]
+        + [("nu_include_massive_neutrinos", C.c_int), ("nu_SumMassNuEV", C.c_double),
+           ("nu_FilenameTransferInfofile", C.c_char * 500), ("OmegaNu", C.c_double), ("OmegaCDM", C.c_double)]
     )
 
 
@@ -370,6 +382,12 @@ def hlib():
         L.mgp_growth_sd.restype = d
         for n in ("mgp_Sq", "mgp_Sphi"):
             getattr(L, n).argtypes = [vp, d, d, d]
+            getattr(L, n).restype = d
+        for n in ("mgp_T_nu", "mgp_T_cdm", "mgp_T_b", "mgp_T_tot", "mgp_T_cb", "mgp_munu_1lpt", "mgp_munu_2lpt"):
+            getattr(L, n).argtypes = [vp, d, d]
+            getattr(L, n).restype = d
+        for n in ("mgp_pk_total", "mgp_pk_cb"):
+            getattr(L, n).argtypes = [vp, d]
             getattr(L, n).restype = d
         for n in ("mgp_GeffoverG", "mgp_screening_potential", "mgp_screening_density", "mgp_mg_pofk_ratio"):
             getattr(L, n).argtypes = [vp, d, d]

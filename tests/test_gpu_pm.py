@@ -152,6 +152,55 @@ def test_rsd_multipoles_match_oracle(axis, mode):
     assert np.abs(Q2[m] - P2[m]).max() > 1e-3 * scale.max()
 
 
+def test_neutrino_term_closed_form():
+    """PtoMesh's neutrino term (src/auxPM.c:383-420): delta <- cdmfac delta +
+    fac(k) cdelta_IC on every stored mode but k = 0, with fac quadratic in ln k
+    (which the cubic Hermite slice reproduces exactly), compared with the closed
+    form, both inside mgpu_pm_density and applied on its own."""
+    n, ns, box = 32, 16, 64.0
+    g = M.MGPU(n, ns, box, OM, scale_dependent=1)
+    g.set_particles(lattice_particles(ns, box, amp=1.0, seed=9))
+    g.pm_density()
+    dk0 = g.get_mesh(M.MESH_DENSITY_K)
+    rng = np.random.default_rng(4)
+    shape = dk0.shape
+    g.set_mesh(M.MESH_CDELTA1, rng.normal(size=shape) + 1j * rng.normal(size=shape))
+    cd = g.get_mesh(M.MESH_CDELTA1)  # the FP32 values on the device
+    kf = 2 * np.pi / box
+    nk, y0 = 400, np.log(kf * 0.5)
+    dy = (np.log(kf * n) - y0) / (nk - 1)
+    yc, c0, c1, c2 = np.log(kf * 4), 0.7, -0.3, 0.05
+    y = y0 + dy * np.arange(nk)
+    tabs = [c0 + c1 * (y - yc) + c2 * (y - yc) ** 2, c1 + 2 * c2 * (y - yc), np.ones(nk), np.zeros(nk)]
+    fac = M.KSlice(y0, dy, nk, *[t.ctypes.data_as(M.C.POINTER(M.C.c_double)) for t in tabs], 1.0)
+    f = np.fft.fftfreq(n) * n
+    n2 = f[:, None, None] ** 2 + f[None, :, None] ** 2 + np.arange(shape[2])[None, None, :] ** 2
+    n2[0, 0, 0] = 1.0
+    yk = np.log(kf * np.sqrt(n2))
+    fk = c0 + c1 * (yk - yc) + c2 * (yk - yc) ** 2
+    fk[0, 0, 0] = 0.0
+    cdmfac = 0.953
+
+    def term(d):
+        r = cdmfac * d + fk * cd
+        r[0, 0, 0] = d[0, 0, 0]
+        return r
+
+    g.set_neutrinos(cdmfac, fac)
+    g.pm_density()
+    got = g.get_mesh(M.MESH_DENSITY_K)
+    assert rel_err(got, term(dk0)) < 1e-6
+    assert got[0, 0, 0] == dk0[0, 0, 0]
+    g.neutrino_response()
+    assert rel_err(g.get_mesh(M.MESH_DENSITY_K), term(got)) < 1e-6
+    g.set_neutrinos(1.0, None)
+    g.pm_density()
+    assert rel_err(g.get_mesh(M.MESH_DENSITY_K), dk0) == 0.0
+    with pytest.raises(M.MGPUError):
+        g.neutrino_response()
+    g.close()
+
+
 def test_deposit_is_deterministic(pos):
     g = M.MGPU(N, NS, BOX, OM)
     g.set_particles(pos)

@@ -153,3 +153,35 @@ def test_rsd_files_written_every_step_and_at_output(tmp_path):
         assert rows.shape[1] == 7 and rows.shape[0] > 3
         assert np.all(np.isfinite(rows)) and np.all(rows[:, 0] > 0)
         assert np.all(rows[:, 4:] >= 0)
+
+
+def test_fofrnu_run_writes_cdm_and_total_pofk(tmp_path):
+    """f(R) with massive neutrinos (FOFRNU, the reference's nu0.2 CAMB tables at
+    32^3 in a 512 Mpc/h box): the run writes the CDM and the total matter P(k)
+    at every step (src/auxPM.c:370-371, :422-426).  Without shot-noise
+    subtraction total/CDM lies between (1 - fnu)^2 and 1 and falls with k, as
+    the neutrinos stream out of the small scales."""
+    import test_host as H
+    outs = tmp_path / "outs.dat"
+    outs.write_text("0, 3\n")
+    hp, c = H.nu_cosmo(tmp_path, Nmesh=32, Nsample=16, pofk_nbins=16, pofk_compute_rsd_pofk=0,
+                       pofk_subtract_shotnoise=0, OutputRedshiftFile=str(outs))
+    c.close()
+    sim = M.Sim(hp)
+    sim.ic()
+    while not sim.finished:
+        sim.advance()
+    sim.close()
+    files = os.listdir(str(tmp_path))
+    cdm = sorted(f for f in files if f.startswith("pofk_") and f.endswith("_CDM.txt"))
+    tot = sorted(f for f in files if f.startswith("pofk_") and f.endswith("_total.txt"))
+    assert len(cdm) == len(tot) >= 3, files
+    assert [f[:-len("CDM.txt")] for f in cdm] == [f[:-len("total.txt")] for f in tot]
+    f2 = ((hp.Omega - hp.OmegaNu) / hp.Omega) ** 2
+    for a, b in zip(cdm, tot):
+        pc = np.loadtxt(os.path.join(str(tmp_path), a), ndmin=2)
+        pt = np.loadtxt(os.path.join(str(tmp_path), b), ndmin=2)
+        np.testing.assert_array_equal(pc[:, 0], pt[:, 0])
+        r = pt[:, 1] / pc[:, 1]
+        assert np.all(r > f2 * 0.99) and np.all(r < 1.01), (a, r)
+        assert r[-1] < r[0], (a, r)

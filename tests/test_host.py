@@ -337,3 +337,128 @@ def test_pofk_file_format(tmp_path):
     np.testing.assert_allclose(rows[:, 1], pk[keep], atol=6e-6)
     np.testing.assert_allclose(rows[:, 2], km[keep], atol=6e-6)
     np.testing.assert_allclose(rows[:, 3], pk[keep] * kb ** 3 / (2 * np.pi ** 2), rtol=1e-4, atol=6e-6)
+
+
+# ---------------------------------------------------------------- massive neutrinos (FOFRNU)
+CAMB = os.path.join(GOLDEN, "camb_nu0.2")
+CAMB_Z = ["0.000", "0.494", "1.059", "2.075", "4.393", "10.103", "20.098", "50.000"]
+
+
+def camb_info(tmp_path):
+    """An info file in the reference's format (directory and count, then file and z)
+    listing the fixture subset of the reference's nu0.2 CAMB tables."""
+    p = tmp_path / "transfer_info.txt"
+    p.write_text(f"{CAMB} {len(CAMB_Z)}\n" + "".join(f"lcdm_nu0.2_transfer_z{z}.dat {z}\n" for z in CAMB_Z))
+    return str(p)
+
+
+def nu_cosmo(tmp_path, **over):
+    vals = tag_values(os.path.join(GOLDEN, "paramfiles", "fofr_nu0.2.inp"))
+    vals["nu_FilenameTransferInfofile"] = camb_info(tmp_path)
+    vals["OutputRedshiftFile"] = os.path.join(GOLDEN, "files/output_redshifts.dat")
+    vals["OutputDir"] = str(tmp_path)
+    vals.update({k: str(v) for k, v in over.items()})
+    pf = tmp_path / "p.txt"
+    pf.write_text("".join(f"{k} {v}\n" for k, v in vals.items()))
+    hp = M.read_params(str(pf), model="FOFRNU", compute_pofk=True)
+    return hp, M.Cosmo(hp)
+
+
+def camb_rows(z):
+    return np.loadtxt(os.path.join(CAMB, f"lcdm_nu0.2_transfer_z{z}.dat"), skiprows=1)
+
+
+def test_fofrnu_paramfile_and_omeganu(in_golden, tmp_path):
+    """The reference's f(R) + massive-nu paramfile parses as FOFRNU (its tag set,
+    with the output list moved to a file that exists here); OmegaNu =
+    SumMassNuEV / (93.14 h^2), OmegaCDM = Omega - OmegaBaryon - OmegaNu."""
+    src = open(os.path.join("paramfiles", "fofr_nu0.2.inp")).read().splitlines()
+    out = os.path.join(GOLDEN, "files/output_redshifts.dat")
+    pf = tmp_path / "fofr_nu0.2.inp"
+    pf.write_text("\n".join(f"OutputRedshiftFile {out}" if l.split("%")[0].split()[:1] == ["OutputRedshiftFile"]
+                             else l for l in src) + "\n")
+    hp = M.read_params(str(pf), model="FOFRNU", compute_pofk=True)
+    assert hp.nu_include_massive_neutrinos == 1 and hp.nu_SumMassNuEV == 0.2
+    with pytest.raises(ValueError, match="nu_SumMassNuEV"):
+        M.read_params(os.path.join("paramfiles", "example_fofr_withscreening.txt"), model="FOFRNU",
+                      compute_pofk=False)
+    hp, c = nu_cosmo(tmp_path, Nmesh=16, Nsample=16)
+    h = hp.HubbleParam
+    assert hp.OmegaNu == pytest.approx(0.2 / (93.14 * h * h), rel=1e-14)
+    assert hp.OmegaCDM == pytest.approx(hp.Omega - hp.OmegaBaryon - hp.OmegaNu, rel=1e-14)
+    c.close()
+
+
+def test_camb_transfer_functions_at_the_table_nodes(tmp_path):
+    """The (ln k, z) splines pass through the CAMB values at every node; T_cb
+    weighs CDM and baryons by their densities."""
+    hp, c = nu_cosmo(tmp_path, Nmesh=16, Nsample=16)
+    for z in ("0.000", "4.393"):
+        r = camb_rows(z)
+        a = 1.0 / (1.0 + float(z))
+        for row in r[::17]:
+            k = row[0]
+            assert c.T_nu(k, a) == pytest.approx(row[5], rel=1e-10)
+            assert c.T_cdm(k, a) == pytest.approx(row[1], rel=1e-10)
+            assert c.T_b(k, a) == pytest.approx(row[2], rel=1e-10)
+            assert c.T_tot(k, a) == pytest.approx(row[6], rel=1e-10)
+            cb = (hp.OmegaBaryon * row[2] + hp.OmegaCDM * row[1]) / (hp.OmegaBaryon + hp.OmegaCDM)
+            assert c.T_cb(k, a) == pytest.approx(cb, rel=1e-10)
+    c.close()
+
+
+def test_munu_limits(tmp_path):
+    """mu_nu (1LPT) -> 1 on scales where neutrinos cluster like CDM and ->
+    (Omega - OmegaNu)/Omega where they free-stream; the 2LPT factor is the latter."""
+    hp, c = nu_cosmo(tmp_path, Nmesh=16, Nsample=16)
+    f = (hp.Omega - hp.OmegaNu) / hp.Omega
+    for a in (0.05, 0.5, 1.0):
+        assert c.munu_2lpt(a, 0.3) == f
+        assert c.munu_1lpt(a, 1e-4) == pytest.approx(1.0, abs=2e-3)
+        assert c.munu_1lpt(a, 5.0) == pytest.approx(f, abs=2e-3)
+        assert f < c.munu_1lpt(a, 0.1) < 1.0
+    c.close()
+
+
+def test_camb_total_power_matches_camb_matter_power(tmp_path):
+    """CAMB's own z = 0 matter P(k) over P_total(k) = 2 pi^2 T_tot^2 (kh) (kh/0.05)^(ns-1) h^3
+    is the primordial amplitude (~2.2e-9) times at most a slow power of k (the CAMB
+    run's tilt need not be the paramfile's) over three decades: a wrong column or a
+    wrong k unit would leave structure in the ratio."""
+    hp, c = nu_cosmo(tmp_path, Nmesh=16, Nsample=16)
+    mp = np.loadtxt(os.path.join(CAMB, "lcdm_nu0.2_matterpower_z0.000.dat"), comments="#")
+    sel = (mp[:, 0] > 1e-3) & (mp[:, 0] < 1.0)
+    ratio = mp[sel, 1] / np.array([c.pk_total(k) for k in mp[sel, 0]])
+    lk, lr = np.log(mp[sel, 0]), np.log(ratio)
+    slope, icpt = np.polyfit(lk, lr, 1)
+    assert abs(slope) < 0.01 and 1.5e-9 < np.exp(icpt + slope * np.log(0.05)) < 3e-9, (slope, icpt)
+    assert np.std(lr - (slope * lk + icpt)) < 5e-3
+    # and P_cb = P_total (T_cb / T_tot)^2
+    for k in (0.01, 0.1, 1.0):
+        assert c.pk_cb(k) == pytest.approx(c.pk_total(k) * (c.T_cb(k, 1.0) / c.T_tot(k, 1.0)) ** 2, rel=1e-13)
+    c.close()
+
+
+def test_fofrnu_growth_matches_ode(tmp_path):
+    """The scale-dependent D(k, a) with neutrinos equals an independent ODE solution
+    with mu = mu_fofr(k, a) mu_nu(k, a) (src/cosmo.c:206-236)."""
+    hp, c = nu_cosmo(tmp_path, Nmesh=16, Nsample=16)
+    om = hp.Omega
+    a_eval = np.array([0.1, 0.4, 1.0])
+
+    def rhs(x, y, k):
+        a = np.exp(x)
+        H = np.sqrt(om / a ** 3 + 1 - om)
+        dH = -1.5 * om / a ** 4 / H
+        k2 = (k * 2997.92458) ** 2
+        mu = 1 + k2 / (3 * (k2 + a * a * c.mass2(a)))
+        return [y[1], -(2 + a * dH / H) * y[1] + 1.5 * om * mu * c.munu_1lpt(a, k) / (a ** 3 * H * H) * y[0]]
+
+    for k in (0.05, 0.5):
+        a0 = 1 / (1 + 200.0)
+        s = integrate.solve_ivp(rhs, (np.log(a0), 0.0), [1, 1], t_eval=np.log(a_eval), args=(k,), method="DOP853",
+                                rtol=1e-10, atol=1e-12)
+        d = s.y[0] / s.y[0][-1]
+        for i, a in enumerate(a_eval):
+            assert c.growth_sd(0, k, a) == pytest.approx(d[i], rel=3e-4), (k, a)
+    c.close()
