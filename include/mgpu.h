@@ -220,6 +220,36 @@ typedef struct {
 int mgpu_get_timers(mgpu_ctx *ctx, mgpu_timers *t, int reset);
 int mgpu_enable_timers(mgpu_ctx *ctx, int on);
 
+/* ---- scale-dependent growth tables; replaces calculate_scale_dependent_growth_factor
+ *      (src/cosmo.c:798-981): the ODEs ode_growth_D / ode_growth_DLCDM (src/cosmo.c:206-261)
+ *      with the hooks GeffoverG, second_order_kernel(k, k/sqrt2, k/sqrt2, 0, a) and
+ *      GeffoverG_neutrino_1LPT/2LPT (src/user_defined_functions.h:618-680, :800-904),
+ *      one k per GPU thread, FP64, Dormand-Prince 5(4) at tolerance 1e-11 ---- */
+typedef struct {
+  int nx, ny;                       /* knots in x and in y */
+  const double *x, *y;              /* increasing knots */
+  const double *z, *zx, *zy, *zxy;  /* value, d/dx, d/dy, d2/dxdy at (x_i, y_j), index j*nx + i */
+} mgpu_spline2d;
+
+typedef struct {
+  int model;      /* MGPU_LCDM, MGPU_FOFR or MGPU_MBETA (the scale-dependent models) */
+  int mg_active;  /* modified_gravity_active and not use_lcdm_growth_factors: GeffoverG and gamma2 on */
+  int lcdm;       /* 1: ode_growth_DLCDM (mu = 1, no gamma2) and its source terms */
+  double omega;
+  double fofr0, nfofr;                      /* f(R) Hu-Sawicki */
+  double assb_symm, beta_symm, range_symm;  /* m(a), beta(a) of the symmetron-like model */
+  double omega_nu, omega_b, omega_cdm;      /* omega_nu > 1e-6 turns the neutrino factors on */
+  mgpu_spline2d t_nu, t_cdm, t_b;           /* CAMB transfer functions in (ln k, z) */
+} mgpu_growth_model;
+
+/* Integrates from x0 = ln a_ini for every ln k of lnk[0..nk) and samples the
+ * solution at x0 + i dx, i < na.  out: 6 nk na doubles, index (f nk + j) na + i,
+ * fields f: D1, dD1/dy (= dD1/dx Q/a), 1.5 Om a mu mu_nu1 D1, D2, dD2/dy,
+ * 1.5 Om a mu mu_nu2 (D2 - D1^2) (mu = mu_nu = 1 for lcdm).  Synchronous; uses
+ * the current device.  MGPU_E_INVALID if an ODE fails (NaN or 10^6 steps). */
+int mgpu_growth_tables(const mgpu_growth_model *m, double x0, double dx, int na, const double *lnk, int nk,
+                       double *out);
+
 /* ---- per-kernel HIP-event timing (no counterpart in the reference) ----
  * When enabled, every launch of the listed kernel classes is bracketed by HIP
  * events on the context's stream; mgpu_kernel_timing synchronises and reports

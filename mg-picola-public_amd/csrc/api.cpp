@@ -376,6 +376,33 @@ int mgpu_neutrino_response(mgpu_ctx *ctx) {
   return ic_nu_apply(ctx);
 }
 
+int mgpu_growth_tables(const mgpu_growth_model *m, double x0, double dx, int na, const double *lnk, int nk,
+                       double *out) {
+  if (!m || !lnk || !out || na < 1 || nk < 1 || !(dx > 0) || !(m->omega > 0))
+    return mgpu_fail(nullptr, MGPU_E_INVALID, "growth tables: bad argument");
+  if (m->model != MGPU_LCDM && m->model != MGPU_FOFR && m->model != MGPU_MBETA)
+    return mgpu_fail(nullptr, MGPU_E_INVALID, "growth tables: scale-dependent models are LCDM, f(R) and m/beta");
+  if (m->mg_active && m->model == MGPU_FOFR && !(m->fofr0 > 0))
+    return mgpu_fail(nullptr, MGPU_E_INVALID, "growth tables: fofr0 must be positive");
+  if (m->mg_active && m->model == MGPU_MBETA && !(m->range_symm > 0))
+    return mgpu_fail(nullptr, MGPU_E_INVALID, "growth tables: range_symm must be positive");
+  if (m->omega_nu > 1e-6) {
+    const mgpu_spline2d *t[3] = {&m->t_nu, &m->t_cdm, &m->t_b};
+    for (const mgpu_spline2d *s : t)
+      if (s->nx < 2 || s->ny < 2 || !s->x || !s->y || !s->z || !s->zx || !s->zy || !s->zxy)
+        return mgpu_fail(nullptr, MGPU_E_INVALID, "growth tables: neutrino factors need the three transfer splines");
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    return mgpu_fail(nullptr, MGPU_E_NODEVICE, "no HIP device visible");
+  }
+  const int rc = growth_tables(m, x0, dx, na, lnk, nk, out);
+  if (rc == MGPU_E_INVALID) return mgpu_fail(nullptr, rc, "growth tables: an ODE failed (NaN or 10^6 steps)");
+  if (rc) return mgpu_fail(nullptr, rc, "growth tables: HIP allocation or copy failed");
+  return MGPU_OK;
+}
+
 int mgpu_power_spectrum(mgpu_ctx *ctx, mgpu_pofk *pk) {
   if (check_ctx(ctx) || !pk) return MGPU_E_INVALID;
   if (!ctx->have_density) return mgpu_fail(ctx, MGPU_E_STATE, "no density(k): call mgpu_pm_density first");

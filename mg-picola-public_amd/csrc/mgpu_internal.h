@@ -121,6 +121,8 @@ struct KScope {
 
 // ---------------------------------------------------------------- helpers
 int mgpu_fail(mgpu_ctx *ctx, int code, const std::string &msg);
+int growth_tables(const mgpu_growth_model *m, double x0, double dx, int na, const double *lnk, int nk,
+                  double *out);
 #define MGPU_HIP(ctx, call)                                                               \
   do {                                                                                    \
     hipError_t e_ = (call);                                                               \

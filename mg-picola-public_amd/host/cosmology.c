@@ -46,6 +46,79 @@ static int growth_rhs(double x, const double *D, double *dD, void *vp) {
 #define ODE_ABS 1e-11
 #define ODE_REL 1e-11
 
+static mgpu_spline2d spline_view(const mgp_spline2d *s) {
+  mgpu_spline2d v = {s->nx, s->ny, s->x, s->y, s->z, s->zx, s->zy, s->zxy};
+  return v;
+}
+
+/* The scale-dependent table of every k on the GPU (mgpu_growth_tables): the same ODEs,
+ * integrator and tolerances as the host loop below; z[f] gets nk * npts values, k-major. */
+static int growth_table_gpu(const struct mgp_cosmo *c, int lcdm, double xini, double dx, int npts,
+                            const double *logk, int nk, double *z[6]) {
+  const mgp_params *p = c->p;
+  mgpu_growth_model m;
+  memset(&m, 0, sizeof m);
+  m.model = c->fofr ? MGPU_FOFR : c->mbeta ? MGPU_MBETA : MGPU_LCDM;
+  m.mg_active = p->modified_gravity_active && !p->use_lcdm_growth_factors;
+  m.lcdm = lcdm;
+  m.omega = p->Omega;
+  m.fofr0 = p->fofr0;
+  m.nfofr = p->nfofr;
+  m.assb_symm = p->assb_symm;
+  m.beta_symm = p->beta_symm;
+  m.range_symm = p->range_symm;
+  if (c->nu) {
+    m.omega_nu = p->OmegaNu;
+    m.omega_b = p->OmegaBaryon;
+    m.omega_cdm = p->OmegaCDM;
+    m.t_nu = spline_view(&c->camb[0]);
+    m.t_cdm = spline_view(&c->camb[1]);
+    m.t_b = spline_view(&c->camb[2]);
+  }
+  const size_t plane = (size_t)nk * npts;
+  double *out = malloc(sizeof(double) * 6 * plane);
+  if (!out) return -1;
+  int rc = mgpu_growth_tables(&m, xini, dx, npts, logk, nk, out);
+  if (!rc)
+    for (int f = 0; f < 6; f++) memcpy(z[f], out + f * plane, sizeof(double) * plane);
+  free(out);
+  return rc;
+}
+
+/* The same table with the host's ODE solver, one OpenMP task per k.  lcdm: the
+ * ode_growth_DLCDM equations and source terms without mu and the neutrino factors. */
+static int growth_table_host(const struct mgp_cosmo *c, int lcdm, double xini, double dx, int npts,
+                             const double *logk, int nk, double *z[6]) {
+  const mgp_params *p = c->p;
+  int failed = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(| : failed)
+  for (int j = 0; j < nk; j++) {
+    const double know = exp(logk[j]);
+    growth_par g = {c, know, lcdm};
+    double y[4] = {1.0, 1.0, -3.0 / 7.0, -6.0 / 7.0};
+    double xs = xini, h = 1e-4;
+    for (int i = 0; i < npts; i++) {
+      const double xn = xini + i * dx, a = exp(xn);
+      if (i > 0 && mgp_ode_integrate(growth_rhs, &g, 4, &xs, xn, y, &h, ODE_ABS, ODE_REL)) failed = 1;
+      const double Q = mgp_Qfactor(c, a), Om = p->Omega;
+      double f1 = 1.0, f2 = 1.0;
+      if (!lcdm) {
+        const double mu = mgp_GeffoverG(c, a, know);
+        f1 = mu * mgp_munu_1lpt(c, a, know);
+        f2 = mu * mgp_munu_2lpt(c, a, know);
+      }
+      const size_t o = (size_t)j * npts + i;
+      z[0][o] = y[0];
+      z[1][o] = y[1] * Q / a;
+      z[2][o] = 1.5 * f1 * Om * a * y[0];
+      z[3][o] = y[2];
+      z[4][o] = y[3] * Q / a;
+      z[5][o] = 1.5 * f2 * Om * a * (y[2] - pow2(y[0]));
+    }
+  }
+  return failed ? -1 : 0;
+}
+
 /* solve_for_growth_factors (src/cosmo.c:409-643) and, for scale-dependent
  * builds, calculate_scale_dependent_growth_factor (src/cosmo.c:798-981) */
 int mgp_solve_growth(struct mgp_cosmo *c) {
@@ -97,60 +170,22 @@ int mgp_solve_growth(struct mgp_cosmo *c) {
     c->sd_dlnk = log(kmax / kmin) / (double)(nk - 1);
     double *z[6];
     for (int f = 0; f < 6; f++) z[f] = malloc(sizeof(double) * (size_t)npts * nk);
-    int failed = 0;
-    if (c->nu) {
-      /* with massive neutrinos LCDM growth is scale-dependent too: its table, mu = 1,
-       * without the neutrino factor in the source terms (src/cosmo.c:829-889) */
-#pragma omp parallel for schedule(dynamic, 4) reduction(| : failed)
-      for (int j = 0; j < nk; j++) {
-        const double know = exp(logk[j]);
-        growth_par g = {c, know, 1};
-        double y[4] = {1.0, 1.0, -3.0 / 7.0, -6.0 / 7.0};
-        double xs = xini, h = 1e-4;
-        for (int i = 0; i < npts; i++) {
-          const double xn = xini + i * dx, a = exp(xn);
-          if (i > 0 && mgp_ode_integrate(growth_rhs, &g, 4, &xs, xn, y, &h, ODE_ABS, ODE_REL)) failed = 1;
-          const double Q = mgp_Qfactor(c, a), Om = p->Omega;
-          const size_t o = (size_t)j * npts + i;
-          z[0][o] = y[0];
-          z[1][o] = y[1] * Q / a;
-          z[2][o] = 1.5 * Om * a * y[0];
-          z[3][o] = y[2];
-          z[4][o] = y[3] * Q / a;
-          z[5][o] = 1.5 * Om * a * (y[2] - pow2(y[0]));
-        }
-      }
-      if (failed) return -1;
-      for (int f = 0; f < 6; f++) mgp_spline2d_init(&c->sd_lcdm[f], x, logk, z[f], npts, nk);
-      c->have_sd_lcdm = 1;
+    /* with massive neutrinos LCDM growth is scale-dependent too: its table first
+     * (src/cosmo.c:829-889), then the model's */
+    int rc = 0;
+    for (int lcdm = c->nu ? 1 : 0; lcdm >= 0 && !rc; lcdm--) {
+      rc = p->growth_on_gpu ? growth_table_gpu(c, lcdm, xini, dx, npts, logk, nk, z)
+                            : growth_table_host(c, lcdm, xini, dx, npts, logk, nk, z);
+      for (int f = 0; f < 6 && !rc; f++) mgp_spline2d_init(lcdm ? &c->sd_lcdm[f] : &c->sd[f], x, logk, z[f], npts, nk);
     }
-#pragma omp parallel for schedule(dynamic, 4) reduction(| : failed)
-    for (int j = 0; j < nk; j++) {
-      const double know = exp(logk[j]);
-      growth_par g = {c, know, 0};
-      double y[4] = {1.0, 1.0, -3.0 / 7.0, -6.0 / 7.0};
-      double xs = xini, h = 1e-4;
-      for (int i = 0; i < npts; i++) {
-        const double xn = xini + i * dx, a = exp(xn);
-        if (i > 0 && mgp_ode_integrate(growth_rhs, &g, 4, &xs, xn, y, &h, ODE_ABS, ODE_REL)) failed = 1;
-        const double mu = mgp_GeffoverG(c, a, know), Q = mgp_Qfactor(c, a), Om = p->Omega;
-        const double nu1 = mgp_munu_1lpt(c, a, know), nu2 = mgp_munu_2lpt(c, a, know);
-        const size_t o = (size_t)j * npts + i;
-        z[0][o] = y[0];
-        z[1][o] = y[1] * Q / a;
-        z[2][o] = 1.5 * mu * nu1 * Om * a * y[0];
-        z[3][o] = y[2];
-        z[4][o] = y[3] * Q / a;
-        z[5][o] = 1.5 * mu * nu2 * Om * a * (y[2] - pow2(y[0]));
-      }
-    }
-    if (failed) return -1;
-    for (int f = 0; f < 6; f++) {
-      mgp_spline2d_init(&c->sd[f], x, logk, z[f], npts, nk);
-      free(z[f]);
-    }
-    c->have_sd = 1;
+    for (int f = 0; f < 6; f++) free(z[f]);
     free(logk);
+    if (rc) {
+      free(x);
+      return -1;
+    }
+    c->have_sd_lcdm = c->nu;
+    c->have_sd = 1;
   }
   free(x);
   return 0;

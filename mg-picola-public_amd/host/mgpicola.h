@@ -69,6 +69,9 @@ typedef struct {
   double nu_SumMassNuEV;
   char nu_FilenameTransferInfofile[500];
   double OmegaNu, OmegaCDM; /* derived: SumMassNuEV/(93.14 h^2), Omega - OmegaBaryon - OmegaNu */
+  /* host option (not a paramfile tag): build the scale-dependent growth tables on the
+   * GPU (mgpu_growth_tables, default) or with the host's own ODE solver */
+  int growth_on_gpu;
 } mgp_params;
 
 /* ---- paramfile (src/read_param.c) ---- */

@@ -22,7 +22,7 @@ MGPU_SYMBOLS = [
     "mgpu_n2_table_len", "mgpu_last_error", "mgpu_device_info", "mgpu_create", "mgpu_destroy", "mgpu_sync",
     "mgpu_set_particles", "mgpu_get_particles", "mgpu_set_lpt", "mgpu_get_lpt", "mgpu_get_disp",
     "mgpu_pm_density", "mgpu_pm_deposit", "mgpu_pm_transform", "mgpu_power_spectrum", "mgpu_rsd_power_spectrum",
-    "mgpu_set_neutrinos", "mgpu_neutrino_response", "mgpu_fifth_force", "mgpu_forces", "mgpu_mesh_to_particles",
+    "mgpu_set_neutrinos", "mgpu_neutrino_response", "mgpu_growth_tables", "mgpu_fifth_force", "mgpu_forces", "mgpu_mesh_to_particles",
     "mgpu_get_displacements", "mgpu_kick", "mgpu_drift", "mgpu_ic_displacement_fields", "mgpu_init_particles",
     "mgpu_lpt_fields", "mgpu_get_mesh", "mgpu_set_mesh", "mgpu_get_timers", "mgpu_enable_timers",
     "mgpu_kernel_name", "mgpu_enable_kernel_timing", "mgpu_kernel_timing", "mgpu_lpt_fields_slices",
@@ -340,7 +340,8 @@ class HostParams(C.Structure):
                                      "InputSpectrum_UnitLength_in_cm")]
         + [("Noutputs", C.c_int), ("out_z", C.c_double * 512), ("out_nsteps", C.c_int * 512)]
         + [("nu_include_massive_neutrinos", C.c_int), ("nu_SumMassNuEV", C.c_double),
-           ("nu_FilenameTransferInfofile", C.c_char * 500), ("OmegaNu", C.c_double), ("OmegaCDM", C.c_double)]
+           ("nu_FilenameTransferInfofile", C.c_char * 500), ("OmegaNu", C.c_double), ("OmegaCDM", C.c_double),
+           ("growth_on_gpu", C.c_int)]
     )
 
 
@@ -586,6 +587,32 @@ class KSlice(C.Structure):
     _fields_ = [("lnk0", C.c_double), ("dlnk", C.c_double), ("nk", C.c_int),
                 ("num", C.POINTER(C.c_double)), ("dnum", C.POINTER(C.c_double)),
                 ("den", C.POINTER(C.c_double)), ("dden", C.POINTER(C.c_double)), ("norm", C.c_double)]
+
+
+class Spline2D(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int)] + [(n, C.POINTER(C.c_double)) for n in
+                                                     ("x", "y", "z", "zx", "zy", "zxy")]
+
+
+class GrowthModel(C.Structure):
+    _fields_ = [("model", C.c_int), ("mg_active", C.c_int), ("lcdm", C.c_int), ("omega", C.c_double),
+                ("fofr0", C.c_double), ("nfofr", C.c_double), ("assb_symm", C.c_double), ("beta_symm", C.c_double),
+                ("range_symm", C.c_double), ("omega_nu", C.c_double), ("omega_b", C.c_double),
+                ("omega_cdm", C.c_double), ("t_nu", Spline2D), ("t_cdm", Spline2D), ("t_b", Spline2D)]
+
+
+def growth_tables(model, x0, dx, na, lnk):
+    """mgpu_growth_tables: (6, nk, na) array of D1, dD1/dy, 1LPT source, D2, dD2/dy, 2LPT source."""
+    L = lib()
+    L.mgpu_growth_tables.argtypes = [C.POINTER(GrowthModel), C.c_double, C.c_double, C.c_int,
+                                     C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double)]
+    lnk = np.ascontiguousarray(lnk, dtype=np.float64)
+    out = np.zeros((6, lnk.size, na))
+    rc = L.mgpu_growth_tables(C.byref(model), x0, dx, na, lnk.ctypes.data_as(C.POINTER(C.c_double)), lnk.size,
+                              out.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc:
+        raise MGPUError(f"mgpu_growth_tables failed ({rc}): {L.mgpu_last_error(None).decode()}")
+    return out
 
 
 def lpt_kslices(cosmo, field, A, AFF):

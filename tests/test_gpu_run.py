@@ -167,6 +167,7 @@ def test_fofrnu_run_writes_cdm_and_total_pofk(tmp_path):
     hp, c = H.nu_cosmo(tmp_path, Nmesh=32, Nsample=16, pofk_nbins=16, pofk_compute_rsd_pofk=0,
                        pofk_subtract_shotnoise=0, OutputRedshiftFile=str(outs))
     c.close()
+    hp.growth_on_gpu = 1
     sim = M.Sim(hp)
     sim.ic()
     while not sim.finished:

@@ -142,6 +142,7 @@ def cosmo_for(tmp_path, model="FOFR", **over):
     pf = tmp_path / "p.txt"
     pf.write_text("".join(f"{k} {v}\n" for k, v in vals.items()))
     hp = M.read_params(str(pf), model=model, compute_pofk=False)
+    hp.growth_on_gpu = 0  # the host's own ODE solver: these tests run without a GPU
     return hp, M.Cosmo(hp)
 
 
@@ -361,6 +362,7 @@ def nu_cosmo(tmp_path, **over):
     pf = tmp_path / "p.txt"
     pf.write_text("".join(f"{k} {v}\n" for k, v in vals.items()))
     hp = M.read_params(str(pf), model="FOFRNU", compute_pofk=True)
+    hp.growth_on_gpu = 0  # the host's own ODE solver: these tests run without a GPU
     return hp, M.Cosmo(hp)
 
 
