@@ -250,6 +250,32 @@ typedef struct {
 int mgpu_growth_tables(const mgpu_growth_model *m, double x0, double dx, int na, const double *lnk, int nk,
                        double *out);
 
+/* ---- learned Green's-function multiplier (no counterpart in the reference: it
+ *      fills the grid_corr slot that Forces leaves at 1, src/auxPM.c:488-498) ----
+ * A 4 -> 64 -> 64 -> 64 -> 1 GELU network maps x = (|k|/k_Nyquist, a, log10 f_R0,
+ * s) to ln M; s is the rms over the mesh of ln(1 + delta) (floored at ln 0.01)
+ * after the deposit; the table takes |k|/k_Nyquist clamped at 1.  Parameters (MGPU_MLP_NPARAMS floats, FP32 master copy):
+ * W1[64][4], b1[64], W2[64][64], b2[64], W3[64][64], b3[64], W4[64], b4; W[out][in].
+ * The forward pass rounds weights to BF16 and runs the 64x64 layers on MFMA with
+ * BF16 activations and FP32 accumulation.  Sample arrays are host memory: x n*4,
+ * targets t and weights w n floats; the loss is sum w (y - t)^2 / sum w. */
+#define MGPU_MLP_NPARAMS 8705
+typedef struct mgpu_mlp mgpu_mlp;
+int mgpu_mlp_create(unsigned long long seed, mgpu_mlp **out); /* W ~ N(0, 1/fan_in), output layer 0: M = 1 */
+void mgpu_mlp_destroy(mgpu_mlp *m);
+int mgpu_mlp_get_params(mgpu_mlp *m, float *params);
+int mgpu_mlp_set_params(mgpu_mlp *m, const float *params); /* also resets the Adam state */
+int mgpu_mlp_forward(mgpu_mlp *m, const float *x, int n, float *y);
+int mgpu_mlp_gradient(mgpu_mlp *m, const float *x, const float *t, const float *w, int n, float *grad, double *loss);
+/* iters full-batch Adam steps (beta 0.9, 0.999, eps 1e-8); loss of the last evaluation */
+int mgpu_mlp_train(mgpu_mlp *m, const float *x, const float *t, const float *w, int n, int iters, float lr,
+                   double *loss);
+/* From the next deposit on: compute s after every deposit and, with m != NULL,
+ * multiply the force Green's function by M(|n|^2; a, theta, s).  a < 0 turns
+ * both off.  The context keeps m (not owned) until it is replaced or turned off. */
+int mgpu_set_learned_multiplier(mgpu_ctx *ctx, mgpu_mlp *m, double a, double theta);
+int mgpu_density_feature(mgpu_ctx *ctx, double *s); /* s of the last deposit */
+
 /* ---- per-kernel HIP-event timing (no counterpart in the reference) ----
  * When enabled, every launch of the listed kernel classes is bracketed by HIP
  * events on the context's stream; mgpu_kernel_timing synchronises and reports

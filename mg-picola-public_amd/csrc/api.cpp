@@ -271,7 +271,7 @@ void mgpu_destroy(mgpu_ctx *ctx) {
                   ctx->cdelta[1], ctx->partial,  ctx->dsum,      ctx->pk,        ctx->n2tab[0],
                   ctx->n2tab[1], ctx->n2tabf[0], ctx->n2tabf[1], ctx->lattice_slot, ctx->slice_buf,
                   ctx->dep_scratch, ctx->winv,    ctx->winvf,     ctx->pk_bin,    ctx->rsd_crd,
-                  ctx->nu_slice, ctx->nu_fac};
+                  ctx->nu_slice, ctx->nu_fac,  ctx->feat_part, ctx->feat,     ctx->green_mult};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (ctx->slice_host) (void)hipHostFree(ctx->slice_host);
@@ -400,6 +400,69 @@ int mgpu_growth_tables(const mgpu_growth_model *m, double x0, double dx, int na,
   const int rc = growth_tables(m, x0, dx, na, lnk, nk, out);
   if (rc == MGPU_E_INVALID) return mgpu_fail(nullptr, rc, "growth tables: an ODE failed (NaN or 10^6 steps)");
   if (rc) return mgpu_fail(nullptr, rc, "growth tables: HIP allocation or copy failed");
+  return MGPU_OK;
+}
+
+int mgpu_mlp_create(unsigned long long seed, mgpu_mlp **out) {
+  if (!out) return mgpu_fail(nullptr, MGPU_E_INVALID, "null argument");
+  *out = nullptr;
+  return mlp_create(seed, out);
+}
+
+void mgpu_mlp_destroy(mgpu_mlp *m) { mlp_destroy(m); }
+
+int mgpu_mlp_get_params(mgpu_mlp *m, float *params) {
+  if (!m || !params) return mgpu_fail(nullptr, MGPU_E_INVALID, "null argument");
+  return mlp_params(m, params, nullptr);
+}
+
+int mgpu_mlp_set_params(mgpu_mlp *m, const float *params) {
+  if (!m || !params) return mgpu_fail(nullptr, MGPU_E_INVALID, "null argument");
+  return mlp_params(m, nullptr, params);
+}
+
+int mgpu_mlp_forward(mgpu_mlp *m, const float *x, int n, float *y) {
+  if (!m || !x || !y || n < 1) return mgpu_fail(nullptr, MGPU_E_INVALID, "MLP forward: bad argument");
+  return mlp_forward(m, x, n, y);
+}
+
+int mgpu_mlp_gradient(mgpu_mlp *m, const float *x, const float *t, const float *w, int n, float *grad,
+                      double *loss) {
+  if (!m || !x || !t || !w || n < 1) return mgpu_fail(nullptr, MGPU_E_INVALID, "MLP gradient: bad argument");
+  return mlp_gradient(m, x, t, w, n, grad, loss);
+}
+
+int mgpu_mlp_train(mgpu_mlp *m, const float *x, const float *t, const float *w, int n, int iters, float lr,
+                   double *loss) {
+  if (!m || !x || !t || !w || n < 1 || iters < 0 || !(lr > 0))
+    return mgpu_fail(nullptr, MGPU_E_INVALID, "MLP training: bad argument");
+  return mlp_train(m, x, t, w, n, iters, lr, loss);
+}
+
+int mgpu_set_learned_multiplier(mgpu_ctx *ctx, mgpu_mlp *m, double a, double theta) {
+  if (check_ctx(ctx)) return MGPU_E_INVALID;
+  if (a < 0) {
+    ctx->learn_on = false;
+    ctx->mlp = nullptr;
+    return MGPU_OK;
+  }
+  int rc;
+  if (!ctx->feat && ((rc = alloc(ctx, (void **)&ctx->feat_part, learned_feature_blocks() * sizeof(double), "feature")) ||
+                     (rc = alloc(ctx, (void **)&ctx->feat, sizeof(double), "feature")) ||
+                     (rc = alloc(ctx, (void **)&ctx->green_mult, ctx->n2len * sizeof(float), "multiplier table"))))
+    return rc;
+  ctx->learn_on = true;
+  ctx->mlp = m;
+  ctx->learn_a = a;
+  ctx->learn_theta = theta;
+  return MGPU_OK;
+}
+
+int mgpu_density_feature(mgpu_ctx *ctx, double *s) {
+  if (check_ctx(ctx) || !s) return MGPU_E_INVALID;
+  if (!ctx->feat) return mgpu_fail(ctx, MGPU_E_STATE, "no density feature: call mgpu_set_learned_multiplier first");
+  MGPU_HIP(ctx, hipMemcpyAsync(s, ctx->feat, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return MGPU_OK;
 }
 

@@ -35,6 +35,7 @@ struct GreenArgs {
   float nf;     // GREEN_LAPLACE: 1.5 Omega / a (Box / (2 pi c/H0))^2 / N^3; GREEN_SMOOTH: 1 / N^3
   float kfac;   // GREEN_SMOOTH: 2 pi R / Box
   int filter;   // GREEN_SMOOTH: MGPU_FILTER_*
+  const float *mult = nullptr;  // GREEN_FORCE: learned multiplier per n^2, or none
 };
 
 __device__ __forceinline__ float window(int filter, float kR) {
@@ -54,7 +55,10 @@ __device__ __forceinline__ void green_mul(float2 S, int nx, int n2, bool valid, 
                                           float2 &o2) {
   float f = 0.f;
   if (valid) {
-    if (MODE == GREEN_FORCE) f = n2 > 0 ? -ga.cnorm / (float)n2 : 0.f;
+    if (MODE == GREEN_FORCE) {
+      f = n2 > 0 ? -ga.cnorm / (float)n2 : 0.f;
+      if (ga.mult) f *= ga.mult[n2];
+    }
     if (MODE == GREEN_LAPLACE) f = n2 > 0 ? -ga.nf / (float)n2 : 0.f;
     if (MODE == GREEN_SMOOTH) f = window(ga.filter, sqrtf((float)n2) * ga.kfac) * ga.nf;
   }
@@ -551,7 +555,7 @@ struct EpiF {
 int fused_green_x(mgpu_ctx *ctx, int mode, const float2 *a, const float2 *b, float2 *o0, float2 *o1, double p0,
                   double p1, int filter) {
   GreenArgs ga{0.f, 0.f, 0.f, filter};
-  if (mode == GREEN_FORCE) ga.cnorm = (float)p0;
+  if (mode == GREEN_FORCE) ga.cnorm = (float)p0, ga.mult = ctx->mlp ? ctx->green_mult : nullptr;
   else ga.nf = (float)p0;
   ga.kfac = (float)p1;
   const double modes = (double)ctx->N * ctx->N * (ctx->N / 2 + 1);
@@ -579,7 +583,7 @@ int fused_x_pair(mgpu_ctx *ctx, int fwd, int mode, const float2 *in, float2 *kou
   a.o0 = o0;
   a.o1 = o1;
   a.ga = GreenArgs{0.f, 0.f, 0.f, filter};
-  if (mode == GREEN_FORCE) a.ga.cnorm = (float)p0;
+  if (mode == GREEN_FORCE) a.ga.cnorm = (float)p0, a.ga.mult = ctx->mlp ? ctx->green_mult : nullptr;
   else a.ga.nf = (float)p0;
   a.ga.kfac = (float)p1;
   a.coupling = (float)coupling;

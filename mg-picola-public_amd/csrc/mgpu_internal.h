@@ -69,6 +69,14 @@ struct mgpu_ctx {
   double *nu_slice = nullptr;  // the factor's ln k slice (4 ny doubles)
   size_t nu_slice_len = 0;
   double *nu_fac = nullptr;    // factor per |n|^2 (n2len doubles)
+  // learned Green's-function multiplier: while learn_on every deposit is followed by
+  // the density feature (feat) and, with a network set, the table M(|n|^2) that the
+  // force pass applies
+  bool learn_on = false;
+  mgpu_mlp *mlp = nullptr;
+  double learn_a = 0.0, learn_theta = 0.0;
+  double *feat_part = nullptr, *feat = nullptr;
+  float *green_mult = nullptr;
   std::vector<double> pk_geom;  // mode-weighted |n| sums and mode counts per bin (2 * nbins), fixed per context
   double *n2tab[2] = {nullptr, nullptr};  // per-|n|^2 device tables
   float *n2tabf[2] = {nullptr, nullptr};  // FP32 copies of n2tab (lattice fold)
@@ -123,6 +131,15 @@ struct KScope {
 int mgpu_fail(mgpu_ctx *ctx, int code, const std::string &msg);
 int growth_tables(const mgpu_growth_model *m, double x0, double dx, int na, const double *lnk, int nk,
                   double *out);
+// learned Green's-function multiplier (mlp.hip)
+int mlp_create(unsigned long long seed, mgpu_mlp **out);
+void mlp_destroy(mgpu_mlp *m);
+int mlp_params(mgpu_mlp *m, float *get, const float *set);
+int mlp_forward(mgpu_mlp *m, const float *x, int n, float *y);
+int mlp_gradient(mgpu_mlp *m, const float *x, const float *t, const float *w, int n, float *grad, double *loss);
+int mlp_train(mgpu_mlp *m, const float *x, const float *t, const float *w, int n, int iters, float lr, double *loss);
+int learned_refresh(mgpu_ctx *ctx, const float *delta_x);
+size_t learned_feature_blocks();
 #define MGPU_HIP(ctx, call)                                                               \
   do {                                                                                    \
     hipError_t e_ = (call);                                                               \

@@ -340,7 +340,9 @@ int pm_deposit(mgpu_ctx *ctx) {
   // With MG screening delta(x) goes to slot 1, where it stays for the screening
   // pass, and the r2c writes delta(k) to slot 0.
   ctx->dx_slot = need_dx ? 1 : 0;
-  return deposit_tiled(ctx, (float *)ctx->slot[ctx->dx_slot]);
+  int rc = deposit_tiled(ctx, (float *)ctx->slot[ctx->dx_slot]);
+  if (!rc && ctx->learn_on) rc = learned_refresh(ctx, (const float *)ctx->slot[ctx->dx_slot]);
+  return rc;
 }
 
 namespace {

@@ -30,6 +30,13 @@ struct mgp_sim {
   int i, ts, nsteps, NoutputStart, finished;
   double loop_time;
   long steps_done;
+  /* learned Green's-function multiplier (host option) and the per-step log of
+   * (a, density feature, CDM P(k)) that its training reads */
+  mgpu_mlp *mlp;
+  int learn_track;
+  double learn_theta;
+  int nlog, caplog;
+  double *log_a, *log_feat, *log_pk; /* log_pk: 3 nbins per entry (P, k mean, modes) */
 };
 typedef struct mgp_sim sim;
 
@@ -190,6 +197,35 @@ static int output(sim *s, double A, double AF, double AFF, double z_label) {
   return 0;
 }
 
+/* one log entry per force evaluation: a, the density feature and the CDM P(k) */
+static int learn_log(sim *s, double A, int have_pk) {
+  double f = 0.0;
+  GPU(mgpu_density_feature(s->g, &f));
+  if (s->nlog == s->caplog) {
+    const int cap = s->caplog ? 2 * s->caplog : 64;
+    double *la = realloc(s->log_a, sizeof(double) * cap), *lf = realloc(s->log_feat, sizeof(double) * cap);
+    if (la) s->log_a = la;
+    if (lf) s->log_feat = lf;
+    double *lp = realloc(s->log_pk, sizeof(double) * 3 * s->nbins * cap);
+    if (lp) s->log_pk = lp;
+    if (!la || !lf || !lp) {
+      snprintf(s->err, s->errlen, "out of host memory for the step log");
+      return -1;
+    }
+    s->caplog = cap;
+  }
+  double *o = s->log_pk + (size_t)3 * s->nbins * s->nlog;
+  for (int b = 0; b < s->nbins; b++) {
+    o[b] = have_pk && s->pk ? s->pk[b] : 0.0;
+    o[s->nbins + b] = have_pk && s->pkk ? s->pkk[b] : 0.0;
+    o[2 * s->nbins + b] = have_pk && s->pkn ? s->pkn[b] : 0.0;
+  }
+  s->log_a[s->nlog] = A;
+  s->log_feat[s->nlog] = f;
+  s->nlog++;
+  return 0;
+}
+
 static int get_displacements(sim *s, double A) {
   const mgp_params *p = s->p;
   mgpu_mg mg;
@@ -206,7 +242,9 @@ static int get_displacements(sim *s, double A) {
     }
     GPU(mgpu_set_neutrinos(s->g, (p->Omega - p->OmegaNu) / p->Omega, &f));
   }
+  if (s->mlp || s->learn_track) GPU(mgpu_set_learned_multiplier(s->g, s->mlp, A, s->learn_theta));
   GPU(mgpu_get_displacements(s->g, p->modified_gravity_active ? &mg : NULL, want_pk ? &pk : NULL, s->sumD));
+  if (s->learn_track && learn_log(s, A, want_pk)) return -1;
   if (want_pk && s->pk &&
       mgp_write_pofk(p, A, "CDM", s->pk, s->pkk, s->pkn, s->nbins, s->bintype, s->kmin, s->kmax) && s->verbose)
     fprintf(stderr, "warning: could not write P(k) file in %s\n", p->OutputDir);
@@ -305,8 +343,31 @@ void mgp_sim_free(mgp_sim *s) {
   free(s->pkn);
   free(s->g1);
   free(s->g2);
+  free(s->log_a);
+  free(s->log_feat);
+  free(s->log_pk);
   free(s);
 }
+
+/* The network's theta input: (log10 f_R0 + 6)/2 for active f(R), -1 otherwise. */
+int mgp_sim_set_learned(mgp_sim *s, mgpu_mlp *m, int track) {
+  const mgp_params *p = s->p;
+  const int fofr = p->model == MGP_MODEL_FOFR || p->model == MGP_MODEL_FOFR_LCDM || p->model == MGP_MODEL_FOFRNU;
+  s->mlp = m;
+  s->learn_track = track;
+  s->learn_theta = fofr && p->modified_gravity_active && p->fofr0 > 0 ? (log10(p->fofr0) + 6.0) / 2.0 : -1.0;
+  if (!m && !track) GPU(mgpu_set_learned_multiplier(s->g, NULL, -1.0, 0.0));
+  return 0;
+}
+int mgp_sim_log_count(const mgp_sim *s) { return s->nlog; }
+int mgp_sim_log_entry(const mgp_sim *s, int i, double *a, double *feat, double *pk) {
+  if (i < 0 || i >= s->nlog) return -1;
+  *a = s->log_a[i];
+  *feat = s->log_feat[i];
+  memcpy(pk, s->log_pk + (size_t)3 * s->nbins * i, sizeof(double) * 3 * s->nbins);
+  return 0;
+}
+int mgp_sim_nbins(const mgp_sim *s) { return s->nbins; }
 
 mgpu_ctx *mgp_sim_gpu(mgp_sim *s) { return s->g; }
 mgp_cosmo *mgp_sim_cosmo(mgp_sim *s) { return s->c; }

@@ -142,6 +142,13 @@ mgpu_ctx *mgp_sim_gpu(mgp_sim *s);
 mgp_cosmo *mgp_sim_cosmo(mgp_sim *s);
 const char *mgp_sim_error(const mgp_sim *s);
 double mgp_sim_a(const mgp_sim *s);
+/* learned Green's-function multiplier: m (not owned) applied from the next force
+ * evaluation on; track = 1 logs a, the density feature and the CDM P(k) of every
+ * force evaluation (P(k) when pofk_compute_every_step) */
+int mgp_sim_set_learned(mgp_sim *s, mgpu_mlp *m, int track);
+int mgp_sim_log_count(const mgp_sim *s);
+int mgp_sim_log_entry(const mgp_sim *s, int i, double *a, double *feat, double *pk /* 3 nbins */);
+int mgp_sim_nbins(const mgp_sim *s);
 
 /* ---- output (src/main.c:792-1149, src/compute_pofk.c:239-261) ---- */
 /* ---- massive neutrinos (CAMB transfer functions, src/read_CAMB_data.h) ---- */

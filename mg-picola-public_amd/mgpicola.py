@@ -22,7 +22,9 @@ MGPU_SYMBOLS = [
     "mgpu_n2_table_len", "mgpu_last_error", "mgpu_device_info", "mgpu_create", "mgpu_destroy", "mgpu_sync",
     "mgpu_set_particles", "mgpu_get_particles", "mgpu_set_lpt", "mgpu_get_lpt", "mgpu_get_disp",
     "mgpu_pm_density", "mgpu_pm_deposit", "mgpu_pm_transform", "mgpu_power_spectrum", "mgpu_rsd_power_spectrum",
-    "mgpu_set_neutrinos", "mgpu_neutrino_response", "mgpu_growth_tables", "mgpu_fifth_force", "mgpu_forces", "mgpu_mesh_to_particles",
+    "mgpu_set_neutrinos", "mgpu_neutrino_response", "mgpu_growth_tables", "mgpu_mlp_create", "mgpu_mlp_destroy",
+    "mgpu_mlp_get_params", "mgpu_mlp_set_params", "mgpu_mlp_forward", "mgpu_mlp_gradient", "mgpu_mlp_train",
+    "mgpu_set_learned_multiplier", "mgpu_density_feature", "mgpu_fifth_force", "mgpu_forces", "mgpu_mesh_to_particles",
     "mgpu_get_displacements", "mgpu_kick", "mgpu_drift", "mgpu_ic_displacement_fields", "mgpu_init_particles",
     "mgpu_lpt_fields", "mgpu_get_mesh", "mgpu_set_mesh", "mgpu_get_timers", "mgpu_enable_timers",
     "mgpu_kernel_name", "mgpu_enable_kernel_timing", "mgpu_kernel_timing", "mgpu_lpt_fields_slices",
@@ -241,6 +243,17 @@ class MGPU:
 
     def neutrino_response(self):
         self._ck(lib().mgpu_neutrino_response(self.ctx), "neutrino_response")
+
+    def set_learned_multiplier(self, mlp, a, theta):
+        lib().mgpu_set_learned_multiplier.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_double]
+        self._ck(lib().mgpu_set_learned_multiplier(self.ctx, mlp.h if mlp is not None else None, a, theta),
+                 "set_learned_multiplier")
+
+    def density_feature(self):
+        lib().mgpu_density_feature.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        s = C.c_double()
+        self._ck(lib().mgpu_density_feature(self.ctx, C.byref(s)), "density_feature")
+        return s.value
 
     def fifth_force(self, mg):
         self._ck(lib().mgpu_fifth_force(self.ctx, C.byref(mg)), "fifth_force")
@@ -563,6 +576,31 @@ class Sim:
         self._ck(hlib().mgp_sim_advance(self.h, C.byref(counted)), "mgp_sim_advance")
         return bool(counted.value)
 
+    def set_learned(self, mlp, track=True):
+        """Apply the learned multiplier of mlp (an MLP, or None) from the next force
+        evaluation on; track logs a, the density feature and the CDM P(k) per step."""
+        L = hlib()
+        L.mgp_sim_set_learned.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        self._mlp = mlp  # the context keeps the pointer
+        self._ck(L.mgp_sim_set_learned(self.h, mlp.h if mlp is not None else None, 1 if track else 0),
+                 "mgp_sim_set_learned")
+
+    def log(self):
+        """[(a, feature, P, k_mean, modes)] of the tracked force evaluations."""
+        L = hlib()
+        for n in ("mgp_sim_log_count", "mgp_sim_nbins"):
+            getattr(L, n).argtypes = [C.c_void_p]
+        L.mgp_sim_log_entry.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                        C.c_void_p]
+        nb = L.mgp_sim_nbins(self.h)
+        out = []
+        for i in range(L.mgp_sim_log_count(self.h)):
+            a, f = C.c_double(), C.c_double()
+            pk = np.zeros(3 * nb)
+            L.mgp_sim_log_entry(self.h, i, C.byref(a), C.byref(f), pk.ctypes.data)
+            out.append((a.value, f.value, pk[:nb], pk[nb:2 * nb], pk[2 * nb:]))
+        return out
+
     @property
     def finished(self):
         return bool(hlib().mgp_sim_finished(self.h))
@@ -587,6 +625,86 @@ class KSlice(C.Structure):
     _fields_ = [("lnk0", C.c_double), ("dlnk", C.c_double), ("nk", C.c_int),
                 ("num", C.POINTER(C.c_double)), ("dnum", C.POINTER(C.c_double)),
                 ("den", C.POINTER(C.c_double)), ("dden", C.POINTER(C.c_double)), ("norm", C.c_double)]
+
+
+class MLP:
+    """The learned multiplier's network on the GPU (libmgpu mgpu_mlp_*)."""
+    NPARAMS = 8705
+
+    def __init__(self, seed=1):
+        L = lib()
+        vp = C.c_void_p
+        fp = C.POINTER(C.c_float)
+        L.mgpu_mlp_create.argtypes = [C.c_ulonglong, C.POINTER(vp)]
+        L.mgpu_mlp_destroy.argtypes = [vp]
+        L.mgpu_mlp_destroy.restype = None
+        L.mgpu_mlp_get_params.argtypes = [vp, fp]
+        L.mgpu_mlp_set_params.argtypes = [vp, fp]
+        L.mgpu_mlp_forward.argtypes = [vp, fp, C.c_int, fp]
+        L.mgpu_mlp_gradient.argtypes = [vp, fp, fp, fp, C.c_int, fp, C.POINTER(C.c_double)]
+        L.mgpu_mlp_train.argtypes = [vp, fp, fp, fp, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_double)]
+        L.mgpu_set_learned_multiplier.argtypes = [vp, vp, C.c_double, C.c_double]
+        L.mgpu_density_feature.argtypes = [vp, C.POINTER(C.c_double)]
+        h = vp()
+        if L.mgpu_mlp_create(seed, C.byref(h)):
+            raise MGPUError(f"mgpu_mlp_create: {L.mgpu_last_error(None).decode()}")
+        self.h = h
+
+    @staticmethod
+    def _f(a, shape=None):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        if shape is not None:
+            a = a.reshape(shape)
+        return a, a.ctypes.data_as(C.POINTER(C.c_float))
+
+    def _ck(self, rc, what):
+        if rc:
+            raise MGPUError(f"{what} failed ({rc}): {lib().mgpu_last_error(None).decode()}")
+
+    @property
+    def params(self):
+        p, pp = self._f(np.zeros(self.NPARAMS))
+        self._ck(lib().mgpu_mlp_get_params(self.h, pp), "mgpu_mlp_get_params")
+        return p
+
+    @params.setter
+    def params(self, v):
+        p, pp = self._f(v, (self.NPARAMS,))
+        self._ck(lib().mgpu_mlp_set_params(self.h, pp), "mgpu_mlp_set_params")
+
+    def forward(self, x):
+        x, xp = self._f(x)
+        y, yp = self._f(np.zeros(len(x)))
+        self._ck(lib().mgpu_mlp_forward(self.h, xp, len(x), yp), "mgpu_mlp_forward")
+        return y
+
+    def gradient(self, x, t, w):
+        x, xp = self._f(x)
+        t, tp = self._f(t)
+        w, wp = self._f(w)
+        g, gp = self._f(np.zeros(self.NPARAMS))
+        loss = C.c_double()
+        self._ck(lib().mgpu_mlp_gradient(self.h, xp, tp, wp, len(x), gp, C.byref(loss)), "mgpu_mlp_gradient")
+        return loss.value, g
+
+    def train(self, x, t, w, iters, lr):
+        x, xp = self._f(x)
+        t, tp = self._f(t)
+        w, wp = self._f(w)
+        loss = C.c_double()
+        self._ck(lib().mgpu_mlp_train(self.h, xp, tp, wp, len(x), iters, lr, C.byref(loss)), "mgpu_mlp_train")
+        return loss.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mgpu_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Spline2D(C.Structure):

@@ -519,3 +519,42 @@ def test_free_streaming_and_wrap():
     dd = np.abs(p - expect)
     assert np.minimum(dd, box - dd).max() < 1e-4
     assert p.min() >= 0 and p.max() < box
+
+
+# ---------------------------------------------------------------- learned multiplier network
+
+def test_mlp_oracle_gradient_is_the_derivative():
+    """The exact (FP64, no rounding) loss gradient of oracle/mlp.py against central
+    finite differences on 60 random parameters, and the closed form of a network
+    whose hidden weights vanish: y = W4 . gelu(b3) + b4 for every input."""
+    import mlp as Q
+    rng = np.random.default_rng(0)
+    p = rng.normal(size=Q.NPARAMS) * 0.3
+    x = rng.normal(size=(29, 4))
+    t = rng.normal(size=29)
+    w = rng.uniform(0.5, 2.0, 29)
+    _, g = Q.loss_and_grad(p, x, t, w, bf16_on=False)
+    for i in rng.choice(Q.NPARAMS, 60, replace=False):
+        h = 1e-6
+        up, dn = p.copy(), p.copy()
+        up[i] += h
+        dn[i] -= h
+        fd = (Q.loss_and_grad(up, x, t, w, False)[0] - Q.loss_and_grad(dn, x, t, w, False)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-7 * np.abs(g).max(), i
+    d = Q.unpack(p)
+    d["W3"][:] = 0.0
+    y = Q.forward(Q.pack(d), x, bf16_on=False)
+    np.testing.assert_allclose(y, d["W4"] @ Q.gelu(d["b3"]) + d["b4"][0], rtol=1e-12)
+
+
+def test_bf16_rounding_is_nearest_even():
+    import mlp as Q
+    one = 1.0
+    ulp = 2.0 ** -7  # BF16 has 8 significant bits
+    assert Q.bf16(one) == one
+    assert Q.bf16(one + ulp / 2) == one                # tie, even neighbour below
+    assert Q.bf16(one + 3 * ulp / 2) == one + 2 * ulp  # tie, even neighbour above
+    assert Q.bf16(one + ulp / 2 + 2.0 ** -20) == one + ulp
+    assert Q.bf16(-3.0) == -3.0
+    assert Q.gelu(np.array([0.0]))[0] == 0.0
+    np.testing.assert_allclose(Q.dgelu(np.array([0.0])), [0.5])
