@@ -17,6 +17,11 @@ step and the roofline of the dominant kernel.
 
 The CPU baseline is the FP64 oracle (a C restatement of the reference's step)
 timed on the host cores for one step of the same path on a bounded sample.
+
+The "learned" object is the metric's second part: the P(k) error at z = 0 for
+k < 1 h/Mpc of a coarse run (Nmesh = Nsample = 512) with the learned
+Green's-function multiplier against the plain run on the 3x finer mesh (1536),
+beside the plain coarse run's error and the time per step of each run.
 """
 import argparse
 import json
@@ -115,6 +120,83 @@ def gpu_bench(steps, warmup, workload):
                 steps_per_run=nsteps)
 
 
+def learned_leg(np_eval=512, box_eval=1024.0, nsteps=30, np_train=64, box_train=128.0):
+    """The metric's second part: the learned Green's-function multiplier on the
+    benchmark's model (f(R) fR0 = 1e-5, scale-dependent, screening, 30 steps).
+    A network is trained on the GPU on coarse (Nmesh = Np) / 3x-fine pairs at
+    64^3 particles in 128 Mpc/h (the same 2 Mpc/h particle spacing and coarse
+    Nyquist k as the evaluation; seeds 101-103, LCDM and f(R)), then at 512^3 in
+    1024 Mpc/h the plain fine run (Nmesh 1536), the plain coarse run and the
+    learned coarse run (Nmesh 512) give max |P/P_fine - 1| over the bins with
+    k < 1 h/Mpc at z = 0 (64 linear bins up to the coarse Nyquist k, shot noise
+    subtracted) and their time per step."""
+    import math
+
+    import learned as LR
+    import mgpicola as M
+    import workloads as W
+
+    out = tempfile.mkdtemp(prefix="mgpicola_learned_")
+
+    def params(tag, nsample, nmesh, box, seed, fofr0):
+        d = os.path.join(out, tag)
+        os.makedirs(d, exist_ok=True)
+        over = dict(Nsample=nsample, Nmesh=nmesh, Box=box, Seed=seed, pofk_nbins=64, pofk_kmin=0.0,
+                    pofk_kmax=math.pi * nsample / box, pofk_compute_every_step=1)
+        if not fofr0:
+            over.update(modified_gravity_active=0, include_screening=0)
+        else:
+            over.update(fofr0=fofr0)
+        model, vals, _ = W.values_for("fofr_sd_512_1024", d, **over)
+        with open(vals["OutputRedshiftFile"], "w") as f:
+            f.write(f"0, {nsteps}\n")
+        pf = os.path.join(d, "params.txt")
+        M.write_paramfile(pf, vals, model=model)
+        return M.read_params(pf, model=model)
+
+    t0 = time.time()
+    pairs = [(params(f"c{s}_{f}", np_train, np_train, box_train, s, f),
+              params(f"f{s}_{f}", np_train, 3 * np_train, box_train, s, f)) for s in (101, 102, 103)
+             for f in (0.0, 1e-5)]
+    mlp = M.MLP(seed=7)
+    hist = LR.train(pairs, mlp, iterations=6, iters=1500, lr=3e-3, eta=0.6,
+                    log=lambda m: print(f"[bench] learned: {m}", file=sys.stderr, flush=True))
+    t_train = time.time() - t0
+
+    def timed_run(hp, m=None):
+        sim = M.Sim(hp)
+        try:
+            sim.set_learned(m, True)
+            sim.ic()
+            sim.gpu.sync()
+            t = time.perf_counter()
+            n = 0
+            while not sim.finished:
+                sim.advance()
+                n += 1
+            sim.gpu.sync()
+            return sim.log(), (time.perf_counter() - t) / n * 1e3
+        finally:
+            sim.close()
+
+    fine, ms_fine = timed_run(params("eval_fine", np_eval, 3 * np_eval, box_eval, 5001, 1e-5))
+    print(f"[bench] learned: fine run {ms_fine:.1f} ms/step", file=sys.stderr, flush=True)
+    hc = params("eval_coarse", np_eval, np_eval, box_eval, 5001, 1e-5)
+    plain, ms_plain = timed_run(hc)
+    lrn, ms_lrn = timed_run(hc, mlp)
+    mlp.close()
+    e_plain, e_lrn = LR.pofk_error(fine, plain)[0], LR.pofk_error(fine, lrn)[0]
+    print(f"[bench] learned: P(k) error plain {e_plain:.4f}, learned {e_lrn:.4f}", file=sys.stderr, flush=True)
+    return {"pofk_err_learned": round(e_lrn, 4), "pofk_err_plain_coarse": round(e_plain, 4), "kmax": 1.0,
+            "ms_per_step_fine": round(ms_fine, 2), "ms_per_step_coarse_learned": round(ms_lrn, 2),
+            "ms_per_step_coarse_plain": round(ms_plain, 2), "speedup_vs_fine": round(ms_fine / ms_lrn, 2),
+            "eval": f"{np_eval}^3 particles, Box {box_eval:g}, coarse Nmesh {np_eval} vs fine {3 * np_eval}, "
+                    f"f(R) fR0=1e-5 scale-dependent + screening, {nsteps} steps, seed 5001",
+            "train": f"{len(pairs)} pairs at {np_train}^3/Box {box_train:g} (coarse {np_train} vs {3 * np_train}), "
+                     f"6 fixed-point iterations, {t_train:.0f} s",
+            "train_history": [[h[0], round(h[2], 4)] for h in hist]}
+
+
 def cpu_baseline(ns=256, nmesh=512, box=512.0):
     """One COLA step of the same f(R) + screening + scale-dependent path in the
     FP64 oracle: deposit, r2c, P(k), fifth force (2 FFTs), Forces (3 c2r),
@@ -188,6 +270,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="fofr_sd_512_1024")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-multiplier P(k) error leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy this many MiB device to device (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -213,6 +296,12 @@ def main():
     per_kernel = {k: {"launches": round(v[0], 2), "ms": round(v[1], 3),
                       "GBps": round(v[2] / (v[1] * 1e-3) / 1e9, 1) if v[1] > 0 else None}
                   for k, v in kt.items() if v[0]}
+    lrn = None
+    if not args.no_learned:
+        try:
+            lrn = learned_leg()
+        except Exception as e:  # the throughput line still prints; the failure is reported in it
+            lrn = {"error": f"{type(e).__name__}: {e}"}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline()
@@ -223,7 +312,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": args.workload, "particles": r["npart"], "nmesh": r["nmesh"], "box": r["box"],
                    "model": "f(R) Hu-Sawicki fR0=1e-5 n=1, scale-dependent, screening, P(k) every step"},
-        "roofline": roof, "cpu_baseline": cpu, "kernels_per_step": per_kernel,
+        "roofline": roof, "cpu_baseline": cpu, "learned": lrn, "kernels_per_step": per_kernel,
     }
     print(json.dumps(line), flush=True)
 
