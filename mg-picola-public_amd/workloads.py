@@ -30,6 +30,9 @@ WORKLOADS = {
     # linear bins: 512^3 particles, 1024^3 mesh, Box 1024 Mpc/h, 30 steps
     "fofr_sd_512_1024": ("FOFR", dict(Nmesh=1024, Nsample=512, Box=1024.0, modified_gravity_active=1,
                                       include_screening=1, pofk_compute_every_step=1, pofk_nbins=512), 30),
+    # the same run on the coarse mesh of the learned multiplier (Nmesh = Nsample = 512)
+    "fofr_sd_512_512": ("FOFR", dict(Nmesh=512, Nsample=512, Box=1024.0, modified_gravity_active=1,
+                                     include_screening=1, pofk_compute_every_step=1, pofk_nbins=256), 30),
 }
 
 
