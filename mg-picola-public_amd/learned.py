@@ -23,7 +23,11 @@ Training is a fixed-point iteration, all of it on the GPU through libmgpu:
      coarse estimator's aliasing dominates it.)  The training runs keep the
      shot noise in P(k), which both runs share;
   4. the network is refitted with full-batch Adam (mgpu_mlp_train) and the
-     next iteration repeats 2-4.
+     next iteration repeats 2-4;
+  5. the iteration is not monotonic (a refit can overshoot into a coarse run
+     whose boosted grid-scale forces heat the particles), so every iterate's
+     error on the pairs is measured, the last one's too, and the best network
+     is kept.
 The plain fine-mesh run stays the reference of the P(k) error.
 """
 import math
@@ -90,24 +94,38 @@ def pofk_error(fine, coarse, kmax=1.0):
     return float(np.max(np.abs(pc[ok] / pf[ok] - 1.0))), kc[ok]
 
 
-def train(pairs, mlp, iterations=6, iters=1500, lr=3e-3, kmax=None, eta=0.6, log=None):
+def train(pairs, mlp, iterations=6, iters=1500, lr=3e-3, kmax=None, eta=0.6, eta_decay=1.0, log=None):
     """pairs: [(hp_coarse, hp_fine)]; returns the training history
-    [(iteration, loss, mean P(k) error over the pairs before the refit)]."""
+    [(iteration, loss, mean P(k) error over the pairs before the refit)].
+    Iteration i uses the gain eta * eta_decay**i."""
     keep = dict(pofk_subtract_shotnoise=0)
     fine_logs = [run_logged(clone(f, **keep)) for _, f in pairs]
     hist = []
-    for it in range(iterations):
+    best = (math.inf, None, -1)
+    for it in range(iterations + 1):
         Xs, Ts, Ws, errs = [], [], [], []
+        e = eta * eta_decay ** it
         for (c, _), fl in zip(pairs, fine_logs):
             cl = run_logged(clone(c, **keep), mlp)
             km = kmax if kmax is not None else math.pi * c.Nmesh / c.Box
-            x, t, w = samples(fl, cl, c, mlp, km, eta=eta, clip=0.5 * eta)
-            Xs.append(x)
-            Ts.append(t)
-            Ws.append(w)
             errs.append(pofk_error(fl, cl, km)[0])
+            if it < iterations:
+                x, t, w = samples(fl, cl, c, mlp, km, eta=e, clip=0.5 * e)
+                Xs.append(x)
+                Ts.append(t)
+                Ws.append(w)
+        err = float(np.mean(errs))
+        if err < best[0]:
+            best = (err, mlp.params.copy(), it)
+        if it == iterations:
+            hist.append((it, None, err))
+            break
         loss = mlp.train(np.concatenate(Xs), np.concatenate(Ts), np.concatenate(Ws), iters, lr)
-        hist.append((it, loss, float(np.mean(errs))))
+        hist.append((it, loss, err))
         if log:
-            log(f"iteration {it}: P(k) error before refit {np.mean(errs):.4f}, loss {loss:.3e}")
+            log(f"iteration {it}: P(k) error before refit {err:.4f}, loss {loss:.3e}")
+    # the iteration is not monotonic: keep the network with the smallest error on the pairs
+    mlp.params = best[1]
+    if log:
+        log(f"kept the network of iteration {best[2]} (P(k) error {best[0]:.4f})")
     return hist
