@@ -159,6 +159,13 @@ int mgpu_kick(mgpu_ctx *ctx, double dda, double A, double s1, double s2, const d
  * scale-dependent lpt = slot0*s1 + slot1*s2 with s = (deltaD, deltaD2);
  * scale-dependent lpt = slot1.  src/main.c:747-787 */
 int mgpu_drift(mgpu_ctx *ctx, double dyyy, double s1, double s2, const double sumV[3]);
+/* Kick then Drift in one pass, for the steps with no output between them
+ * (src/main.c:688-787).  sumV: in, the mean velocity before the kick, which the
+ * drift uses (the kick term has zero mean, so it is the mean after the kick up
+ * to rounding); out, the mean after the kick.  Disp is left as gathered (not
+ * reduced by sumD), so a further mgpu_kick needs a new mgpu_mesh_to_particles. */
+int mgpu_kick_drift(mgpu_ctx *ctx, double dda, double A, double s1k, double s2k, const double sumD[3], double dyyy,
+                    double s1d, double s2d, double sumV[3]);
 
 /* ---- initial conditions ---- */
 /* displacement_fields (Gaussian): seed table + per-column RANLUX stream,

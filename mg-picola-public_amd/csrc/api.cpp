@@ -530,6 +530,15 @@ int mgpu_drift(mgpu_ctx *ctx, double dyyy, double s1, double s2, const double su
   return pm_drift(ctx, dyyy, s1, s2, sumV);
 }
 
+int mgpu_kick_drift(mgpu_ctx *ctx, double dda, double A, double s1k, double s2k, const double sumD[3], double dyyy,
+                    double s1d, double s2d, double sumV[3]) {
+  if (check_ctx(ctx) || !sumD || !sumV) return MGPU_E_INVALID;
+  if (!ctx->have_disp) return mgpu_fail(ctx, MGPU_E_STATE, "no Disp: call mgpu_mesh_to_particles first");
+  const int rc = pm_kick_drift(ctx, dda, A, s1k, s2k, sumD, dyyy, s1d, s2d, sumV);
+  ctx->have_disp = false;  // Disp was not reduced by sumD
+  return rc;
+}
+
 int mgpu_ic_displacement_fields(mgpu_ctx *ctx, unsigned long seed, int sphere_mode, int amplitude_fixed, int inverted,
                                 const double *pk_n2) {
   if (check_ctx(ctx) || !pk_n2) return MGPU_E_INVALID;

@@ -221,6 +221,8 @@ int pm_step_fused(mgpu_ctx *ctx, const mgpu_mg *mg, mgpu_pofk *out, double sumD[
 int pm_gather(mgpu_ctx *ctx, double sumD[3]);
 int pm_kick(mgpu_ctx *ctx, double dda, double A, double s1, double s2, const double sumD[3], double sumV[3]);
 int pm_drift(mgpu_ctx *ctx, double dyyy, double s1, double s2, const double sumV[3]);
+int pm_kick_drift(mgpu_ctx *ctx, double dda, double A, double s1k, double s2k, const double sumD[3], double dyyy,
+                  double s1d, double s2d, double sumV[3]);
 int pm_reduce3(mgpu_ctx *ctx, int nblocks, double out[3]);
 
 // ---------------------------------------------------------------- IC / LPT (ic.hip)

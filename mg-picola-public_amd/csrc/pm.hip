@@ -322,6 +322,44 @@ __global__ __launch_bounds__(kBlock) void k_drift(float *__restrict__ pos, const
   }
 }
 
+// Kick followed by Drift in one pass over the particles (src/main.c:688-787 on
+// the steps with no output between them).  The drift's mean velocity is the
+// mean before the kick (sV): the kick term has zero mean (Disp - sumD, and the
+// LPT fields, whose k = 0 mode is zero), so it is the mean after the kick up to
+// rounding.  Disp is not written back (only a second kick at an output step
+// needs Disp - sumD, and that takes the separate passes).
+__global__ __launch_bounds__(kBlock) void k_kick_drift(float *__restrict__ vel, float *__restrict__ pos,
+                                                       const float *__restrict__ disp, const float *__restrict__ l0,
+                                                       const float *__restrict__ l1, size_t np, int sdk, double s1k,
+                                                       double s2k, int sdd, double s1d, double s2d, double omega,
+                                                       double dda, double A, double dyyy, float box, int use_cola,
+                                                       double sD0, double sD1, double sD2, double sV0, double sV1,
+                                                       double sV2, double *__restrict__ partial) {
+#pragma clang fp contract(off)  // same rounding sequence as the separate passes
+  const size_t base = (size_t)blockIdx.x * kChunk + threadIdx.x;
+  double s[3] = {0, 0, 0};
+  const double sD[3] = {sD0, sD1, sD2}, sV[3] = {sV0, sV1, sV2};
+#pragma unroll 2
+  for (int r = 0; r < kRed; r++) {
+    const size_t i = base + (size_t)r * kBlock;
+    if (i >= np) break;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const size_t idx = (size_t)a * np + i;
+      const float dv = (float)((double)disp[idx] - sD[a]);
+      const double lk = use_cola ? lpt_term(l0, l1, s1k, s2k, sdk, idx) : 0.0;
+      const double force = -1.5 * omega * (double)dv - use_cola * lk / A;
+      const float v = (float)((double)vel[idx] + force * dda);
+      vel[idx] = v;
+      s[a] += (double)v;
+      const float p = (float)((double)pos[idx] + ((double)v - sV[a]) * dyyy);
+      const double ld = use_cola ? lpt_term(l0, l1, s1d, s2d, sdd, idx) : 0.0;
+      pos[idx] = wrapf((float)((double)p + use_cola * ld), box);
+    }
+  }
+  block_sum3(s, partial);
+}
+
 }  // namespace
 
 // =================================================================== host side
@@ -687,6 +725,33 @@ int pm_drift(mgpu_ctx *ctx, double dyyy, double s1, double s2, const double sumV
                        sumV[0], sumV[1], sumV[2]);
   }
   MGPU_HIP(ctx, hipGetLastError());
+  return MGPU_OK;
+}
+
+int pm_kick_drift(mgpu_ctx *ctx, double dda, double A, double s1k, double s2k, const double sumD[3], double dyyy,
+                  double s1d, double s2d, double sumV[3]) {
+  PhaseTimer pt(ctx, &ctx->t.kick);
+  ctx->sorted_valid = false;
+  if (ctx->np == 0) {
+    sumV[0] = sumV[1] = sumV[2] = 0;
+    return MGPU_OK;
+  }
+  const unsigned nb = cdiv(ctx->np, kChunk);
+  const bool sdep = ctx->p.scale_dependent;
+  {
+    // vel, pos read + written, Disp and the LPT vectors read
+    KScope ks(ctx, MGPU_K_KICK, 84.0 * ctx->np);
+    hipLaunchKernelGGL(k_kick_drift, dim3(nb), dim3(kBlock), 0, ctx->stream, ctx->vel, ctx->pos,
+                       (const float *)ctx->disp, (const float *)ctx->lpt[0], (const float *)ctx->lpt[1], ctx->np,
+                       sdep ? 1 : 0, s1k, s2k, sdep ? 2 : 0, s1d, s2d, ctx->p.omega, dda, A, dyyy,
+                       (float)ctx->p.box, ctx->p.use_cola, sumD[0], sumD[1], sumD[2], sumV[0], sumV[1], sumV[2],
+                       ctx->partial);
+  }
+  MGPU_HIP(ctx, hipGetLastError());
+  double s[3];
+  int rc = pm_reduce3(ctx, (int)nb, s);
+  if (rc) return rc;
+  for (int a = 0; a < 3; a++) sumV[a] = s[a] / (double)ctx->np;
   return MGPU_OK;
 }
 

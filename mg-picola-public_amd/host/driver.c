@@ -32,6 +32,7 @@ struct mgp_sim {
   long steps_done;
   /* learned Green's-function multiplier (host option) and the per-step log of
    * (a, density feature, CDM P(k)) that its training reads */
+  int vmean_valid; /* sumV holds the mean of the current velocities (after a kick) */
   mgpu_mlp *mlp;
   int learn_track;
   double learn_theta;
@@ -124,6 +125,31 @@ static int drift(sim *s, double A, double AFF, double AF, double Di, double Di2)
     s2 = mgp_growth(s->c, MGP_G_D2, AFF) - Di2;
   }
   GPU(mgpu_drift(s->g, dyyy, s1, s2, s->sumV));
+  return 0;
+}
+
+/* Kick then Drift in one GPU pass (the steps with no output between them): the
+ * same time factors as kick() and drift(); the drift's mean velocity is the one
+ * before the kick (s->sumV, valid once a kick has run). */
+static int kick_drift(sim *s, double AI, double AF, double A, double AFF, double Di, double Di2) {
+  double dda, dyyy;
+  if (s->StdDA == 0) {
+    dda = mgp_Sphi(s->c, AI, AF, A);
+    dyyy = mgp_Sq(s->c, A, AFF, AF);
+  } else {
+    extern double mgp_SphiStd(const mgp_cosmo *, double, double);
+    extern double mgp_SqStd(const mgp_cosmo *, double, double);
+    dda = mgp_SphiStd(s->c, AI, AF);
+    dyyy = mgp_SqStd(s->c, A, AFF);
+  }
+  double s1k = 0, s2k = 0, s1d = 0, s2d = 0;
+  if (!s->p->scale_dependent) {
+    s1k = mgp_growth(s->c, MGP_G_ddDddy, A);
+    s2k = mgp_growth(s->c, MGP_G_ddD2ddy, A);
+    s1d = mgp_growth(s->c, MGP_G_D, AFF) - Di;
+    s2d = mgp_growth(s->c, MGP_G_D2, AFF) - Di2;
+  }
+  GPU(mgpu_kick_drift(s->g, dda, A, s1k, s2k, s->sumD, dyyy, s1d, s2d, s->sumV));
   return 0;
 }
 
@@ -442,19 +468,25 @@ int mgp_sim_advance(mgp_sim *s, int *counted) {
   if (p->scale_dependent) {
     if (lpt_fields2(s, MGP_FIELD_ddDddy, MGP_FIELD_deltaD, A, s->AFF)) return -1;
   }
-  if (kick(s, s->AI, s->AF, A)) return -1;
-  if (s->ts == 0 && s->i != s->NoutputStart) {
-    if (output(s, A, s->AF, s->AFF, p->out_z[s->i - 1])) return -1;
-    if (s->i == p->Noutputs) {
-      s->finished = 1;
-      return 0;
-    }
-    s->AI = A;
-    s->AF = s->stepDistr == 0 ? A + da * 0.5 : A * exp(da * 0.5);
-    s->sumD[0] = s->sumD[1] = s->sumD[2] = 0.0;
+  const int output_now = s->ts == 0 && s->i != s->NoutputStart;
+  if (!output_now && s->vmean_valid) {
+    if (kick_drift(s, s->AI, s->AF, A, s->AFF, s->Di, s->Di2)) return -1;
+  } else {
     if (kick(s, s->AI, s->AF, A)) return -1;
+    s->vmean_valid = 1;
+    if (output_now) {
+      if (output(s, A, s->AF, s->AFF, p->out_z[s->i - 1])) return -1;
+      if (s->i == p->Noutputs) {
+        s->finished = 1;
+        return 0;
+      }
+      s->AI = A;
+      s->AF = s->stepDistr == 0 ? A + da * 0.5 : A * exp(da * 0.5);
+      s->sumD[0] = s->sumD[1] = s->sumD[2] = 0.0;
+      if (kick(s, s->AI, s->AF, A)) return -1;
+    }
+    if (drift(s, A, s->AFF, s->AF, s->Di, s->Di2)) return -1;
   }
-  if (drift(s, A, s->AFF, s->AF, s->Di, s->Di2)) return -1;
   s->A = s->AFF;
   s->AI = s->AF;
   s->Di = mgp_growth(s->c, MGP_G_D, s->A);

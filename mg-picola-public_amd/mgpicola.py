@@ -24,7 +24,7 @@ MGPU_SYMBOLS = [
     "mgpu_pm_density", "mgpu_pm_deposit", "mgpu_pm_transform", "mgpu_power_spectrum", "mgpu_rsd_power_spectrum",
     "mgpu_set_neutrinos", "mgpu_neutrino_response", "mgpu_growth_tables", "mgpu_mlp_create", "mgpu_mlp_destroy",
     "mgpu_mlp_get_params", "mgpu_mlp_set_params", "mgpu_mlp_forward", "mgpu_mlp_gradient", "mgpu_mlp_train",
-    "mgpu_set_learned_multiplier", "mgpu_density_feature", "mgpu_fifth_force", "mgpu_forces", "mgpu_mesh_to_particles",
+    "mgpu_set_learned_multiplier", "mgpu_density_feature", "mgpu_kick_drift", "mgpu_fifth_force", "mgpu_forces", "mgpu_mesh_to_particles",
     "mgpu_get_displacements", "mgpu_kick", "mgpu_drift", "mgpu_ic_displacement_fields", "mgpu_init_particles",
     "mgpu_lpt_fields", "mgpu_get_mesh", "mgpu_set_mesh", "mgpu_get_timers", "mgpu_enable_timers",
     "mgpu_kernel_name", "mgpu_enable_kernel_timing", "mgpu_kernel_timing", "mgpu_lpt_fields_slices",
@@ -254,6 +254,16 @@ class MGPU:
         s = C.c_double()
         self._ck(lib().mgpu_density_feature(self.ctx, C.byref(s)), "density_feature")
         return s.value
+
+    def kick_drift(self, dda, A, s1k, s2k, sumD, dyyy, s1d, s2d, sumV):
+        """Kick then Drift in one pass; sumV in: mean velocity before the kick; returns the mean after."""
+        L = lib()
+        d = C.c_double
+        L.mgpu_kick_drift.argtypes = [C.c_void_p, d, d, d, d, C.POINTER(d), d, d, d, C.POINTER(d)]
+        sd = (d * 3)(*sumD)
+        sv = (d * 3)(*sumV)
+        self._ck(L.mgpu_kick_drift(self.ctx, dda, A, s1k, s2k, sd, dyyy, s1d, s2d, sv), "kick_drift")
+        return np.array(sv[:])
 
     def fifth_force(self, mg):
         self._ck(lib().mgpu_fifth_force(self.ctx, C.byref(mg)), "fifth_force")

@@ -394,3 +394,47 @@ def test_kick_drift_match_oracle(pos, sd):
         O.drift(rpos, rvel, l0, l1, t1, t2, 0, BOX, dyyy, 1, rsV)
     gpos, _ = g.get_particles()
     np.testing.assert_array_equal(gpos, rpos)
+
+
+@pytest.mark.parametrize("sd", [0, 1])
+def test_fused_kick_drift_equals_separate_passes(pos, sd):
+    """mgpu_kick_drift against mgpu_kick + mgpu_drift on the same state: the
+    velocities and the returned mean velocity bit for bit; the positions within
+    FP32 rounding (the drift takes the mean velocity from before the kick, equal
+    to the mean after it up to rounding because the kick term has zero mean)."""
+    rng = np.random.default_rng(12)
+    vel = (rng.normal(size=(3, NS ** 3)) * 3.0).astype(np.float32)
+    l0 = (rng.normal(size=(3, NS ** 3)) * 0.7).astype(np.float32)
+    l1 = (rng.normal(size=(3, NS ** 3)) * 0.1).astype(np.float32)
+    l0 -= l0.mean(axis=1, keepdims=True)  # LPT fields have zero mean
+    l1 -= l1.mean(axis=1, keepdims=True)
+    dda, A, s1k, s2k, dyyy, s1d, s2d = 0.37, 0.4, 0.8, -0.2, 0.9, 0.05, 0.01
+
+    def state():
+        g = M.MGPU(N, NS, BOX, OM, scale_dependent=sd)
+        g.set_particles(pos, vel)
+        g.set_lpt(0, l0)
+        g.set_lpt(1, l1)
+        g.pm_density()
+        g.forces()
+        return g, g.mesh_to_particles()
+
+    g, sumD = state()
+    sV0 = vel.astype(np.float64).mean(axis=1)
+    sV = g.kick(dda, A, s1k, s2k, sumD)
+    g.drift(dyyy, s1d, s2d, sV)
+    p_sep, v_sep = g.get_particles()
+    g.close()
+    g, sumD2 = state()
+    np.testing.assert_array_equal(sumD2, sumD)
+    sV_f = g.kick_drift(dda, A, s1k, s2k, sumD, dyyy, s1d, s2d, sV0)
+    p_f, v_f = g.get_particles()
+    np.testing.assert_array_equal(v_f, v_sep)
+    np.testing.assert_array_equal(sV_f, sV)
+    d = np.abs(p_f - p_sep)
+    d = np.minimum(d, BOX - d)  # periodic
+    assert d.max() < 1e-5 * BOX, d.max()
+    assert np.abs(sV - sV0).max() < 1e-3 * np.abs(sV0).max()  # the premise: the kick keeps the mean
+    with pytest.raises(M.MGPUError):  # Disp was not reduced by sumD: a further kick needs a new gather
+        g.kick(dda, A, s1k, s2k, [0.0, 0.0, 0.0])
+    g.close()
