@@ -1,7 +1,6 @@
 // C ABI of libmgpu (include/mgpu.h): context management, particle I/O and the
 // entry points that replace the reference's PM/COLA routines.
 #include "mgpu_internal.h"
-#include "cic.h"
 
 #include <cmath>
 #include <cstring>
@@ -198,12 +197,7 @@ int mgpu_create(const mgpu_params *p, mgpu_ctx **out) {
       if ((rc = alloc(ctx, (void **)&ctx->lattice_slot, 6 * nl * sizeof(float2), "lattice spectra"))) return fail(rc);
     }
   }
-  {
-    // per-block partial sums: the particle kernels' blocks, or one row per tile of the tiled gather
-    const size_t ntile1 = (size_t)(ctx->N / cic_tile_size(ctx->N));
-    ctx->partial_len = 3 * std::max<size_t>(std::max<size_t>(cdiv(ctx->np, 256), (size_t)ctx->N), ntile1 * ntile1 * ntile1) +
-                       3 * 1024;
-  }
+  ctx->partial_len = 3 * (size_t)std::max<size_t>(cdiv(ctx->np, 256), (size_t)ctx->N) + 3 * 1024;
   if ((rc = alloc(ctx, (void **)&ctx->partial, ctx->partial_len * sizeof(double), "reduction scratch"))) return fail(rc);
   if ((rc = alloc(ctx, (void **)&ctx->dsum, 8 * sizeof(double), "sums"))) return fail(rc);
   const int nb = std::max(1, p->pofk_nbins);

@@ -274,10 +274,9 @@ __global__ __launch_bounds__(512) void k_gather_tile(const float4 *__restrict__ 
                                                      const unsigned *__restrict__ off, int N, int nt1, int HP2,
                                                      double scale, const float *__restrict__ g0,
                                                      const float *__restrict__ g1, const float *__restrict__ g2,
-                                                     float *__restrict__ disp, double *__restrict__ partial) {
+                                                     float *__restrict__ disp) {
   constexpr int B = T + 1, B2 = B * B, B3 = B2 * B;
   __shared__ float f[3][B3];
-  __shared__ double red[3][8];
   const unsigned nt = (unsigned)nt1 * nt1 * nt1;
   const unsigned t = nt % 8 == 0 ? (blockIdx.x & 7) * (nt >> 3) + (blockIdx.x >> 3) : blockIdx.x;
   const int ti = t / (nt1 * nt1), tj = (t / nt1) % nt1, tk = t % nt1;
@@ -328,7 +327,6 @@ __global__ __launch_bounds__(512) void k_gather_tile(const float4 *__restrict__ 
     }
   }
   __syncthreads();
-  double sum[3] = {0.0, 0.0, 0.0};
   for (; q < p1; q += NT) {
     const float4 r = nxt;
     if (q + NT < p1) nxt = rec[q + NT];
@@ -349,28 +347,9 @@ __global__ __launch_bounds__(512) void k_gather_tile(const float4 *__restrict__ 
       const float *m = f[a];
       const double d = m[b00] * w000 + m[b00 + 1] * w001 + m[b01] * w010 + m[b01 + 1] * w011 + m[b10] * w100 +
                        m[b10 + 1] * w101 + m[b11] * w110 + m[b11 + 1] * w111;
-      const float df = (float)d;
-      disp[(size_t)a * np + i] = df;
-      sum[a] += (double)df;
+      disp[(size_t)a * np + i] = (float)d;
     }
   }
-  // the tile's sums of the stored Disp for the mean: per thread in record order,
-  // then waves and the 8 wave sums in a fixed order; one row of partials per tile
-  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
-#pragma unroll
-  for (int a = 0; a < 3; a++) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) sum[a] += __shfl_down(sum[a], o);
-    if (lane == 0) red[a][w] = sum[a];
-  }
-  __syncthreads();
-  if (threadIdx.x == 0)
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-      double acc = red[a][0];
-      for (int q2 = 1; q2 < 8; q2++) acc += red[a][q2];
-      partial[(size_t)t * 3 + a] = acc;
-    }
 }
 
 TileGeom make_geom(int N) {
@@ -455,17 +434,15 @@ int deposit_tiled(mgpu_ctx *ctx, float *d, const float4 *rsd) {
 
 // Disp at the particles from the force meshes f0, f1, f2 (real layout), by
 // the bins of the last deposit (the positions must not have changed since)
-int gather_tiled(mgpu_ctx *ctx, const float *f0, const float *f1, const float *f2, int *ntiles) {
+int gather_tiled(mgpu_ctx *ctx, const float *f0, const float *f1, const float *f2) {
   const int N = ctx->N;
   if (!ctx->sorted_valid) return mgpu_fail(ctx, MGPU_E_STATE, "gather by tiles needs the bins of the last deposit");
   const int T = cic_tile_size(N), nt1 = N / T;
   const unsigned nt = (unsigned)nt1 * nt1 * nt1;
   const double scale = (double)N / ctx->p.box;
   auto kern = T == 16 ? k_gather_tile<16> : T == 8 ? k_gather_tile<8> : k_gather_tile<4>;
-  if (3 * (size_t)nt > ctx->partial_len) return mgpu_fail(ctx, MGPU_E_INVALID, "reduction scratch too small");
   hipLaunchKernelGGL(kern, dim3(nt), dim3(512), 0, ctx->stream, ctx->dep_rec, ctx->np, ctx->dep_off, N, nt1,
-                     2 * ctx->HP, scale, f0, f1, f2, ctx->disp, ctx->partial);
+                     2 * ctx->HP, scale, f0, f1, f2, ctx->disp);
   MGPU_HIP(ctx, hipGetLastError());
-  *ntiles = (int)nt;
   return MGPU_OK;
 }

@@ -204,8 +204,7 @@ size_t deposit_scratch_bytes(int N, size_t np);
 // (with rsd != NULL: particles at the redshift-space records of k_rsd_coords instead)
 int deposit_tiled(mgpu_ctx *ctx, float *d, const float4 *rsd = nullptr);
 // Disp from the three force meshes by the last deposit's bins (sorted_valid must hold)
-// the tiled gather also leaves one row of Disp sums per tile in ctx->partial (ntiles rows)
-int gather_tiled(mgpu_ctx *ctx, const float *f0, const float *f1, const float *f2, int *ntiles);
+int gather_tiled(mgpu_ctx *ctx, const float *f0, const float *f1, const float *f2);
 
 // ---------------------------------------------------------------- PM kernels (pm.hip)
 int pm_deposit(mgpu_ctx *ctx);                         // particles -> delta(x) in slot0 (+ copy slot1)

@@ -185,8 +185,7 @@ __device__ __forceinline__ void block_sum3(double s[3], double *__restrict__ par
 // MtoParticles (src/auxPM.c:560-644): CIC interpolation of the three force
 // meshes (WPAR = 1), Disp stored as float.  This direct form serves when the
 // deposit's bins do not describe the current positions; otherwise the gather
-// goes by tiles (gather_tiled), which forms the sums for the mean per tile; after
-// this direct form k_sum3 forms them in particle order.
+// goes by tiles (gather_tiled).  Either way k_sum3 forms the sums for the mean.
 __global__ __launch_bounds__(kBlock) void k_gather(const float *__restrict__ pos, size_t np, int N, int HP2,
                                                    double scale, const float *__restrict__ g0,
                                                    const float *__restrict__ g1, const float *__restrict__ g2,
@@ -665,25 +664,26 @@ int pm_gather(mgpu_ctx *ctx, double sumD[3]) {
               *f2 = (const float *)ctx->slot[ctx->force_slot[2]];
   // compulsory: positions and Disp (12 B each per particle) and the three force meshes once
   const double bytes = 24.0 * ctx->np + 12.0 * (double)N * N * N;
-  int nparts = (int)nb;
   if (ctx->sorted_valid) {
-    // by the deposit's tiles, with the Disp sums per tile (fixed tile and record order)
-    KScope ks(ctx, MGPU_K_GATHER, bytes);
-    int rc = gather_tiled(ctx, f0, f1, f2, &nparts);
-    if (rc) return rc;
-  } else {
+    // by the deposit's tiles, then the sums in particle order
     {
       KScope ks(ctx, MGPU_K_GATHER, bytes);
-      hipLaunchKernelGGL(k_gather, dim3(cdiv(ctx->np, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->pos, ctx->np, N,
-                         2 * ctx->HP, (double)N / ctx->p.box, f0, f1, f2, ctx->disp);
+      int rc = gather_tiled(ctx, f0, f1, f2);
+      if (rc) return rc;
     }
-    MGPU_HIP(ctx, hipGetLastError());
+  } else {
+    KScope ks(ctx, MGPU_K_GATHER, bytes);
+    hipLaunchKernelGGL(k_gather, dim3(cdiv(ctx->np, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->pos, ctx->np, N,
+                       2 * ctx->HP, (double)N / ctx->p.box, f0, f1, f2, ctx->disp);
+  }
+  MGPU_HIP(ctx, hipGetLastError());
+  {
     KScope ks(ctx, MGPU_K_MEAN, 12.0 * ctx->np);
     hipLaunchKernelGGL(k_sum3, dim3(nb), dim3(kBlock), 0, ctx->stream, ctx->disp, ctx->np, ctx->partial);
   }
   MGPU_HIP(ctx, hipGetLastError());
   double s[3];
-  int rc = pm_reduce3(ctx, nparts, s);
+  int rc = pm_reduce3(ctx, (int)nb, s);
   if (rc) return rc;
   for (int a = 0; a < 3; a++) sumD[a] = s[a] / (double)ctx->np;
   return MGPU_OK;

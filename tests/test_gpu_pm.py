@@ -91,9 +91,8 @@ def test_deposit_clustered_matches_oracle(n, ns):
 
 @pytest.mark.parametrize("n,ns", [(12, 6), (24, 12), (32, 16)])
 def test_tiled_gather_equals_direct_gather(n, ns):
-    """The gather by the deposit's tiles gives the same Disp (bit for bit) as the
-    direct gather in particle order, and the same mean up to the order of its
-    FP64 sums (per tile in record order against particle order)."""
+    """The gather by the deposit's tiles gives the same Disp and the same mean
+    (bit for bit) as the direct gather in particle order."""
     p = clustered_particles(n, ns, BOX, seed=n + 1)
     print("n, ns =", n, ns, flush=True)
     g = M.MGPU(n, ns, BOX, OM)
@@ -106,11 +105,7 @@ def test_tiled_gather_equals_direct_gather(n, ns):
     s_direct = g.mesh_to_particles()
     d_direct = g.get_disp()
     assert np.array_equal(d_tiled, d_direct)
-    scale = np.abs(d_direct.astype(np.float64)).mean()
-    assert np.abs(s_tiled - s_direct).max() <= 1e-13 * scale
-    g.set_particles(p)
-    s_direct2 = g.mesh_to_particles()
-    assert np.array_equal(s_direct2, s_direct)  # each form is deterministic
+    assert np.array_equal(s_tiled, s_direct)
     c = O.cfg(n, ns, BOX, OM)
     d, _, _ = oracle_density(c, p)
     rdisp, _ = O.gather(c, p, O.forces(c, d, None))
