@@ -120,7 +120,7 @@ def gpu_bench(steps, warmup, workload):
                 steps_per_run=nsteps)
 
 
-def learned_leg(np_eval=512, box_eval=1024.0, nsteps=30, np_train=128, box_train=256.0, iterations=14):
+def learned_leg(np_eval=512, box_eval=1024.0, nsteps=30, np_train=128, box_train=256.0, iterations=20):
     """The metric's second part: the learned Green's-function multiplier on the
     benchmark's model (f(R) fR0 = 1e-5, scale-dependent, screening, 30 steps).
     A network is trained on the GPU on coarse (Nmesh = Np) / 3x-fine pairs at
