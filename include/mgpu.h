@@ -283,6 +283,19 @@ int mgpu_mlp_train(mgpu_mlp *m, const float *x, const float *t, const float *w, 
 int mgpu_set_learned_multiplier(mgpu_ctx *ctx, mgpu_mlp *m, double a, double theta);
 int mgpu_density_feature(mgpu_ctx *ctx, double *s); /* s of the last deposit */
 
+/* ---- FFT schedule (no counterpart in the reference: FFTW plans its own) ----
+ * The strided column passes work on tiles of 8 or 16 columns; the width changes
+ * no column's arithmetic (results are identical), only speed.  which: 0 the
+ * Nmesh transforms (and every pass fused with them), 1 the Nsample lattice
+ * transforms of the scale-dependent LPT fields. */
+int mgpu_set_fft_tile(mgpu_ctx *ctx, int which, int tcols); /* 16 only where the tile fits a workgroup */
+int mgpu_get_fft_tile(mgpu_ctx *ctx, int which, int *tcols);
+/* Measure both widths on this device (forward + inverse y pass, best of three)
+ * and keep the faster, for both plans.  It uses the force meshes and the
+ * lattice spectra as scratch: call it before the initial conditions.
+ * ms: 4 doubles (main 8, main 16, lattice 8, lattice 16; -1 where not run). */
+int mgpu_autotune_fft(mgpu_ctx *ctx, double ms[4]);
+
 /* ---- per-kernel HIP-event timing (no counterpart in the reference) ----
  * When enabled, every launch of the listed kernel classes is bracketed by HIP
  * events on the context's stream; mgpu_kernel_timing synchronises and reports

@@ -466,6 +466,39 @@ int mgpu_density_feature(mgpu_ctx *ctx, double *s) {
   return MGPU_OK;
 }
 
+static FFTPlan *plan_of(mgpu_ctx *ctx, int which) {
+  if (which == 0) return &ctx->fft;
+  if (which == 1 && ctx->fft_lattice.n > 0) return &ctx->fft_lattice;
+  return nullptr;
+}
+
+int mgpu_set_fft_tile(mgpu_ctx *ctx, int which, int tcols) {
+  if (check_ctx(ctx)) return MGPU_E_INVALID;
+  FFTPlan *pl = plan_of(ctx, which);
+  if (!pl) return mgpu_fail(ctx, MGPU_E_INVALID, "no such FFT plan");
+  if (tcols != 8 && tcols != 16) return mgpu_fail(ctx, MGPU_E_INVALID, "tile width must be 8 or 16");
+  if (tcols == 16 && !fft_tile16_fits(ctx, pl->n))
+    return mgpu_fail(ctx, MGPU_E_INVALID, "16-column tiles do not fit a workgroup at this size");
+  pl->tcols = tcols;
+  return MGPU_OK;
+}
+
+int mgpu_get_fft_tile(mgpu_ctx *ctx, int which, int *tcols) {
+  if (check_ctx(ctx) || !tcols) return MGPU_E_INVALID;
+  FFTPlan *pl = plan_of(ctx, which);
+  if (!pl) return mgpu_fail(ctx, MGPU_E_INVALID, "no such FFT plan");
+  *tcols = pl->tcols;
+  return MGPU_OK;
+}
+
+int mgpu_autotune_fft(mgpu_ctx *ctx, double ms[4]) {
+  if (check_ctx(ctx) || !ms) return MGPU_E_INVALID;
+  ms[0] = ms[1] = ms[2] = ms[3] = -1.0;
+  int rc = fft_autotune(ctx, &ctx->fft, ctx->slot[3], ms);
+  if (!rc && ctx->fft_lattice.n > 0 && ctx->lattice_slot) rc = fft_autotune(ctx, &ctx->fft_lattice, ctx->lattice_slot, ms + 2);
+  return rc;
+}
+
 int mgpu_power_spectrum(mgpu_ctx *ctx, mgpu_pofk *pk) {
   if (check_ctx(ctx) || !pk) return MGPU_E_INVALID;
   if (!ctx->have_density) return mgpu_fail(ctx, MGPU_E_STATE, "no density(k): call mgpu_pm_density first");

@@ -184,13 +184,17 @@ bool fft_size_supported(int n) {
   }
 }
 
+// 16-column tiles fit one workgroup's LDS and thread limit
+bool fft_tile16_fits(const mgpu_ctx *ctx, int n) {
+  const size_t need16 = (size_t)n * 16 * sizeof(float2);
+  return need16 <= ctx->lds_max && n * 16 / (n % 3 == 0 ? 24 : 16) <= 1024;
+}
+
 int fft_plan_init(mgpu_ctx *ctx, FFTPlan *plan, int n, int tcols) {
   if (!fft_size_supported(n)) return mgpu_fail(ctx, MGPU_E_INVALID, "unsupported FFT size " + std::to_string(n));
   plan->n = n;
   plan->hp = ((n / 2 + 1) + 15) / 16 * 16;
-  // 16-column tiles only where requested and the tile fits one workgroup's LDS and thread limit
-  size_t need16 = (size_t)n * 16 * sizeof(float2);
-  plan->tcols = (tcols == 16 && need16 <= ctx->lds_max && n * 16 / (n % 3 == 0 ? 24 : 16) <= 1024) ? 16 : 8;
+  plan->tcols = (tcols == 16 && fft_tile16_fits(ctx, n)) ? 16 : 8;
   std::vector<float2> h(n);
   for (int m = 0; m < n; m++) {
     long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)m / (long double)n;
@@ -198,6 +202,41 @@ int fft_plan_init(mgpu_ctx *ctx, FFTPlan *plan, int n, int tcols) {
   }
   MGPU_HIP(ctx, hipMalloc(&plan->tw, sizeof(float2) * n));
   MGPU_HIP(ctx, hipMemcpy(plan->tw, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+  return MGPU_OK;
+}
+
+// The column passes' tile width measured on this device: the forward and the
+// inverse y pass over buf (a scratch mesh of the plan's size) with each width
+// that fits, one warm-up and the best of three; the faster becomes
+// plan->tcols.  The width changes no column's arithmetic, only the schedule.
+int fft_autotune(mgpu_ctx *ctx, FFTPlan *plan, float2 *buf, double ms[2]) {
+  const bool can16 = fft_tile16_fits(ctx, plan->n);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    if (e0) (void)hipEventDestroy(e0);
+    return mgpu_fail(ctx, MGPU_E_HIP, "event creation failed");
+  }
+  const int widths[2] = {8, 16};
+  int rc = MGPU_OK;
+  for (int w = 0; w < 2 && !rc; w++) {
+    ms[w] = -1.0;
+    if (widths[w] == 16 && !can16) continue;
+    plan->tcols = widths[w];
+    for (int rep = 0; rep < 4 && !rc; rep++) {
+      if (hipEventRecord(e0, ctx->stream) != hipSuccess) rc = MGPU_E_HIP;
+      if (!rc) rc = fft_pass_col(ctx, plan, buf, 1, 1);
+      if (!rc) rc = fft_pass_col(ctx, plan, buf, 1, 0);
+      float t = 0.f;
+      if (!rc && (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                  hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+        rc = MGPU_E_HIP;
+      if (!rc && rep > 0 && (ms[w] < 0 || t < ms[w])) ms[w] = t;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc == MGPU_E_HIP ? mgpu_fail(ctx, rc, "FFT autotune timing failed") : rc;
+  plan->tcols = (ms[1] >= 0 && ms[1] <= ms[0]) ? 16 : 8;
   return MGPU_OK;
 }
 

@@ -159,6 +159,8 @@ struct PhaseTimer {
 // ---------------------------------------------------------------- FFT (fft.hip)
 int fft_plan_init(mgpu_ctx *ctx, FFTPlan *plan, int n, int tcols = 16);
 void fft_plan_free(FFTPlan *plan);
+bool fft_tile16_fits(const mgpu_ctx *ctx, int n);
+int fft_autotune(mgpu_ctx *ctx, FFTPlan *plan, float2 *buf, double ms[2]);
 bool fft_size_supported(int n);
 // in-place 3D transforms on a slot with the padded layout above
 // in != nullptr: the real input is read from in (same padded layout) and buf only written

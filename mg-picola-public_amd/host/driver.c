@@ -346,6 +346,18 @@ mgp_sim *mgp_sim_create(mgp_params *p, int verbose, char *err, size_t errlen) {
     snprintf(s->err, s->errlen, "mgpu_create failed: %s", mgpu_last_error(NULL));
     goto fail;
   }
+  {
+    /* the column passes' tile width measured on this device, before any data */
+    double ms[4];
+    if (mgpu_autotune_fft(s->g, ms)) {
+      snprintf(s->err, s->errlen, "FFT autotune failed: %s", mgpu_last_error(s->g));
+      goto fail;
+    }
+    int t0 = 0;
+    mgpu_get_fft_tile(s->g, 0, &t0);
+    if (verbose)
+      printf("[mgpicola] FFT tiles: %d columns (8: %.3f ms, 16: %.3f ms per y pass pair)\n", t0, ms[0], ms[1]);
+  }
   s->pk = calloc(s->nbins, sizeof(double));
   s->pkk = calloc(s->nbins, sizeof(double));
   s->pkn = calloc(s->nbins, sizeof(double));

@@ -24,7 +24,8 @@ MGPU_SYMBOLS = [
     "mgpu_pm_density", "mgpu_pm_deposit", "mgpu_pm_transform", "mgpu_power_spectrum", "mgpu_rsd_power_spectrum",
     "mgpu_set_neutrinos", "mgpu_neutrino_response", "mgpu_growth_tables", "mgpu_mlp_create", "mgpu_mlp_destroy",
     "mgpu_mlp_get_params", "mgpu_mlp_set_params", "mgpu_mlp_forward", "mgpu_mlp_gradient", "mgpu_mlp_train",
-    "mgpu_set_learned_multiplier", "mgpu_density_feature", "mgpu_kick_drift", "mgpu_fifth_force", "mgpu_forces", "mgpu_mesh_to_particles",
+    "mgpu_set_learned_multiplier", "mgpu_density_feature", "mgpu_kick_drift", "mgpu_set_fft_tile",
+    "mgpu_get_fft_tile", "mgpu_autotune_fft", "mgpu_fifth_force", "mgpu_forces", "mgpu_mesh_to_particles",
     "mgpu_get_displacements", "mgpu_kick", "mgpu_drift", "mgpu_ic_displacement_fields", "mgpu_init_particles",
     "mgpu_lpt_fields", "mgpu_get_mesh", "mgpu_set_mesh", "mgpu_get_timers", "mgpu_enable_timers",
     "mgpu_kernel_name", "mgpu_enable_kernel_timing", "mgpu_kernel_timing", "mgpu_lpt_fields_slices",
@@ -254,6 +255,22 @@ class MGPU:
         s = C.c_double()
         self._ck(lib().mgpu_density_feature(self.ctx, C.byref(s)), "density_feature")
         return s.value
+
+    def set_fft_tile(self, which, tcols):
+        lib().mgpu_set_fft_tile.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        self._ck(lib().mgpu_set_fft_tile(self.ctx, which, tcols), "set_fft_tile")
+
+    def fft_tile(self, which=0):
+        lib().mgpu_get_fft_tile.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+        t = C.c_int()
+        self._ck(lib().mgpu_get_fft_tile(self.ctx, which, C.byref(t)), "get_fft_tile")
+        return t.value
+
+    def autotune_fft(self):
+        lib().mgpu_autotune_fft.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        ms = (C.c_double * 4)()
+        self._ck(lib().mgpu_autotune_fft(self.ctx, ms), "autotune_fft")
+        return list(ms)
 
     def kick_drift(self, dda, A, s1k, s2k, sumD, dyyy, s1d, s2d, sumV):
         """Kick then Drift in one pass; sumV in: mean velocity before the kick; returns the mean after."""

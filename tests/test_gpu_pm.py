@@ -438,3 +438,33 @@ def test_fused_kick_drift_equals_separate_passes(pos, sd):
     with pytest.raises(M.MGPUError):  # Disp was not reduced by sumD: a further kick needs a new gather
         g.kick(dda, A, s1k, s2k, [0.0, 0.0, 0.0])
     g.close()
+
+
+@pytest.mark.parametrize("n,ns,sd", [(32, 16, 0), (48, 16, 1), (64, 32, 1)])
+def test_fft_tile_width_changes_no_result(n, ns, sd):
+    """Tiles of 8 and 16 columns give bit-identical density, forces and Disp; the
+    autotuner times both (where 16 fits) and keeps one of them."""
+    p = clustered_particles(n, ns, BOX, seed=n)
+    out = []
+    for t in (8, 16):
+        g = M.MGPU(n, ns, BOX, OM, scale_dependent=sd)
+        g.set_fft_tile(0, t)
+        assert g.fft_tile(0) == t
+        g.set_particles(p)
+        g.pm_density()
+        g.forces()
+        g.mesh_to_particles()
+        out.append([g.get_mesh(M.MESH_DENSITY_K)] + [g.get_mesh(q) for q in (M.MESH_FORCE_X, M.MESH_FORCE_Y,
+                                                                             M.MESH_FORCE_Z)] + [g.get_disp()])
+        g.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+    g = M.MGPU(n, ns, BOX, OM, scale_dependent=sd)
+    ms = g.autotune_fft()
+    assert ms[0] > 0 and ms[1] > 0  # both widths fit at these sizes
+    assert g.fft_tile(0) == (16 if ms[1] <= ms[0] else 8)
+    if sd:
+        assert ms[2] > 0 and g.fft_tile(1) in (8, 16)
+    with pytest.raises(M.MGPUError):
+        g.set_fft_tile(0, 12)
+    g.close()
