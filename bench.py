@@ -21,7 +21,9 @@ timed on the host cores for one step of the same path on a bounded sample.
 The "learned" object is the metric's second part: the P(k) error at z = 0 for
 k < 1 h/Mpc of a coarse run (Nmesh = Nsample = 512) with the learned
 Green's-function multiplier against the plain run on the 3x finer mesh (1536),
-beside the plain coarse run's error and the time per step of each run.
+beside the plain coarse run's error and the time per step of each run.  The
+"ensemble" object times a slice of the batched nDGP ensemble run one
+realization after the other and concurrently (one host thread and stream each).
 """
 import argparse
 import json
@@ -201,6 +203,43 @@ def learned_leg(np_eval=512, box_eval=1024.0, nsteps=30, np_train=128, box_train
             "train_history": [[h[0], round(h[2], 4)] for h in hist]}
 
 
+def ensemble_leg(per_size=4, threads=12, nsteps=20):
+    """A slice of the batched nDGP ensemble (rc H0 = 1, density screening,
+    Box = 2 Np, Nmesh = 2 Np, 20 steps): per_size realizations at each of 96^3,
+    128^3 and 192^3 particles, in fixed-amplitude pairs (delta and -delta),
+    run one after the other and then concurrently on `threads` host threads
+    (one libmgpu context and stream each).  Whole-ensemble wall time, set-up
+    and initial conditions included, and the particle-steps/s it gives."""
+    import ensemble as E
+    import mgpicola as M
+    import workloads as W
+
+    out = tempfile.mkdtemp(prefix="mgpicola_ens_")
+    hps = []
+    for name in ("dgp_96_192", "dgp_128_256", "dgp_192_384"):
+        for k in range(per_size):
+            d = os.path.join(out, f"{name}_{k}")
+            os.makedirs(d, exist_ok=True)
+            model, vals, _ = W.values_for(name, d, Seed=7000 + 10 * len(hps) // 2, amplitude_fixed_initial_condition=1,
+                                          inverted_initial_condition=k % 2)
+            with open(vals["OutputRedshiftFile"], "w") as f:
+                f.write(f"0, {nsteps}\n")
+            pf = os.path.join(d, "params.txt")
+            M.write_paramfile(pf, vals, model=model)
+            hps.append(M.read_params(pf, model=model))
+    work = sum(h.Nsample ** 3 for h in hps) * nsteps
+    seq, t_seq = E.run_ensemble([M.HostParams.from_buffer_copy(h) for h in hps], threads=1)
+    print(f"[bench] ensemble: {len(hps)} runs one after the other {t_seq:.2f} s", file=sys.stderr, flush=True)
+    con, t_con = E.run_ensemble([M.HostParams.from_buffer_copy(h) for h in hps], threads=threads)
+    print(f"[bench] ensemble: concurrently on {threads} threads {t_con:.2f} s", file=sys.stderr, flush=True)
+    return {"realizations": len(hps), "steps": nsteps, "threads": threads,
+            "seconds_sequential": round(t_seq, 3), "seconds_concurrent": round(t_con, 3),
+            "particle_steps_per_s_sequential": round(work / t_seq, 1),
+            "particle_steps_per_s_concurrent": round(work / t_con, 1), "speedup": round(t_seq / t_con, 2),
+            "workload": f"nDGP rcH0=1 + screening, {per_size} each of 96^3/192, 128^3/256, 192^3/384 "
+                        f"(Box = 2 Np), fixed-amplitude pairs, {nsteps} steps; wall time incl. set-up and ICs"}
+
+
 def cpu_baseline(ns=256, nmesh=512, box=512.0):
     """One COLA step of the same f(R) + screening + scale-dependent path in the
     FP64 oracle: deposit, r2c, P(k), fifth force (2 FFTs), Forces (3 c2r),
@@ -275,6 +314,7 @@ def main():
     ap.add_argument("--workload", default="fofr_sd_512_1024")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-multiplier P(k) error leg")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the concurrent-ensemble leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy this many MiB device to device (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -306,6 +346,12 @@ def main():
             lrn = learned_leg()
         except Exception as e:  # the throughput line still prints; the failure is reported in it
             lrn = {"error": f"{type(e).__name__}: {e}"}
+    ens = None
+    if not args.no_ensemble:
+        try:
+            ens = ensemble_leg()
+        except Exception as e:  # the throughput line still prints; the failure is reported in it
+            ens = {"error": f"{type(e).__name__}: {e}"}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline()
@@ -316,7 +362,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": args.workload, "particles": r["npart"], "nmesh": r["nmesh"], "box": r["box"],
                    "model": "f(R) Hu-Sawicki fR0=1e-5 n=1, scale-dependent, screening, P(k) every step"},
-        "roofline": roof, "cpu_baseline": cpu, "learned": lrn, "kernels_per_step": per_kernel,
+        "roofline": roof, "cpu_baseline": cpu, "learned": lrn, "ensemble": ens, "kernels_per_step": per_kernel,
     }
     print(json.dumps(line), flush=True)
 

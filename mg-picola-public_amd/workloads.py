@@ -30,6 +30,14 @@ WORKLOADS = {
     # linear bins: 512^3 particles, 1024^3 mesh, Box 1024 Mpc/h, 30 steps
     "fofr_sd_512_1024": ("FOFR", dict(Nmesh=1024, Nsample=512, Box=1024.0, modified_gravity_active=1,
                                       include_screening=1, pofk_compute_every_step=1, pofk_nbins=512), 30),
+    # nDGP (rc H0 = 1) with density screening: the realizations of the batched ensemble,
+    # Box = 2 Np Mpc/h, Nmesh = 2 Np, 20 steps (sizes 96, 128 and 192 particles per side)
+    "dgp_96_192": ("DGP", dict(Nmesh=192, Nsample=96, Box=192.0, modified_gravity_active=1, include_screening=1,
+                               Rsmooth=1.0, rcH0_DGP=1.0), 20),
+    "dgp_128_256": ("DGP", dict(Nmesh=256, Nsample=128, Box=256.0, modified_gravity_active=1, include_screening=1,
+                                Rsmooth=1.0, rcH0_DGP=1.0), 20),
+    "dgp_192_384": ("DGP", dict(Nmesh=384, Nsample=192, Box=384.0, modified_gravity_active=1, include_screening=1,
+                                Rsmooth=1.0, rcH0_DGP=1.0), 20),
     # the same run on the coarse mesh of the learned multiplier (Nmesh = Nsample = 512)
     "fofr_sd_512_512": ("FOFR", dict(Nmesh=512, Nsample=512, Box=1024.0, modified_gravity_active=1,
                                      include_screening=1, pofk_compute_every_step=1, pofk_nbins=256), 30),
