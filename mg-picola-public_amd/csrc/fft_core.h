@@ -255,11 +255,17 @@ constexpr int col_ept(int L, int T) { return elems_per_thread(L); }
 constexpr int col_threads(int L, int T) { return L * T / col_ept(L, T); }
 
 // ---------------------------------------------------------------- contiguous z pass
+// threads per block of the z passes (rows per block = this / (N/2) x elements per thread):
+// one wave per block measured fastest at 1024^3 (r2c 1.99 -> 1.79 ms, screening pass
+// 4.32 -> 3.65 ms against 256 threads)
+#ifndef MGPU_ZROW_THREADS
+#define MGPU_ZROW_THREADS 64
+#endif
 template <int N>
 struct RowCfg {
   static constexpr int M = N / 2;
   static constexpr int E = elems_per_thread(M);
-  static constexpr int RB = 256 * E / M > 0 ? 256 * E / M : 1;  // rows per block
+  static constexpr int RB = MGPU_ZROW_THREADS * E / M > 0 ? MGPU_ZROW_THREADS * E / M : 1;  // rows per block
   static constexpr int NT = M * RB / E;
   static constexpr int LS = M + 1;  // LDS row stride
   static_assert(NT <= 1024, "row tile too large");
