@@ -29,6 +29,16 @@ namespace {
 
 constexpr double kFix = 4294967296.0;  // 2^32
 constexpr int kTB = 256;
+// threads per block of the tile deposit (a multiple of 64, at least 64) and of the tiled gather
+#ifndef MGPU_DEPTILE_THREADS
+#define MGPU_DEPTILE_THREADS 256
+#endif
+#ifndef MGPU_GATHER_THREADS
+#define MGPU_GATHER_THREADS 512
+#endif
+constexpr int kDepTileNT = MGPU_DEPTILE_THREADS, kGatherNT = MGPU_GATHER_THREADS;
+static_assert(kDepTileNT % 64 == 0 && kDepTileNT >= 64 && kDepTileNT <= 1024, "deposit block size");
+static_assert(kGatherNT % 64 == 0 && kGatherNT >= 64 && kGatherNT <= 1024, "gather block size");
 
 constexpr double kInvFix = 1.0 / 4294967296.0;
 
@@ -178,7 +188,7 @@ __global__ __launch_bounds__(kTB) void k_dep_scatter(const unsigned *__restrict_
 // local coordinate -1 (s = 1) or T (s = 0) fall outside the tile and are
 // skipped; every corner is added by exactly one tile.
 template <bool RSD>
-__global__ __launch_bounds__(kTB) void k_dep_tile(const float4 *__restrict__ rec, const unsigned *__restrict__ off,
+__global__ __launch_bounds__(kDepTileNT) void k_dep_tile(const float4 *__restrict__ rec, const unsigned *__restrict__ off,
                                                   TileGeom g, double scale, double wpar, int HP2,
                                                   float *__restrict__ d) {
   extern __shared__ unsigned long long blk[];
@@ -270,7 +280,7 @@ __global__ __launch_bounds__(kTB) void k_dep_tile(const float4 *__restrict__ rec
 // dealt to the 8 XCDs round robin), so the +1 halo a tile reads was mostly
 // fetched into the same L2 by its neighbour.
 template <int T>
-__global__ __launch_bounds__(512) void k_gather_tile(const float4 *__restrict__ rec, size_t np,
+__global__ __launch_bounds__(kGatherNT) void k_gather_tile(const float4 *__restrict__ rec, size_t np,
                                                      const unsigned *__restrict__ off, int N, int nt1, int HP2,
                                                      double scale, const float *__restrict__ g0,
                                                      const float *__restrict__ g1, const float *__restrict__ g2,
@@ -288,7 +298,7 @@ __global__ __launch_bounds__(512) void k_gather_tile(const float4 *__restrict__ 
   // rows (lx, ly) of each mesh: T contiguous cells, then the z = T column.
   // Every load of the thread is issued before the first LDS store, so the
   // fill costs one memory latency, not one per row group.
-  constexpr int NT = 512, KR = (B2 * T + NT - 1) / NT, KC = (B2 + NT - 1) / NT;
+  constexpr int NT = kGatherNT, KR = (B2 * T + NT - 1) / NT, KC = (B2 + NT - 1) / NT;
   const int gz = Z0 + T < N ? Z0 + T : 0;
   float v[3][KR + KC];
 #pragma unroll
@@ -425,7 +435,7 @@ int deposit_tiled(mgpu_ctx *ctx, float *d, const float4 *rsd) {
   {
     // compulsory: the particle records read once, delta written
     KScope ks(ctx, MGPU_K_DEPOSIT, 16.0 * np + 4.0 * cells);
-    hipLaunchKernelGGL(rsd ? k_dep_tile<true> : k_dep_tile<false>, dim3(nt), dim3(kTB), shmem, ctx->stream, rec, off,
+    hipLaunchKernelGGL(rsd ? k_dep_tile<true> : k_dep_tile<false>, dim3(nt), dim3(kDepTileNT), shmem, ctx->stream, rec, off,
                        g, scale, wpar, 2 * ctx->HP, d);
   }
   MGPU_HIP(ctx, hipGetLastError());
@@ -441,7 +451,7 @@ int gather_tiled(mgpu_ctx *ctx, const float *f0, const float *f1, const float *f
   const unsigned nt = (unsigned)nt1 * nt1 * nt1;
   const double scale = (double)N / ctx->p.box;
   auto kern = T == 16 ? k_gather_tile<16> : T == 8 ? k_gather_tile<8> : k_gather_tile<4>;
-  hipLaunchKernelGGL(kern, dim3(nt), dim3(512), 0, ctx->stream, ctx->dep_rec, ctx->np, ctx->dep_off, N, nt1,
+  hipLaunchKernelGGL(kern, dim3(nt), dim3(kGatherNT), 0, ctx->stream, ctx->dep_rec, ctx->np, ctx->dep_off, N, nt1,
                      2 * ctx->HP, scale, f0, f1, f2, ctx->disp);
   MGPU_HIP(ctx, hipGetLastError());
   return MGPU_OK;
