@@ -119,7 +119,10 @@ __global__ __launch_bounds__(RowCfg<N>::NT) void k_fft_z_c2r(const float2 *buf, 
 
 // tiles per block of the column passes: two (software-pipelined) for the
 // large power-of-two sizes, where the registers for the prefetch are there
-constexpr int col_tpb(int L) { return (L % 3 != 0 && L >= 256) ? 2 : 1; }
+#ifndef MGPU_COL_TPB
+#define MGPU_COL_TPB 2
+#endif
+constexpr int col_tpb(int L) { return (L % 3 != 0 && L >= 256) ? MGPU_COL_TPB : 1; }
 
 template <int L, int T, int SIGN>
 int launch_col(mgpu_ctx *ctx, float2 *buf, size_t line_stride, size_t outer_stride, int nouter, int ntiles,
