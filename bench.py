@@ -133,7 +133,8 @@ def learned_leg(np_eval=512, box_eval=1024.0, nsteps=30, np_train=128, box_train
     k < 1 h/Mpc at z = 0 (64 linear bins up to the coarse Nyquist k, shot noise
     subtracted) and their time per step (steps 2-30, as in the throughput
     measurement: the first step carries one-time costs and the output with its
-    snapshot write is not a step)."""
+    snapshot write is not a step); the same network's errors on the LCDM run of
+    the same initial conditions go in "lcdm"."""
     import math
 
     import learned as LR
@@ -190,10 +191,17 @@ def learned_leg(np_eval=512, box_eval=1024.0, nsteps=30, np_train=128, box_train
     hc = params("eval_coarse", np_eval, np_eval, box_eval, 5001, 1e-5)
     plain, ms_plain = timed_run(hc)
     lrn, ms_lrn = timed_run(hc, mlp)
-    mlp.close()
     e_plain, e_lrn = LR.pofk_error(fine, plain)[0], LR.pofk_error(fine, lrn)[0]
     print(f"[bench] learned: P(k) error plain {e_plain:.4f}, learned {e_lrn:.4f}", file=sys.stderr, flush=True)
+    # the same network on the LCDM run of the same initial conditions
+    fine_l, _ = timed_run(params("eval_fine_lcdm", np_eval, 3 * np_eval, box_eval, 5001, 0.0))
+    hcl = params("eval_coarse_lcdm", np_eval, np_eval, box_eval, 5001, 0.0)
+    el_plain = LR.pofk_error(fine_l, timed_run(hcl)[0])[0]
+    el_lrn = LR.pofk_error(fine_l, timed_run(hcl, mlp)[0])[0]
+    print(f"[bench] learned: LCDM P(k) error plain {el_plain:.4f}, learned {el_lrn:.4f}", file=sys.stderr, flush=True)
+    mlp.close()
     return {"pofk_err_learned": round(e_lrn, 4), "pofk_err_plain_coarse": round(e_plain, 4), "kmax": 1.0,
+            "lcdm": {"pofk_err_learned": round(el_lrn, 4), "pofk_err_plain_coarse": round(el_plain, 4)},
             "ms_per_step_fine": round(ms_fine, 2), "ms_per_step_coarse_learned": round(ms_lrn, 2),
             "ms_per_step_coarse_plain": round(ms_plain, 2), "speedup_vs_fine": round(ms_fine / ms_lrn, 2),
             "eval": f"{np_eval}^3 particles, Box {box_eval:g}, coarse Nmesh {np_eval} vs fine {3 * np_eval}, "
