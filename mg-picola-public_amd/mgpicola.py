@@ -334,9 +334,9 @@ class MGPU:
         return out.view(np.complex128).reshape(n, n, n // 2 + 1)
 
     def set_mesh(self, which, arr):
-        a = np.ascontiguousarray(arr)
+        a = np.asarray(arr)
         if np.iscomplexobj(a):
-            a = a.astype(np.complex128).view(np.float64)
+            a = np.ascontiguousarray(a, np.complex128).view(np.float64)
         a = np.ascontiguousarray(a, np.float64).ravel()
         self._ck(lib().mgpu_set_mesh(self.ctx, which, _ptr(a)), "set_mesh")
 

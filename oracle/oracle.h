@@ -19,6 +19,19 @@
 #ifndef MGPICOLA_ORACLE_H
 #define MGPICOLA_ORACLE_H
 
+/* Planted mistakes, for the tests that show each pin catches a plausible slip
+ * (tests/test_oracle_planted.py builds one library per number): 0 = none (the
+ * default build); otherwise one line of the restatement is wrong on purpose.
+ *   1 force Nyquist index -N/2 instead of +N/2      7 IC: the skipped k = 0 mode draws nothing
+ *   2 2LPT factor +3/7 instead of -3/7              8 Kick: COLA term with + sign
+ *   3 P(k): kz > 0 modes weighted 1 instead of 2    9 Drift: 2LPT term growth from order 1
+ *   4 deposit: ghost slab not folded onto x = 0    10 LPT field: growth index not mirrored
+ *   5 f(R) mass term without a^2                   11 seed table: two shell loops swapped
+ *   6 IC: delta ~ P instead of sqrt(P)                                                    */
+#ifndef OR_PLANT
+#define OR_PLANT 0
+#endif
+
 #include <stddef.h>
 
 #ifdef __cplusplus

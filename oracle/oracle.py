@@ -47,40 +47,74 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        L = C.CDLL(LIB)
-        L.or_real_len.restype = C.c_size_t
-        L.or_real_len.argtypes = [C.c_int]
-        L.or_fft_c2c_1d.argtypes = [C.c_int, _D, C.c_int]
-        L.or_fft_r2c_3d.argtypes = [C.c_int, _D]
-        L.or_fft_c2r_3d.argtypes = [C.c_int, _D]
-        L.or_pm_deposit.argtypes = [C.POINTER(Cfg), C.c_size_t, _P3, _D]
-        L.or_pofk.argtypes = [C.POINTER(Cfg), _D, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _D, _D, _D]
-        L.or_adjust_pofk.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
-                                     C.POINTER(C.c_double), C.POINTER(C.c_double)]
-        L.or_pm_deposit_rsd.argtypes = [C.POINTER(Cfg), C.c_size_t, _P3, _P3, _P3, _P3, C.c_int, C.c_double, C.c_int,
-                                        C.c_int, C.c_double, C.c_double, _D]
-        L.or_rsd_multipoles.argtypes = [C.POINTER(Cfg), _D, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _D,
-                                        _D, _D, _D, _D]
-        L.or_mg_fifth_force.restype = C.c_int
-        L.or_mg_fifth_force.argtypes = [C.POINTER(Cfg), C.POINTER(MG), _D, _D, _D]
-        L.or_pm_forces.argtypes = [C.POINTER(Cfg), _D, C.c_void_p, _D, _D, _D]
-        L.or_pm_gather.argtypes = [C.POINTER(Cfg), C.c_size_t, _P3, _D, _D, _D, _P3, C.POINTER(C.c_double)]
-        L.or_kick.argtypes = [C.c_size_t, _P3, _P3, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_int,
-                              C.c_double, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double),
-                              C.POINTER(C.c_double)]
-        L.or_drift.argtypes = [C.c_size_t, _P3, _P3, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_int,
-                               C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double)]
-        L.or_ranlxd_seed.argtypes = [C.POINTER(Ranlxd), C.c_ulong]
-        L.or_ranlxd_uniform.restype = C.c_double
-        L.or_ranlxd_uniform.argtypes = [C.POINTER(Ranlxd)]
-        L.or_ranlxd_get.restype = C.c_ulong
-        L.or_ranlxd_get.argtypes = [C.POINTER(Ranlxd)]
-        L.or_seedtable.argtypes = [C.c_int, C.c_ulong, _U]
-        L.or_ic_delta.argtypes = [C.POINTER(Cfg), C.c_ulong, C.c_int, C.c_int, C.c_int, _D, _D]
-        L.or_ic_displacements.argtypes = [C.POINTER(Cfg), _D, C.c_int, _P3, _P3, C.c_void_p]
-        L.or_lpt_field.argtypes = [C.POINTER(Cfg), _D, _D, _P3]
-        _lib = L
+        _lib = _load(LIB)
     return _lib
+
+
+def build_planted(plant, outdir):
+    """Compile a copy of the oracle with planted mistake number `plant` (see
+    oracle.h) into outdir; returns the library path."""
+    path = os.path.join(outdir, f"liboracle_plant{plant}.so")
+    src = [os.path.join(HERE, f) for f in ("oracle_fft.c", "oracle_pm.c", "oracle_ic.c")]
+    subprocess.check_call(["gcc", "-O2", "-fPIC", "-fopenmp", "-std=gnu11", f"-DOR_PLANT={plant}", "-shared",
+                           "-o", path] + src + ["-lm"])
+    return path
+
+
+class using:
+    """Context manager: route every call of this module to another build of the
+    oracle (a planted-mistake library) and restore the normal one afterwards."""
+
+    def __init__(self, path):
+        self.path = path
+
+    def __enter__(self):
+        global _lib
+        lib()
+        self.saved = _lib
+        _lib = _load(self.path)
+        return _lib
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self.saved
+        return False
+
+
+def _load(path):
+    L = C.CDLL(path)
+    L.or_real_len.restype = C.c_size_t
+    L.or_real_len.argtypes = [C.c_int]
+    L.or_fft_c2c_1d.argtypes = [C.c_int, _D, C.c_int]
+    L.or_fft_r2c_3d.argtypes = [C.c_int, _D]
+    L.or_fft_c2r_3d.argtypes = [C.c_int, _D]
+    L.or_pm_deposit.argtypes = [C.POINTER(Cfg), C.c_size_t, _P3, _D]
+    L.or_pofk.argtypes = [C.POINTER(Cfg), _D, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _D, _D, _D]
+    L.or_adjust_pofk.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                 C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.or_pm_deposit_rsd.argtypes = [C.POINTER(Cfg), C.c_size_t, _P3, _P3, _P3, _P3, C.c_int, C.c_double, C.c_int,
+                                    C.c_int, C.c_double, C.c_double, _D]
+    L.or_rsd_multipoles.argtypes = [C.POINTER(Cfg), _D, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _D,
+                                    _D, _D, _D, _D]
+    L.or_mg_fifth_force.restype = C.c_int
+    L.or_mg_fifth_force.argtypes = [C.POINTER(Cfg), C.POINTER(MG), _D, _D, _D]
+    L.or_pm_forces.argtypes = [C.POINTER(Cfg), _D, C.c_void_p, _D, _D, _D]
+    L.or_pm_gather.argtypes = [C.POINTER(Cfg), C.c_size_t, _P3, _D, _D, _D, _P3, C.POINTER(C.c_double)]
+    L.or_kick.argtypes = [C.c_size_t, _P3, _P3, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_int,
+                          C.c_double, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double),
+                          C.POINTER(C.c_double)]
+    L.or_drift.argtypes = [C.c_size_t, _P3, _P3, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_int,
+                           C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double)]
+    L.or_ranlxd_seed.argtypes = [C.POINTER(Ranlxd), C.c_ulong]
+    L.or_ranlxd_uniform.restype = C.c_double
+    L.or_ranlxd_uniform.argtypes = [C.POINTER(Ranlxd)]
+    L.or_ranlxd_get.restype = C.c_ulong
+    L.or_ranlxd_get.argtypes = [C.POINTER(Ranlxd)]
+    L.or_seedtable.argtypes = [C.c_int, C.c_ulong, _U]
+    L.or_ic_delta.argtypes = [C.POINTER(Cfg), C.c_ulong, C.c_int, C.c_int, C.c_int, _D, _D]
+    L.or_ic_displacements.argtypes = [C.POINTER(Cfg), _D, C.c_int, _P3, _P3, C.c_void_p]
+    L.or_lpt_field.argtypes = [C.POINTER(Cfg), _D, _D, _P3]
+    return L
 
 
 def cfg(nmesh, nsample, box, omega, model=LCDM, mg_active=0, include_screening=0, filter=FILTER_NONE):

@@ -82,8 +82,13 @@ void or_seedtable(int N, unsigned long seed, unsigned int *t) {
     for (int j = 0; j < i + 1; j++) t[j * N + i] = DRAW;
     for (int j = 0; j < i; j++) t[(N - 1 - i) * N + j] = DRAW;
     for (int j = 0; j < i + 1; j++) t[(N - 1 - j) * N + i] = DRAW;
-    for (int j = 0; j < i; j++) t[i * N + (N - 1 - j)] = DRAW;
-    for (int j = 0; j < i + 1; j++) t[j * N + (N - 1 - i)] = DRAW;
+    if (OR_PLANT == 11) {
+      for (int j = 0; j < i + 1; j++) t[j * N + (N - 1 - i)] = DRAW;
+      for (int j = 0; j < i; j++) t[i * N + (N - 1 - j)] = DRAW;
+    } else {
+      for (int j = 0; j < i; j++) t[i * N + (N - 1 - j)] = DRAW;
+      for (int j = 0; j < i + 1; j++) t[j * N + (N - 1 - i)] = DRAW;
+    }
     for (int j = 0; j < i; j++) t[(N - 1 - i) * N + (N - 1 - j)] = DRAW;
     for (int j = 0; j < i + 1; j++) t[(N - 1 - j) * N + (N - 1 - i)] = DRAW;
   }
@@ -112,6 +117,7 @@ void or_ic_delta(const or_cfg *c, unsigned long seed, int sphere_mode, int amp_f
       or_ranlxd r;
       or_ranlxd_seed(&r, seedtable[i * N + j]);
       for (int k = 0; k < N / 2; k++) {
+        if (OR_PLANT == 7 && i == 0 && j == 0 && k == 0) continue;
         double phase = or_ranlxd_uniform(&r) * 2 * OR_PI, ampl = 0.0;
         if (!amp_fixed) {
           do { ampl = or_ranlxd_uniform(&r); } while (ampl == 0);
@@ -131,7 +137,7 @@ void or_ic_delta(const or_cfg *c, unsigned long seed, int sphere_mode, int amp_f
         int ii = i < N / 2 ? i : N - i, jj = j < N / 2 ? j : N - j;
         double p_of_k = pk_n2[ii * ii + jj * jj + k * k];
         if (!amp_fixed) p_of_k *= -log(ampl);
-        double delta = pow(box, -1.5) * sqrt(p_of_k);
+        double delta = pow(box, -1.5) * (OR_PLANT == 6 ? p_of_k : sqrt(p_of_k));
         if (inverted) delta *= -1.0;
         double re = delta * cos(phase), im = delta * sin(phase);
         if (k > 0) {
@@ -270,9 +276,10 @@ void or_ic_displacements(const or_cfg *c, const double *cd, int scale_dependent,
             double dis = lattice_read(N, Np, cdisp[a], n, m, p);
             double dis2 = lattice_read(N, Np, cdisp2[a], n, m, p) / nmesh3;
             za[a][q] = (float)dis;
-            lpt[a][q] = (float)(-3. / 7. * dis2);
+            const double f37 = OR_PLANT == 2 ? 3. / 7. : -3. / 7.;
+            lpt[a][q] = (float)(f37 * dis2);
             s1[a] += dis;
-            s2[a] += -3. / 7. * dis2;
+            s2[a] += f37 * dis2;
           }
         }
     for (int a = 0; a < 3; a++) {
@@ -303,6 +310,7 @@ void or_lpt_field(const or_cfg *c, const double *cd, const double *g_n2, float *
         double kv[3] = {kcomp(N, i, box), kcomp(N, j, box), kcomp(N, k, box)};
         double kmag2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
         int ii = i < N / 2 ? i : N - i, jj = j < N / 2 ? j : N - j, kk = k < N / 2 ? k : N - k;
+        if (OR_PLANT == 10) ii = i % (N / 2 + 1);
         double g = g_n2[ii * ii + jj * jj + kk * kk];
         for (int a = 0; a < 3; a++) {
           if (kmag2 > 0.0) {

@@ -49,7 +49,7 @@ void or_pm_deposit(const or_cfg *c, size_t np, const float *const pos[3], double
   /* ghost slab (x = N) onto slab 0, with the +1 of src/auxPM.c:353-355 */
   const size_t slice = (size_t)N * 2 * (N / 2 + 1);
   if (np != 0)
-    for (size_t i = 0; i < slice; i++) density[i] += density[(size_t)N * slice + i] + 1.0;
+    for (size_t i = 0; i < slice; i++) density[i] += (OR_PLANT == 4 ? 0.0 : density[(size_t)N * slice + i]) + 1.0;
 }
 
 /* ------------------------------------------------------------------------
@@ -120,14 +120,15 @@ void or_pofk(const or_cfg *c, const double *dk, int nbins, int bintype, double k
         if (nk >= 0 && nk < nbins) {
           double gc = 1.0 / pow(gx * gy * gz, 4.0) * norm;
           const double *v = dk + 2 * cidx(N, i, j, k);
-          pofk[nk] += 2.0 * (v[0] * v[0] + v[1] * v[1]) * gc;
-          kmean[nk] += 2.0 * kmag;
-          nmodes[nk] += 2.0;
+          const double w = OR_PLANT == 3 ? 1.0 : 2.0;
+          pofk[nk] += w * (v[0] * v[0] + v[1] * v[1]) * gc;
+          kmean[nk] += w * kmag;
+          nmodes[nk] += w;
           if (j != N / 2 && j != 0) {
             v = dk + 2 * cidx(N, i, N - j, k);
-            pofk[nk] += 2.0 * (v[0] * v[0] + v[1] * v[1]) * gc;
-            kmean[nk] += 2.0 * kmag;
-            nmodes[nk] += 2.0;
+            pofk[nk] += w * (v[0] * v[0] + v[1] * v[1]) * gc;
+            kmean[nk] += w * kmag;
+            nmodes[nk] += w;
           }
         }
       }
@@ -194,7 +195,7 @@ int or_mg_fifth_force(const or_cfg *c, const or_mg *m, double *dk, const double 
   }
   double *one = (double *)malloc(sizeof(double) * total);
   if (c->model == OR_FOFR || c->model == OR_MBETA) {
-    const double massterm2 = pow(m->a, 2) * m->mass2 / pow((2.0 * OR_PI) * OR_INVERSE_H0_MPCH / c->box, 2);
+    const double massterm2 = (OR_PLANT == 5 ? 1.0 : pow(m->a, 2)) * m->mass2 / pow((2.0 * OR_PI) * OR_INVERSE_H0_MPCH / c->box, 2);
     if (c->include_screening) {
       /* DivideByLaplacian, src/mg.h:21-64 */
       double nf = 1.0 / pow((double)N, 3);
@@ -272,8 +273,8 @@ void or_pm_forces(const or_cfg *c, double *dk, const double *phik, double *f0, d
           continue;
         }
         int dd[3];
-        dd[0] = i > N / 2 ? i - N : i;
-        dd[1] = j > N / 2 ? j - N : j;
+        dd[0] = (OR_PLANT == 1 ? i >= N / 2 : i > N / 2) ? i - N : i;
+        dd[1] = (OR_PLANT == 1 ? j >= N / 2 : j > N / 2) ? j - N : j;
         dd[2] = k;
         double RK = (double)(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]);
         double KK = -1.0 / RK;
@@ -350,7 +351,7 @@ void or_kick(size_t np, float *const vel[3], float *const disp[3], const float *
   for (size_t n = 0; n < np; n++)
     for (int a = 0; a < 3; a++) {
       disp[a][n] -= sumD[a];
-      double force = -1.5 * omega * disp[a][n] - use_cola * lpt_term(d1, d2, s1, s2, sd, a, n) / A;
+      double force = -1.5 * omega * disp[a][n] + (OR_PLANT == 8 ? 1 : -1) * use_cola * lpt_term(d1, d2, s1, s2, sd, a, n) / A;
       vel[a][n] += force * dda;
       s[a] += vel[a][n];
     }
@@ -372,7 +373,7 @@ void or_drift(size_t np, float *const pos[3], const float *const vel[3], const f
   for (size_t n = 0; n < np; n++)
     for (int a = 0; a < 3; a++) {
       pos[a][n] += (vel[a][n] - sumV[a]) * dyyy;
-      pos[a][n] = wrapf((float)(pos[a][n] + use_cola * lpt_term(d1, d2, s1, s2, sd, a, n)), box);
+      pos[a][n] = wrapf((float)(pos[a][n] + use_cola * lpt_term(d1, d2, s1, OR_PLANT == 9 ? s1 : s2, sd, a, n)), box);
     }
 }
 

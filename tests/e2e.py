@@ -23,8 +23,10 @@ def wrapf(x, box):
     return x.astype(np.float32)
 
 
-def oracle_run(hp, cosmo, nsteps):
-    """Non-scale-dependent COLA run z_init -> 0 in nsteps steps; returns (pos, vel, sumV)."""
+def oracle_run(hp, cosmo, nsteps, pofk=None):
+    """COLA run z_init -> 0 in nsteps steps; returns (pos, vel, sumV), and with
+    pofk = (nbins, bintype, subtract, kmin, kmax) also the binned P(k) of every
+    force evaluation as a list of (P, k_mean, modes)."""
     N, Ns, box = hp.Nmesh, hp.Nsample, hp.Box
     model = {M.MODELS["FOFR"]: O.FOFR, M.MODELS["FOFR_LCDM"]: O.FOFR, M.MODELS["DGP"]: O.DGP,
              M.MODELS["BRANSDICKE"]: O.BRANSDICKE, M.MODELS["MBETA"]: O.MBETA}[hp.model]
@@ -53,6 +55,7 @@ def oracle_run(hp, cosmo, nsteps):
     AI = AF = A
     da = (1.0 - A) / nsteps
     sV = np.zeros(3)
+    pks = []
     for i in range(nsteps + 1):
         final = i == nsteps
         AF = A if final else A + 0.5 * da
@@ -60,6 +63,9 @@ def oracle_run(hp, cosmo, nsteps):
         d = O.deposit(c, pos)
         dx = d.copy()
         O.fft_r2c_3d(N, d)
+        if pofk is not None:
+            nb, bt, sub, kmin, kmax = pofk
+            pks.append(O.pofk(c, d, nb, bt, kmin, kmax, sub))
         phik = None
         if hp.modified_gravity_active:
             m = cosmo.mg(A)
@@ -85,6 +91,8 @@ def oracle_run(hp, cosmo, nsteps):
                     1, sV)
         A, AI = AFF, AF
         Di, Di2 = cosmo.growth(M.G_D, A), cosmo.growth(M.G_D2, A)
+    if pofk is not None:
+        return pos, vel, sV, pks
     return pos, vel, sV
 
 

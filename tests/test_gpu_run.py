@@ -18,8 +18,9 @@ from helpers import rel_err
 pytestmark = pytest.mark.gpu
 
 
-def make_params(tmp_path, name, nsteps, **over):
-    model, vals, _ = W.values_for(name, str(tmp_path), **over)
+def make_params(tmp_path, name, nsteps, model=None, **over):
+    wmodel, vals, _ = W.values_for(name, str(tmp_path), **over)
+    model = model or wmodel
     with open(vals["OutputRedshiftFile"], "w") as f:
         f.write(f"0, {nsteps}\n")
     pf = os.path.join(str(tmp_path), "params.txt")
@@ -57,6 +58,86 @@ def test_end_to_end_matches_oracle(tmp_path, case):
         assert sum(f.startswith("pofk_" + case["name"]) for f in files) == case["nsteps"] + 1, files
 
 
+def run_gpu(hp, track=False):
+    sim = M.Sim(hp)
+    if track:
+        sim.set_learned(None, track=True)  # logs the P(k) of every force evaluation; no multiplier
+    sim.ic()
+    while not sim.finished:
+        sim.advance()
+    gpos, gvel = sim.gpu.get_particles()
+    log = sim.log() if track else None
+    sim.close()
+    return gpos, gvel, log
+
+
+@pytest.mark.parametrize("case", [
+    # nDGP rc H0 = 1 with density screening (Gaussian smoothing)
+    dict(name="dgp_96_192", model="DGP", nsteps=6, over=dict(Nmesh=64, Nsample=32, Box=64.0)),
+    # Jordan-Brans-Dicke (w = 100): G_eff(a) from the JBD background
+    dict(name="lcdm_64_128", model="BRANSDICKE", nsteps=6,
+         over=dict(Nmesh=64, Nsample=32, Box=64.0, modified_gravity_active=1)),
+    # m(a)-beta(a) symmetron-like model with screening and scale-dependent growth
+    dict(name="lcdm_64_128", model="MBETA", nsteps=6,
+         over=dict(Nmesh=64, Nsample=32, Box=64.0, modified_gravity_active=1, include_screening=1)),
+    # the benchmark's path (f(R) f_R0 = 1e-5, scale-dependent, screening, P(k) every step) at 128^3 / 256^3:
+    # two-tile column passes, lattice fold m = 2 from a 256 mesh
+    dict(name="fofr_sd_512_1024", model=None, nsteps=4, over=dict(Nmesh=256, Nsample=128, Box=256.0,
+                                                                  pofk_nbins=128)),
+])
+def test_end_to_end_models_match_oracle(tmp_path, case):
+    """Whole runs of the other gravity models and of the benchmark's path at
+    128^3 particles on a 256^3 mesh: final positions within 1e-4 of the box."""
+    hp = make_params(tmp_path, case["name"], case["nsteps"], model=case["model"], **case["over"])
+    print("case:", case, flush=True)
+    gpos, gvel, _ = run_gpu(hp)
+    cosmo = M.Cosmo(hp)
+    rpos, rvel, _ = oracle_run(hp, cosmo, case["nsteps"])
+    d = np.abs(periodic_diff(gpos, rpos, hp.Box))
+    v = np.abs(gvel.astype(np.float64) - rvel)
+    print("max |dpos| / Box =", d.max() / hp.Box, " max |dvel| / max |vel| =", v.max() / np.abs(rvel).max(),
+          flush=True)
+    assert d.max() <= 1e-4 * hp.Box
+    assert v.max() <= 1e-3 * np.abs(rvel).max()
+
+
+def test_end_to_end_pofk_every_step_and_density(tmp_path):
+    """The plain LCDM workload (64^3 particles, 128^3 mesh, Box 256, 10 steps)
+    with P(k) at every force evaluation: each of the 11 P(k) (raw binned power,
+    no shot-noise constant) within 0.1 % per bin of the oracle's on its own
+    trajectory, identical mode counts; then the density of the final positions
+    against the oracle's deposit of the same positions to 1e-6 of 1 + delta."""
+    nsteps = 10
+    hp = make_params(tmp_path, "lcdm_64_128", nsteps, pofk_compute_every_step=1, pofk_nbins=64,
+                     pofk_subtract_shotnoise=0)
+    gpos, gvel, log = run_gpu(hp, track=True)
+    assert len(log) == nsteps + 1
+    cosmo = M.Cosmo(hp)
+    nb, bt, sub, kmin, kmax = O.adjust_pofk(hp.Nmesh, hp.pofk_nbins, hp.pofk_bintype, 0, 0.0, 0.0)
+    rpos, rvel, _, rpk = oracle_run(hp, cosmo, nsteps, pofk=(nb, bt, sub, kmin, kmax))
+    d = np.abs(periodic_diff(gpos, rpos, hp.Box))
+    assert d.max() <= 1e-4 * hp.Box
+    worst = 0.0
+    for step, ((a, _, pk, kk, nm), (rp, rk, rn)) in enumerate(zip(log, rpk)):
+        assert np.array_equal(nm, rn), step
+        sel = (rn > 0) & (rk > 0)  # the k = 0 bin holds the mean, zero by construction
+        np.testing.assert_allclose(kk[sel], rk[sel], rtol=1e-12)
+        e = np.abs(pk[sel] - rp[sel]) / np.abs(rp[sel])
+        worst = max(worst, e.max())
+        print(f"step {step} a {a:.4f}: P(k) max rel err {e.max():.2e}", flush=True)
+        assert e.max() < 1e-3, step
+    print("worst per-bin P(k) error over the run:", worst, flush=True)
+    g = M.MGPU(hp.Nmesh, hp.Nsample, hp.Box, hp.Omega)
+    g.set_particles(gpos)
+    g.pm_deposit()
+    dx = g.get_mesh(M.MESH_DENSITY_X)
+    g.close()
+    ref = O.real_view(O.deposit(O.cfg(hp.Nmesh, hp.Nsample, hp.Box, hp.Omega), gpos), hp.Nmesh)
+    err = np.abs(dx - ref) / np.maximum(1.0, np.abs(1.0 + ref))
+    assert err.max() < 1e-6, err.max()
+    assert ref.max() > 10.0  # the z = 0 field is clustered
+
+
 @pytest.mark.parametrize("n,ns,sd", [(32, 16, 0), (32, 32, 0), (24, 12, 1), (48, 24, 1), (24, 24, 1), (48, 16, 1),
                                      (32, 12, 1)])
 def test_ic_matches_oracle(n, ns, sd):
@@ -87,6 +168,43 @@ def test_ic_matches_oracle(n, ns, sd):
         za, lp = O.ic_displacements(c, cd, 0)
         assert rel_err(g.get_lpt(0), za) < 1e-5
         assert rel_err(g.get_lpt(1), lp) < 2e-5
+
+
+@pytest.mark.parametrize("sphere,fixed,inverted", [(1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 1)])
+@pytest.mark.parametrize("n,ns,sd", [(32, 16, 0), (24, 24, 0), (48, 24, 1)])
+def test_ic_variants_match_oracle(n, ns, sd, sphere, fixed, inverted):
+    """SphereMode = 1 (|k| < k_Nyquist of the particle grid), amplitude-fixed
+    modes (|delta| = sqrt(P)) and inverted initial conditions (delta -> -delta),
+    src/2LPT.c:347-412, against the oracle on the same RNG stream; with the
+    amplitude fixed, every retained mode has |delta|^2 = P / Box^3 exactly up to
+    FP32 rounding."""
+    box = 2.0 * n
+    kf = 2 * np.pi / box
+    n2 = np.arange(3 * (n // 2) ** 2 + 1)
+    k = kf * np.sqrt(n2)
+    pk = np.where(n2 > 0, 2e4 * k / (1 + (k / 0.05) ** 2.5), 0.0)
+    print("case", n, ns, sd, sphere, fixed, inverted, flush=True)
+    g = M.MGPU(n, ns, box, 0.3, scale_dependent=sd)
+    g.ic_displacement_fields(5001, pk, sphere_mode=sphere, amplitude_fixed=fixed, inverted=inverted)
+    c = O.cfg(n, ns, box, 0.3)
+    cd = O.ic_delta(c, 5001, pk, sphere, fixed, inverted)
+    got = g.get_mesh(M.MESH_CDELTA1)
+    ref = O.complex_view(cd, n)
+    assert rel_err(got, ref) < 1e-6
+    if fixed:
+        kx = np.rint(np.fft.fftfreq(n) * n).astype(np.int64)
+        m2 = kx[:, None, None] ** 2 + kx[None, :, None] ** 2 + np.arange(n // 2 + 1)[None, None, :] ** 2
+        nz = np.abs(ref) > 0
+        want = pk[m2] / box ** 3
+        np.testing.assert_allclose(np.abs(got[nz]) ** 2, want[nz], rtol=2e-6)
+    if sd:
+        (cd2,) = O.ic_displacements(c, cd, 1)
+        assert rel_err(g.get_mesh(M.MESH_CDELTA2), O.complex_view(cd2, n)) < 5e-6
+    else:
+        za, lp = O.ic_displacements(c, cd, 0)
+        assert rel_err(g.get_lpt(0), za) < 1e-5
+        assert rel_err(g.get_lpt(1), lp) < 2e-5
+    g.close()
 
 
 def test_lpt_slices_match_host_tables(tmp_path):

@@ -17,7 +17,22 @@ Pinned:
                          screening of a plane wave, JBD G_eff scaling
   2LPT                   single plane wave (zero source), two orthogonal waves
                          (analytic source and displacement)
-  Kick / Drift           free streaming with zero force, periodic wrap
+  Kick / Drift           free streaming with zero force, periodic wrap; the COLA
+                         identity (zero kick when the PM force is the LPT force,
+                         ZA and 2LPT slots); a 1D Zel'dovich plane wave run through
+                         deposit, forces, gather, Kick and Drift follows
+                         x = q + D(a) Psi(q) before shell crossing
+  Seed table             hand-derived draw order of src/2LPT.c:262-271 for N = 2, 4;
+                         the table of N is the table of 2N's inner shells
+  IC delta(k)            amplitude-fixed modes |delta|^2 = P / Box^3, inverted = -delta,
+                         Hermitian kz = 0 plane, zero Nyquist planes and DC, exact cube
+                         and sphere cuts, draws consumed by skipped modes (the phases
+                         do not depend on Nsample), phase of a mode = 2 pi x the
+                         column stream's draw of that mode, |delta|^2 Box^3 / P
+                         exponentially distributed
+  LPT fields             g = 1 gives the ZA field; the 2LPT field of the stored
+                         source with g = -3/7/N^3 gives the 2LPT displacement;
+                         plane waves with a k-dependent growth g(|k|)
 """
 import numpy as np
 import pytest
@@ -558,3 +573,316 @@ def test_bf16_rounding_is_nearest_even():
     assert Q.bf16(-3.0) == -3.0
     assert Q.gelu(np.array([0.0]))[0] == 0.0
     np.testing.assert_allclose(Q.dgelu(np.array([0.0])), [0.5])
+
+
+# ---------------------------------------------------------------- COLA identity and Zel'dovich run
+
+def test_kick_cola_identity_zero_residual():
+    """COLA integrates the residual to LPT: when the gathered PM force equals
+    the LPT force, -1.5 Omega (Disp - <Disp>) cancels the LPT term (D ddD1 +
+    D2 ddD2)/A with ddD1 = 1.5 Omega A D1 and ddD2 = 1.5 Omega A (D2 - D1^2)
+    (the growth equations, src/cosmo.c:206-239), so the kick is zero
+    (src/main.c:707-716).  Disp here is the force of the 2LPT trajectory in the
+    Forces sign convention: -(D1 Psi1 + (D2 - D1^2) Psi2)."""
+    rng = np.random.default_rng(4)
+    npart, om, A, D1, D2 = 4096, 0.3, 0.4, 0.43, -0.09
+    psi1 = rng.normal(size=(3, npart)).astype(np.float32)
+    psi2 = (0.1 * rng.normal(size=(3, npart))).astype(np.float32)
+    psi1 -= psi1.mean(axis=1, keepdims=True)
+    psi2 -= psi2.mean(axis=1, keepdims=True)
+    disp = (-(D1 * psi1.astype(np.float64) + (D2 - D1 ** 2) * psi2.astype(np.float64))).astype(np.float32)
+    sumD = disp.astype(np.float64).mean(axis=1)
+    s1, s2 = 1.5 * om * A * D1, 1.5 * om * A * (D2 - D1 ** 2)
+    vel = np.zeros((3, npart), np.float32)
+    O.kick(vel, disp.copy(), psi1, psi2, s1, s2, 0, om, 0.7, A, 1, sumD)
+    assert np.abs(vel).max() < 1e-6 * 0.7 * 1.5 * om * np.abs(disp).max()
+    # and the two terms are each non-zero: without COLA the same force kicks
+    vel0 = np.zeros_like(vel)
+    O.kick(vel0, disp.copy(), psi1, psi2, s1, s2, 0, om, 0.7, A, 0, sumD)
+    assert np.abs(vel0).max() > 0.1 * 0.7 * 1.5 * om * np.abs(disp).max()
+    # scale-dependent layout: slot D holds the pre-multiplied ddD field (sum of both orders)
+    lpt = (psi1.astype(np.float64) * s1 + psi2.astype(np.float64) * s2).astype(np.float32)
+    vel_sd = np.zeros_like(vel)
+    O.kick(vel_sd, disp.copy(), lpt, np.zeros_like(lpt), 0.0, 0.0, 1, om, 0.7, A, 1, sumD)
+    assert np.abs(vel_sd).max() < 1e-5 * 0.7 * 1.5 * om * np.abs(disp).max()
+
+
+def test_drift_cola_moves_with_lpt_growth():
+    """With zero residual velocity, Drift adds exactly dD1 Psi1 + dD2 Psi2
+    (src/main.c:762-769): the particle stays on x = q + D1 Psi1 + D2 Psi2."""
+    rng = np.random.default_rng(5)
+    npart, box = 2048, 100.0
+    q = rng.uniform(10, 90, size=(3, npart))
+    psi1 = rng.normal(size=(3, npart)).astype(np.float32)
+    psi2 = (0.1 * rng.normal(size=(3, npart))).astype(np.float32)
+    D1a, D1b, D2a, D2b = 0.3, 0.42, -0.04, -0.08
+    pos = (q + D1a * psi1 + D2a * psi2).astype(np.float32)
+    vel = np.zeros((3, npart), np.float32)
+    O.drift(pos, vel, psi1, psi2, D1b - D1a, D2b - D2a, 0, box, 0.6, 1, np.zeros(3))
+    expect = q + D1b * psi1.astype(np.float64) + D2b * psi2.astype(np.float64)
+    assert np.abs(pos - expect).max() < 2e-5
+    # scale-dependent layout: the Delta D fields of the two orders, stored pre-multiplied
+    pos2 = (q + D1a * psi1 + D2a * psi2).astype(np.float32)
+    f1 = ((D1b - D1a) * psi1.astype(np.float64)).astype(np.float32)
+    f2 = ((D2b - D2a) * psi2.astype(np.float64)).astype(np.float32)
+    O.drift(pos2, vel, f1, f2, 0.0, 0.0, 1, box, 0.6, 1, np.zeros(3))
+    assert np.abs(pos2 - expect).max() < 2e-5
+
+
+def zeldovich_1d(ns, box, eps):
+    """Lattice q at cell centres and the 1D displacement Psi_x = eps sin(2 pi q_x / Box)
+    (zero in y, z).  With particles off the mesh nodes and moved by less than
+    half a cell, the CIC weights are linear in the displacement."""
+    q = (np.indices((ns, ns, ns)).reshape(3, -1).astype(np.float64) + 0.5) * (box / ns)
+    psi = np.zeros_like(q)
+    psi[0] = eps * np.sin(2 * np.pi * q[0] / box)
+    return q, psi
+
+
+def test_pm_force_of_zeldovich_wave_is_lpt_force():
+    """In 1D before shell crossing the force on a particle is fixed by the mass
+    ordering: Disp = -D Psi(q) for x = q + D Psi.  The PM force (CIC deposit,
+    Green's function, CIC gather) reproduces it up to the discreteness of the
+    deposit and the gather, under 1 % for one wave on 32 cells."""
+    n, ns, box, eps, D = 32, 32, 64.0, 0.05, 0.8  # density contrast 0.4 %: linear
+    q, psi = zeldovich_1d(ns, box, eps)
+    pos = np.mod(q + D * psi, box).astype(np.float32)
+    c = O.cfg(n, ns, box, 0.3)
+    d = O.deposit(c, pos)
+    O.fft_r2c_3d(n, d)
+    disp, _ = O.gather(c, pos, O.forces(c, d, None))
+    expect = -D * psi[0]
+    np.testing.assert_allclose(disp[0], expect, atol=1e-2 * np.abs(expect).max())
+    assert np.abs(disp[0] - expect).max() > 1e-5 * np.abs(expect).max()  # a mesh, not the continuum
+    assert np.abs(disp[1:]).max() < 1e-9
+
+
+def zeldovich_run(nsteps, frac, use_cola):
+    """Kick/Drift run of a 1D Zel'dovich wave (amplitude frac Box/2pi, so the
+    density contrast at a = 1 is frac) on 32^3 particles and a 32^3 mesh in a
+    64 Mpc/h LCDM box from a = 0.05 to 1, with the reference's schedule and the
+    host's growth factors and time integrals; returns the displacement from
+    x = q + D1(a) Psi(q) and the residual velocity, both over D1 Psi's amplitude."""
+    import mgpicola as M
+    import workloads as W
+    import tempfile
+    import os
+    n, ns, box = 32, 32, 64.0
+    out = tempfile.mkdtemp()
+    model, vals, _ = W.values_for("lcdm_64_128", out, Nmesh=n, Nsample=ns, Box=box, Init_Redshift=19.0)
+    with open(vals["OutputRedshiftFile"], "w") as f:
+        f.write(f"0, {nsteps}\n")
+    pf = os.path.join(out, "p.txt")
+    M.write_paramfile(pf, vals, model=model)
+    hp = M.read_params(pf, model=model)
+    cosmo = M.Cosmo(hp)
+    eps = frac * box / (2 * np.pi)
+    q, psi = zeldovich_1d(ns, box, eps)
+    psi32 = psi.astype(np.float32)
+    zero = np.zeros_like(psi32)
+    c = O.cfg(n, ns, box, hp.Omega)
+    A = 1.0 / 20.0
+    pos = np.mod(q + cosmo.growth(M.G_D, A) * psi, box).astype(np.float32)
+    # COLA: zero residual velocity; otherwise the linear growing-mode velocity dD/dy Psi
+    vel = np.zeros_like(pos) if use_cola else (cosmo.growth(M.G_dDdy, A) * psi).astype(np.float32)
+    AI = A
+    da = (1.0 - A) / nsteps
+    for i in range(nsteps):
+        AF, AFF = A + 0.5 * da, A + da
+        d = O.deposit(c, pos)
+        O.fft_r2c_3d(n, d)
+        disp, sD = O.gather(c, pos, O.forces(c, d, None))
+        sV = O.kick(vel, disp, psi32, zero, cosmo.growth(M.G_ddDddy, A), cosmo.growth(M.G_ddD2ddy, A), 0,
+                    hp.Omega, cosmo.Sphi(AI, AF, A), A, use_cola, sD)
+        O.drift(pos, vel, psi32, zero, cosmo.growth(M.G_D, AFF) - cosmo.growth(M.G_D, A), 0.0, 0, box,
+                cosmo.Sq(A, AFF, AF), use_cola, sV)
+        A, AI = AFF, AF
+    assert abs(A - 1.0) < 1e-12
+    D1 = cosmo.growth(M.G_D, 1.0)
+    dx = pos.astype(np.float64) - (q + D1 * psi)
+    dx -= box * np.round(dx / box)
+    assert np.abs(dx[1:]).max() < 1e-5
+    vres = vel if use_cola else vel - cosmo.growth(M.G_dDdy, 1.0) * psi
+    return np.abs(dx).max() / (D1 * eps), np.abs(vres).max() / (cosmo.growth(M.G_dDdy, 1.0) * eps)
+
+
+def test_zeldovich_cola_run_follows_growth():
+    """In 1D the Zel'dovich trajectory x = q + D1(a) Psi(q) is exact until
+    shell crossing, so a COLA run (src/main.c:394-611, Kick and Drift
+    src/main.c:688-787) keeps a residual displacement and velocity at the level
+    of the PM force's own error (the CIC window, 0.6 %, and its second-order
+    discreteness): 12 steps from a = 0.05 stay within 3 %.  It pins the force,
+    COLA, Sphi and Sq terms together.  The same 12 steps without COLA are off by
+    a third: the time stepping error that COLA removes."""
+    ex, ev = zeldovich_run(12, 0.05, 1)
+    print("COLA: position", ex, "velocity", ev)
+    assert ex < 0.03 and ev < 0.05
+    sx, _ = zeldovich_run(12, 0.05, 0)
+    print("standard PM: position", sx)
+    assert sx > 0.15
+
+
+# ---------------------------------------------------------------- seed table and IC delta(k)
+
+def _draws(seed, count):
+    return (0x7FFFFFFF * O.ranlxd_stream(seed, count)).astype(np.uint32)
+
+
+def test_seedtable_draw_order_by_hand():
+    """src/2LPT.c:262-271 worked by hand: N = 2 fills (0,0), (1,0), (0,1), (1,1) with
+    draws 0, 1, 2, 3 in that order; N = 4 as listed (index i*N + j -> draw)."""
+    d = _draws(77, 16)
+    assert list(O.seedtable(2, 77)) == [d[0], d[2], d[1], d[3]]
+    order4 = [0, 5, 11, 2, 4, 6, 12, 10, 7, 9, 15, 13, 1, 8, 14, 3]
+    assert list(O.seedtable(4, 77)) == [d[k] for k in order4]
+
+
+@pytest.mark.parametrize("n", [6, 8, 12])
+def test_seedtable_inner_shells_do_not_depend_on_n(n):
+    """Shells are filled from the centre (index 0 and N-1) outwards, so the
+    seeds of N's shells reappear at the same places in the table of 2N, and
+    every entry gets exactly one draw."""
+    a = O.seedtable(n, 5001).reshape(n, n)
+    b = O.seedtable(2 * n, 5001).reshape(2 * n, 2 * n)
+    h = n // 2
+    for i in range(n):
+        for j in range(n):
+            bi = i if i < h else 2 * n - (n - i)
+            bj = j if j < h else 2 * n - (n - j)
+            assert a[i, j] == b[bi, bj], (i, j)
+    assert sorted(a.ravel()) == sorted(_draws(5001, n * n))
+
+
+def ic_pk(n, box):
+    kf = 2 * np.pi / box
+    n2 = np.arange(3 * (n // 2) ** 2 + 1)
+    k = kf * np.sqrt(n2)
+    return np.where(n2 > 0, 3e3 * k / (1 + (k / 0.1) ** 2.5), 0.0)
+
+
+def signed(n):
+    i = np.arange(n)
+    return np.where(i < n // 2, i, i - n)
+
+
+@pytest.mark.parametrize("n,ns,sphere", [(16, 16, 0), (16, 8, 0), (16, 8, 1), (12, 6, 1)])
+def test_ic_delta_fixed_amplitude_cut_and_symmetry(n, ns, sphere):
+    box = 3.0 * n
+    pk = ic_pk(n, box)
+    c = O.cfg(n, ns, box, 0.3)
+    cd = O.complex_view(O.ic_delta(c, 5001, pk, sphere, 1, 0), n)
+    s = signed(n)
+    kx, ky, kz = s[:, None, None], s[None, :, None], np.arange(n // 2 + 1)[None, None, :]
+    n2 = kx ** 2 + ky ** 2 + kz ** 2
+    nyq = (np.abs(kx) == n // 2) | (np.abs(ky) == n // 2) | (kz == n // 2)
+    if sphere:
+        keep = n2 <= (ns // 2) ** 2
+    else:
+        keep = (np.abs(kx) <= ns // 2) & (np.abs(ky) <= ns // 2) & (kz <= ns // 2)
+    keep = keep & ~nyq & (n2 > 0)
+    amp2 = np.abs(cd) ** 2
+    np.testing.assert_allclose(amp2[keep], pk[n2[keep]] / box ** 3, rtol=1e-12)
+    assert np.all(amp2[~keep] == 0.0)
+    # Hermitian kz = 0 plane: delta(-kx, -ky, 0) = conj delta(kx, ky, 0)
+    p0 = cd[:, :, 0]
+    mirror = p0[(-np.arange(n)) % n][:, (-np.arange(n)) % n]
+    np.testing.assert_allclose(p0, np.conj(mirror), atol=0)
+    # inverted initial conditions: exactly -delta
+    inv = O.complex_view(O.ic_delta(c, 5001, pk, sphere, 1, 1), n)
+    assert np.array_equal(inv, -cd)
+
+
+def test_ic_delta_draws_consumed_by_skipped_modes():
+    """The cut to |n_a| <= Nsample/2 (src/2LPT.c:380-386) only removes modes:
+    the modes both cuts keep carry the same numbers for Nsample = 16 and
+    Nsample = 8 (the IC of a coarser particle grid is the finer one's, cut)."""
+    n, box = 16, 48.0
+    pk = ic_pk(n, box)
+    big = O.complex_view(O.ic_delta(O.cfg(n, 16, box, 0.3), 5001, pk, 0, 0, 0), n)
+    small = O.complex_view(O.ic_delta(O.cfg(n, 8, box, 0.3), 5001, pk, 0, 0, 0), n)
+    nz = small != 0
+    assert nz.sum() > 100
+    assert np.array_equal(small[nz], big[nz])
+
+
+@pytest.mark.parametrize("amp_fixed", [0, 1])
+def test_ic_delta_phase_is_column_stream_draw(amp_fixed):
+    """The phase of mode (i, j, k), k > 0, is 2 pi times draw number k (amplitude
+    fixed) or 2k (phase and amplitude drawn per mode) of the stream seeded with
+    the seed table entry of column (i, j) (src/2LPT.c:337-357); without fixed
+    amplitudes the modulus is sqrt(-ln u P / Box^3) with u the next draw.  The
+    k = 0 mode of column (0, 0) is skipped but still draws (src/2LPT.c:347-357),
+    so that column's modes k > 0 keep their place in the stream."""
+    n, box = 12, 36.0
+    pk = ic_pk(n, box)
+    cd = O.complex_view(O.ic_delta(O.cfg(n, n, box, 0.3), 5001, pk, 0, amp_fixed, 0), n)
+    st = O.seedtable(n, 5001).reshape(n, n)
+    s = signed(n)
+    for (i, j, k) in [(1, 2, 3), (5, 0, 1), (0, 7, 4), (11, 11, 2), (3, 9, 5), (0, 0, 1), (0, 0, 4)]:
+        u = O.ranlxd_stream(int(st[i, j]), 2 * (k + 1))
+        if amp_fixed:
+            ph, mod2 = 2 * np.pi * u[k], pk[s[i] ** 2 + s[j] ** 2 + k * k] / box ** 3
+        else:
+            ph = 2 * np.pi * u[2 * k]
+            mod2 = -np.log(u[2 * k + 1]) * pk[s[i] ** 2 + s[j] ** 2 + k * k] / box ** 3
+        v = cd[i, j, k]
+        assert abs(np.angle(v) - np.angle(np.exp(1j * ph))) < 1e-9, (i, j, k)
+        assert abs(abs(v) ** 2 - mod2) < 1e-9 * mod2, (i, j, k)
+
+
+def test_ic_delta_amplitudes_are_exponential():
+    """|delta|^2 Box^3 / P = -ln u is exponentially distributed with mean 1
+    (Rayleigh-distributed amplitudes of a Gaussian field)."""
+    n, box = 32, 96.0
+    pk = ic_pk(n, box)
+    cd = O.complex_view(O.ic_delta(O.cfg(n, n, box, 0.3), 5001, pk, 0, 0, 0), n)
+    s = signed(n)
+    n2 = s[:, None, None] ** 2 + s[None, :, None] ** 2 + np.arange(n // 2 + 1)[None, None, :] ** 2
+    nz = np.abs(cd) > 0
+    nz[:, :, 0] = False  # the kz = 0 plane holds each draw twice (Hermitian copies)
+    x = (np.abs(cd[nz]) ** 2 * box ** 3 / pk[n2[nz]])
+    assert x.size > 5000
+    assert abs(x.mean() - 1.0) < 4.0 / np.sqrt(x.size)
+    # Kolmogorov-Smirnov distance to 1 - exp(-x)
+    xs = np.sort(x)
+    ks = np.max(np.abs(np.arange(1, xs.size + 1) / xs.size - (1 - np.exp(-xs))))
+    assert ks < 1.6 / np.sqrt(xs.size)
+
+
+# ---------------------------------------------------------------- LPT fields (scale-dependent build)
+
+def test_lpt_field_with_unit_growth_is_za():
+    n, ns, box = 16, 8, 48.0
+    c = O.cfg(n, ns, box, 0.3)
+    cd = O.ic_delta(c, 5001, ic_pk(n, box))
+    za, lp = O.ic_displacements(c, cd, 0)
+    g1 = np.ones(3 * (n // 2) ** 2 + 1)
+    np.testing.assert_allclose(O.lpt_field(c, cd, g1), za, rtol=0, atol=1e-6 * np.abs(za).max())
+    (cd2,) = O.ic_displacements(c, cd, 1)
+    g2 = np.full_like(g1, -3.0 / 7.0 / n ** 3)
+    np.testing.assert_allclose(O.lpt_field(c, cd2, g2), lp, rtol=0, atol=1e-6 * np.abs(lp).max())
+    assert np.abs(lp).max() > 1e-3 * np.abs(za).max()
+
+
+def test_lpt_field_plane_waves_with_scale_dependent_growth():
+    """Two waves of different |k| with growth g(|n|^2): each displacement is
+    its own ZA field scaled by g of its wavenumber, read at the lattice
+    (N = 2 Nsample, so the lattice points are mesh points)."""
+    n, ns, box = 16, 8, 64.0
+    A, B = 0.05, 0.03
+    m1, m2 = (0, 0, 2), (2, 2, 0)
+    cd = wave_spectrum(n, [(A, m1), (B, m2)])
+    n2 = np.arange(3 * (n // 2) ** 2 + 1)
+    g = 1.0 + 0.1 * n2  # g(4) = 1.4, g(8) = 1.8
+    out = O.lpt_field(O.cfg(n, ns, box, 0.3), cd, g)
+    qi = np.indices((ns, ns, ns)).reshape(3, -1) * (n // ns)
+    kf = 2 * np.pi / box
+    expect = np.zeros((3, ns ** 3))
+    for amp, m in ((A, m1), (B, m2)):
+        kv = kf * np.array(m, float)
+        th = 2 * np.pi * (np.array(m)[:, None] * qi).sum(0) / n
+        gm = g[int(np.dot(m, m))]
+        for a in range(3):
+            expect[a] += -amp * gm * kv[a] / kv.dot(kv) * np.sin(th)
+    expect -= expect.mean(axis=1, keepdims=True)
+    np.testing.assert_allclose(out, expect, atol=1e-7)
