@@ -669,6 +669,7 @@ int mgpu_set_mesh(mgpu_ctx *ctx, int mesh, const double *host) {
   }
   MGPU_HIP(ctx, hipMemcpyAsync(dst, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice, ctx->stream));
   MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (mesh == MGPU_MESH_DENSITY_X) ctx->have_deposit = true;  // a density to transform (FFTW's r2c input)
   if (mesh == MGPU_MESH_DENSITY_K) ctx->have_density = true;
   if (mesh == MGPU_MESH_PHI_K) ctx->have_phik = true;
   if (mesh >= MGPU_MESH_FORCE_X && mesh <= MGPU_MESH_FORCE_Z) ctx->have_forces = true;

@@ -16,12 +16,14 @@ constexpr int kBlock = 256;
 // compute_power_spectrum (src/compute_pofk.c:71-236): every mode of the half
 // spectrum with weight 1 (kz = 0, N/2) or 2, |n| binned linearly or
 // logarithmically, CIC deconvolution 1/(W_x W_y W_z)^4 and 1/N^6.
+// Only bins [b0, b0 + nb) are accumulated (in LDS), so a bin count beyond
+// the LDS takes several launches over bin ranges.
 __global__ __launch_bounds__(kBlock) void k_pofk(const float2 *__restrict__ dk, int N, int HP,
                                                  const double *__restrict__ winv, const int16_t *__restrict__ pbin,
-                                                 int nbins, int bintype, double kmin, double kmax,
+                                                 int nbins, int bintype, double kmin, double kmax, int b0, int nb,
                                                  double *__restrict__ out) {
-  extern __shared__ double sh[];  // 3*nbins
-  for (int b = threadIdx.x; b < 3 * nbins; b += blockDim.x) sh[b] = 0.0;
+  extern __shared__ double sh[];  // 3*nb
+  for (int b = threadIdx.x; b < 3 * nb; b += blockDim.x) sh[b] = 0.0;
   __syncthreads();
   const int H = N / 2 + 1;
   const size_t total = (size_t)N * N * H;
@@ -37,17 +39,17 @@ __global__ __launch_bounds__(kBlock) void k_pofk(const float2 *__restrict__ dk, 
     if (pbin) nk = pbin[n2];  // the same formula, evaluated by the host
     else if (bintype == 0) nk = (int)((kmag - kmin) / (kmax - kmin) * nbins + 0.5);
     else nk = kmag <= 0.0 ? -1 : (int)(log(kmag / kmin) / lnrange * nbins + 0.5);
-    if (nk < 0 || nk >= nbins) continue;
+    if (nk < 0 || nk >= nbins || nk < b0 || nk >= b0 + nb) continue;
     const double w = (k == 0 || k == N / 2) ? 1.0 : 2.0;
     const float2 v = dk[rowi * HP + k];
     const double p = ((double)v.x * v.x + (double)v.y * v.y) * winv[d0] * winv[d1] * winv[d2];
-    atomicAdd(&sh[nk], w * p);
-    atomicAdd(&sh[nbins + nk], w * kmag);
-    atomicAdd(&sh[2 * nbins + nk], w);
+    atomicAdd(&sh[nk - b0], w * p);
+    atomicAdd(&sh[nb + nk - b0], w * kmag);
+    atomicAdd(&sh[2 * nb + nk - b0], w);
   }
   __syncthreads();
-  for (int b = threadIdx.x; b < 3 * nbins; b += blockDim.x)
-    if (sh[b] != 0.0) atomicAdd(&out[b], sh[b]);
+  for (int b = threadIdx.x; b < 3 * nb; b += blockDim.x)
+    if (sh[b] != 0.0) atomicAdd(&out[(b / nb) * nbins + b0 + b % nb], sh[b]);
 }
 
 // ------------------------------------------------------------------ RSD multipoles
@@ -427,28 +429,41 @@ int pm_pofk(mgpu_ctx *ctx, const float2 *dk, mgpu_pofk *out) {
   double *dwin;
   int rc;
   if ((rc = pofk_setup(ctx, &dwin))) return rc;
-  const size_t shmem = 3 * (size_t)nbins * sizeof(double);
+  // bins per launch: as many as the LDS holds (three doubles each)
+  const int chunk = std::max(1, std::min(nbins, (int)(ctx->lds_max / (3 * sizeof(double)))));
+  const size_t shmem = 3 * (size_t)chunk * sizeof(double);
   if (shmem > 65536) {
     MGPU_HIP(ctx, hipFuncSetAttribute((const void *)k_pofk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
   }
   const size_t modes = (size_t)N * N * (N / 2 + 1);
   unsigned grid = std::min<size_t>(cdiv(modes, kBlock), 4096);
-  {
+  for (int b0 = 0; b0 < nbins; b0 += chunk) {
+    const int nb = std::min(chunk, nbins - b0);
     KScope ks(ctx, MGPU_K_POFK, 8.0 * modes);
-    hipLaunchKernelGGL(k_pofk, dim3(grid), dim3(kBlock), shmem, ctx->stream, dk, N, ctx->HP, dwin, ctx->pk_bin,
-                       nbins, ctx->p.pofk_bintype, ctx->p.pofk_kmin, ctx->p.pofk_kmax, ctx->pk);
+    hipLaunchKernelGGL(k_pofk, dim3(grid), dim3(kBlock), 3 * (size_t)nb * sizeof(double), ctx->stream, dk, N, ctx->HP,
+                       dwin, ctx->pk_bin, nbins, ctx->p.pofk_bintype, ctx->p.pofk_kmin, ctx->p.pofk_kmax, b0, nb,
+                       ctx->pk);
+    MGPU_HIP(ctx, hipGetLastError());
   }
-  MGPU_HIP(ctx, hipGetLastError());
   return pofk_finish(ctx, out, false);
 }
 
+// The x pass that bins P(k) holds its tile of T columns and the bins in LDS.
+static bool pofk_fused_fits(const mgpu_ctx *ctx, int T) {
+  const size_t need = (size_t)ctx->N * T * sizeof(float2) + sizeof(double) * (size_t)ctx->p.pofk_nbins +
+                      sizeof(float) * (size_t)(ctx->N / 2 + 1);
+  return need <= ctx->lds_max;
+}
+
 // PtoMesh's r2c of delta(x) (slot dx_slot) into slot 0, with P(k) when out != NULL.
-// Once the bin geometry is known the power is binned inside the last (x) pass.
+// Once the bin geometry is known the power is binned inside the last (x) pass,
+// when the bins fit that pass's LDS; otherwise by k_pofk after the transform.
 int pm_transform(mgpu_ctx *ctx, mgpu_pofk *out) {
   float2 *dk = ctx->slot[0];
   const float *dx = (const float *)ctx->slot[ctx->dx_slot];
   int rc;
-  if (!out || !ctx->pk_bin || ctx->pk_geom.size() != 2 * (size_t)ctx->p.pofk_nbins) {
+  if (!out || !ctx->pk_bin || ctx->pk_geom.size() != 2 * (size_t)ctx->p.pofk_nbins ||
+      !pofk_fused_fits(ctx, ctx->fft.tcols)) {
     {
       PhaseTimer pt(ctx, &ctx->t.ptomesh);
       if ((rc = fft_r2c_3d(ctx, &ctx->fft, dk, dx))) return rc;
@@ -571,7 +586,10 @@ static int forces_tail(mgpu_ctx *ctx) {
 bool pm_step_fusable(const mgpu_ctx *ctx, const mgpu_mg *mg, const mgpu_pofk *out) {
   if (ctx->nu_on) return false;  // the neutrino term sits between the density's x pass and the next
   if (out && (!ctx->pk_bin || ctx->pk_geom.size() != 2 * (size_t)ctx->p.pofk_nbins)) return false;
-  if (!ctx->p.mg_active || ctx->p.model == MGPU_LCDM) return true;
+  // the density pair bins P(k) in LDS: 8-column tiles with the Green's force (LCDM), else the plan's width
+  const bool lcdm = !ctx->p.mg_active || ctx->p.model == MGPU_LCDM;
+  if (out && !pofk_fused_fits(ctx, lcdm ? 8 : ctx->fft.tcols)) return false;
+  if (lcdm) return true;
   if (!mg || !ctx->p.include_screening) return false;
   return ctx->p.model == MGPU_FOFR || ctx->p.model == MGPU_MBETA || ctx->p.model == MGPU_DGP;
 }

@@ -309,29 +309,19 @@ void mgp_ic_pk_table(const mgp_cosmo *c, double *pk) {
   double s8r2 = 1.0;
   if (p->modified_gravity_active && p->input_pofk_is_for_lcdm) s8r2 = pow2(mgp_mg_sigma8_enhancement(c, 1.0));
   const int rescale = p->modified_gravity_active && p->input_pofk_is_for_lcdm;
-  /* slice of the scale-dependent D table at a = 1 for the ratio */
-  double *sv = NULL, *sdv = NULL;
-  if (rescale && p->scale_dependent) {
-    sv = malloc(sizeof(double) * c->sd[0].ny);
-    sdv = malloc(sizeof(double) * c->sd[0].ny);
-    mgp_spline2d_slice(&c->sd[MGP_G_D], 0.0, sv, sdv);
-  }
-  const double dl = mgp_spline_eval(&c->glcdm[MGP_G_D], 0.0);
-  const double ratio_nsd = rescale && !p->scale_dependent ? pow2(mgp_spline_eval(&c->g[MGP_G_D], 0.0) / dl) : 1.0;
   pk[0] = 0.0;
 #pragma omp parallel for schedule(static)
   for (size_t n2 = 1; n2 < len; n2++) {
     const double k = 2.0 * MGP_PI / p->Box * sqrt((double)n2);
     double v = mgp_power_spec(c, k);
     if (rescale) {
-      if (p->scale_dependent) v *= pow2(mgp_slice_eval(&c->sd[MGP_G_D], sv, sdv, log(k)) / dl);
-      else v *= ratio_nsd;
+      /* mg_pofk_ratio(k, 1) (src/2LPT.c:395-396): against the scale-dependent LCDM
+       * growth when massive neutrinos make it scale-dependent */
+      v *= mgp_mg_pofk_ratio(c, k, 1.0);
       if (!p->input_sigma8_is_for_lcdm) v /= s8r2;
     }
     pk[n2] = v;
   }
-  free(sv);
-  free(sdv);
 }
 
 /* per-|n|^2 LPT growth tables for libmgpu (from_cdisp_store_to_ZA,

@@ -208,7 +208,7 @@ static int output(sim *s, double A, double AF, double AFF, double z_label) {
         double lptv = p->scale_dependent ? (double)l1[i] : (double)l0[i] * dDdy + (double)l1[i] * dD2dy;
         vel[i] = (float)(velfac * fac * (vel[i] - s->sumV[k] + lptv * p->UseCOLA));
       }
-    if (p->gadget_style) rc = mgp_write_gadget(p, A, z_label, pos, vel, np);
+    rc = p->gadget_style ? mgp_write_gadget(p, A, z_label, pos, vel, np) : mgp_write_ascii(p, A, z_label, pos, vel, np);
     if (!rc) rc = mgp_write_info(p, A, np);
   }
   free(pos);
@@ -481,7 +481,7 @@ int mgp_sim_advance(mgp_sim *s, int *counted) {
     if (lpt_fields2(s, MGP_FIELD_ddDddy, MGP_FIELD_deltaD, A, s->AFF)) return -1;
   }
   const int output_now = s->ts == 0 && s->i != s->NoutputStart;
-  if (!output_now && s->vmean_valid) {
+  if (!output_now && s->vmean_valid && p->fuse_kick_drift) {
     if (kick_drift(s, s->AI, s->AF, A, s->AFF, s->Di, s->Di2)) return -1;
   } else {
     if (kick(s, s->AI, s->AF, A)) return -1;

@@ -72,6 +72,12 @@ typedef struct {
   /* host option (not a paramfile tag): build the scale-dependent growth tables on the
    * GPU (mgpu_growth_tables, default) or with the host's own ODE solver */
   int growth_on_gpu;
+  /* host option (not a paramfile tag): on steps with no output between Kick and
+   * Drift, run both in one particle pass (mgpu_kick_drift, default 1).  Its drift
+   * subtracts the mean velocity from before the kick, which equals the mean after
+   * the kick up to rounding (the kick term has zero mean); 0 runs the reference's
+   * separate Kick and Drift with the mean of the kicked velocities (src/main.c:771). */
+  int fuse_kick_drift;
 } mgp_params;
 
 /* ---- paramfile (src/read_param.c) ---- */
@@ -168,6 +174,8 @@ int mgp_write_rsd_pofk(const mgp_params *p, double a, const double *const y[5], 
 int mgp_write_pofk(const mgp_params *p, double a, const char *label, const double *pofk, const double *kmean,
                    const double *nmodes, int nbins, int bintype, double kmin, double kmax);
 int mgp_write_gadget(const mgp_params *p, double A, double z_label, const float *pos, const float *vel, size_t np);
+/* the ASCII snapshot of builds without GADGET_STYLE (src/main.c:999-1027); gadget_style = 0 */
+int mgp_write_ascii(const mgp_params *p, double A, double z_label, const float *pos, const float *vel, size_t np);
 int mgp_write_info(const mgp_params *p, double A, size_t np);
 
 #ifdef __cplusplus

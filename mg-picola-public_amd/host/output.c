@@ -134,6 +134,31 @@ int mgp_write_gadget(const mgp_params *p, double A, double z_label, const float 
   return rc;
 }
 
+/* Output, ASCII branch (builds without GADGET_STYLE), src/main.c:999-1027: the
+ * particle count on the first line, then x y z [Mpc/h] vx vy vz [km/s] per
+ * particle; the same file name as the GADGET file; vel already in km/s. */
+int mgp_write_ascii(const mgp_params *p, double A, double z_label, const float *pos, const float *vel, size_t np) {
+  double Z = z_label;
+  int zint, zfrac;
+  if (Z < 0.0) zint = zfrac = 0;
+  else {
+    zint = (int)floor(Z);
+    zfrac = (int)((Z - zint) * 1000);
+  }
+  char fn[1200];
+  snprintf(fn, sizeof fn, "%s/%s_z%dp%03d.%d", p->OutputDir, p->FileBase, zint, zfrac, 0);
+  FILE *fp = fopen(fn, "w");
+  if (!fp) return -1;
+  const double lengthfac = p->UnitLength_in_cm / 3.085678e24;
+  int rc = fprintf(fp, "%u\n", (unsigned)np) < 0;
+  for (size_t n = 0; n < np && !rc; n++)
+    rc = fprintf(fp, "%12.6f %12.6f %12.6f %12.6f %12.6f %12.6f\n", (float)(lengthfac * pos[n]),
+                 (float)(lengthfac * pos[np + n]), (float)(lengthfac * pos[2 * np + n]), vel[n], vel[np + n],
+                 vel[2 * np + n]) < 0;
+  if (fclose(fp)) rc = 1;
+  return rc ? -1 : 0;
+}
+
 /* Output_Info, src/main.c:1067-1099 (single task) */
 int mgp_write_info(const mgp_params *p, double A, size_t np) {
   double Z = 1.0 / A - 1.0;

@@ -111,6 +111,7 @@ void mgp_params_defaults(mgp_params *p, int model) {
   p->scale_dependent = model == MGP_MODEL_FOFR || model == MGP_MODEL_MBETA || model == MGP_MODEL_FOFRNU;
   p->gadget_style = 1;
   p->growth_on_gpu = 1;
+  p->fuse_kick_drift = 1;
 }
 
 int mgp_param_tag_count(const mgp_params *p) {

@@ -8,6 +8,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <omp.h>
 
 #define OR_PI 3.14159265358979323846          /* src/vars.h:61 */
 #define OR_INVERSE_H0_MPCH 2997.92458         /* src/vars.h:65 */
@@ -19,32 +20,71 @@ static inline size_t cidx(int n, int ix, int iy, int iz) { return ((size_t)ix * 
  * PtoMesh: CIC deposit, density initialised to -1, ghost slab added to
  * slab 0 with +1.  src/auxPM.c:280-356.
  * ---------------------------------------------------------------------- */
+/* one particle's eight CIC updates, in the reference's order */
+static inline void cic_one(int N, double scaleBox, double WPAR, double *density, float px, float py, float pz) {
+  double X = px * scaleBox, Y = py * scaleBox, Z = pz * scaleBox;
+  unsigned IX = (unsigned)X, IY = (unsigned)Y, IZ = (unsigned)Z;
+  double DX = X - (double)IX, DY = Y - (double)IY, DZ = Z - (double)IZ;
+  double TX = 1.0 - DX, TY = 1.0 - DY, TZ = 1.0 - DZ;
+  DY *= WPAR;
+  TY *= WPAR;
+  if (IY >= (unsigned)N) IY = 0;
+  if (IZ >= (unsigned)N) IZ = 0;
+  unsigned IXn = IX + 1, IYn = IY + 1, IZn = IZ + 1;
+  if (IYn >= (unsigned)N) IYn = 0;
+  if (IZn >= (unsigned)N) IZn = 0;
+  density[ridx(N, IX, IY, IZ)] += TX * TY * TZ;
+  density[ridx(N, IX, IY, IZn)] += TX * TY * DZ;
+  density[ridx(N, IX, IYn, IZ)] += TX * DY * TZ;
+  density[ridx(N, IX, IYn, IZn)] += TX * DY * DZ;
+  density[ridx(N, IXn, IY, IZ)] += DX * TY * TZ;
+  density[ridx(N, IXn, IY, IZn)] += DX * TY * DZ;
+  density[ridx(N, IXn, IYn, IZ)] += DX * DY * TZ;
+  density[ridx(N, IXn, IYn, IZn)] += DX * DY * DZ;
+}
+
 void or_pm_deposit(const or_cfg *c, size_t np, const float *const pos[3], double *density) {
   const int N = c->nmesh;
   const double scaleBox = (double)N / c->box;
   const double WPAR = pow((double)N / (double)c->nsample, 3);
   const size_t total = or_real_len(N);
+#pragma omp parallel for schedule(static)
   for (size_t i = 0; i < total; i++) density[i] = -1.0;
-  for (size_t i = 0; i < np; i++) {
-    double X = pos[0][i] * scaleBox, Y = pos[1][i] * scaleBox, Z = pos[2][i] * scaleBox;
-    unsigned IX = (unsigned)X, IY = (unsigned)Y, IZ = (unsigned)Z;
-    double DX = X - (double)IX, DY = Y - (double)IY, DZ = Z - (double)IZ;
-    double TX = 1.0 - DX, TY = 1.0 - DY, TZ = 1.0 - DZ;
-    DY *= WPAR;
-    TY *= WPAR;
-    if (IY >= (unsigned)N) IY = 0;
-    if (IZ >= (unsigned)N) IZ = 0;
-    unsigned IXn = IX + 1, IYn = IY + 1, IZn = IZ + 1;
-    if (IYn >= (unsigned)N) IYn = 0;
-    if (IZn >= (unsigned)N) IZn = 0;
-    density[ridx(N, IX, IY, IZ)] += TX * TY * TZ;
-    density[ridx(N, IX, IY, IZn)] += TX * TY * DZ;
-    density[ridx(N, IX, IYn, IZ)] += TX * DY * TZ;
-    density[ridx(N, IX, IYn, IZn)] += TX * DY * DZ;
-    density[ridx(N, IXn, IY, IZ)] += DX * TY * TZ;
-    density[ridx(N, IXn, IY, IZn)] += DX * TY * DZ;
-    density[ridx(N, IXn, IYn, IZ)] += DX * DY * TZ;
-    density[ridx(N, IXn, IYn, IZn)] += DX * DY * DZ;
+  const int nth = omp_get_max_threads();
+  if (nth == 1 || np < 65536 || N < 8) {
+    for (size_t i = 0; i < np; i++) cic_one(N, scaleBox, WPAR, density, pos[0][i], pos[1][i], pos[2][i]);
+  } else {
+    /* Threads own blocks of S >= 2 x-slabs; a particle in slab IX writes slabs IX and
+     * IX + 1, so blocks two apart never touch the same cells: even blocks run
+     * concurrently, then odd blocks.  Particles are visited in index order within
+     * their slab, so the result does not depend on the thread count. */
+    int S = N / (4 * nth);
+    if (S < 2) S = 2;
+    const int nb = (N + S - 1) / S;
+    size_t *start = (size_t *)calloc((size_t)N + 1, sizeof(size_t));
+    unsigned *slab = (unsigned *)malloc(np * sizeof(unsigned));
+    size_t *order = (size_t *)malloc(np * sizeof(size_t));
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < np; i++) slab[i] = (unsigned)(pos[0][i] * scaleBox);
+    for (size_t i = 0; i < np; i++) start[slab[i] + 1]++;
+    for (int x = 0; x < N; x++) start[x + 1] += start[x];
+    size_t *fill = (size_t *)malloc((size_t)N * sizeof(size_t));
+    memcpy(fill, start, (size_t)N * sizeof(size_t));
+    for (size_t i = 0; i < np; i++) order[fill[slab[i]]++] = i;
+    for (int phase = 0; phase < 2; phase++) {
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int b = phase; b < nb; b += 2) {
+        const int x1 = (b + 1) * S < N ? (b + 1) * S : N;
+        for (size_t q = start[b * S]; q < start[x1]; q++) {
+          const size_t i = order[q];
+          cic_one(N, scaleBox, WPAR, density, pos[0][i], pos[1][i], pos[2][i]);
+        }
+      }
+    }
+    free(start);
+    free(slab);
+    free(order);
+    free(fill);
   }
   /* ghost slab (x = N) onto slab 0, with the +1 of src/auxPM.c:353-355 */
   const size_t slice = (size_t)N * 2 * (N / 2 + 1);
@@ -190,6 +230,7 @@ int or_mg_fifth_force(const or_cfg *c, const or_mg *m, double *dk, const double 
   const size_t total = or_real_len(N);
   if (!c->mg_active) return 0;
   if (c->model == OR_BRANSDICKE) {
+#pragma omp parallel for schedule(static)
     for (size_t i = 0; i < (size_t)N * N * H + (size_t)N * H; i++) { dk[2 * i] *= m->geff; dk[2 * i + 1] *= m->geff; }
     return 0;
   }
@@ -200,6 +241,7 @@ int or_mg_fifth_force(const or_cfg *c, const or_mg *m, double *dk, const double 
       /* DivideByLaplacian, src/mg.h:21-64 */
       double nf = 1.0 / pow((double)N, 3);
       nf *= 1.5 * c->omega / m->a * pow(c->box / OR_INVERSE_H0_MPCH / (2.0 * OR_PI), 2);
+#pragma omp parallel for schedule(static)
       for (int i = 0; i < N; i++)
         for (int j = 0; j < N; j++)
           for (int k = 0; k < H; k++) {
@@ -210,12 +252,14 @@ int or_mg_fifth_force(const or_cfg *c, const or_mg *m, double *dk, const double 
             one[2 * ind + 1] = nf * dk[2 * ind + 1] * KK;
           }
       or_fft_c2r_3d(N, one);
+#pragma omp parallel for schedule(static)
       for (size_t i = 0; i < total; i++) phik[i] = dx[i] * screen_potential(c, m, one[i]);
     } else {
       memcpy(phik, dx, sizeof(double) * total);
     }
     or_fft_r2c_3d(N, phik);
     /* EffDensitykToPhiofk, src/mg.h:74-116 */
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < N; i++)
       for (int j = 0; j < N; j++)
         for (int k = 0; k < H; k++) {
@@ -232,6 +276,7 @@ int or_mg_fifth_force(const or_cfg *c, const or_mg *m, double *dk, const double 
     } else {
       /* SmoothDensityField, src/mg.h:268-309 */
       const double nf = 1.0 / pow((double)N, 3);
+#pragma omp parallel for schedule(static)
       for (int i = 0; i < N; i++)
         for (int j = 0; j < N; j++)
           for (int k = 0; k < H; k++) {
@@ -242,6 +287,7 @@ int or_mg_fifth_force(const or_cfg *c, const or_mg *m, double *dk, const double 
             one[2 * ind + 1] = dk[2 * ind + 1] * s;
           }
       or_fft_c2r_3d(N, one);
+#pragma omp parallel for schedule(static)
       for (size_t i = 0; i < total; i++) phik[i] = dx[i] * (m->coupling * screen_density(c, m, one[i]));
       or_fft_r2c_3d(N, phik);
     }
@@ -261,9 +307,12 @@ void or_pm_forces(const or_cfg *c, double *dk, const double *phik, double *f0, d
   const int N = c->nmesh, H = N / 2 + 1;
   const double Scale = 2. * M_PI / c->box;
   const size_t ncplx = (size_t)N * N * H;
-  if (phik)
+  if (phik) {
+#pragma omp parallel for schedule(static)
     for (size_t i = 0; i < ncplx + (size_t)N * H; i++) { dk[2 * i] += phik[2 * i]; dk[2 * i + 1] += phik[2 * i + 1]; }
+  }
   double *F[3] = {f0, f1, f2};
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < N; i++)
     for (int j = 0; j < N; j++)
       for (int k = 0; k < H; k++) {
@@ -347,15 +396,20 @@ static inline double lpt_term(const float *const d1[3], const float *const d2[3]
 void or_kick(size_t np, float *const vel[3], float *const disp[3], const float *const d1[3],
              const float *const d2[3], double s1, double s2, int sd, double omega, double dda, double A,
              int use_cola, const double sumD[3], double sumV[3]) {
-  double s[3] = {0, 0, 0};
+  double s0 = 0, s1v = 0, s2v = 0;
+#pragma omp parallel for reduction(+ : s0, s1v, s2v) schedule(static)
   for (size_t n = 0; n < np; n++)
     for (int a = 0; a < 3; a++) {
       disp[a][n] -= sumD[a];
       double force = -1.5 * omega * disp[a][n] + (OR_PLANT == 8 ? 1 : -1) * use_cola * lpt_term(d1, d2, s1, s2, sd, a, n) / A;
       vel[a][n] += force * dda;
-      s[a] += vel[a][n];
+      if (a == 0) s0 += vel[a][n];
+      else if (a == 1) s1v += vel[a][n];
+      else s2v += vel[a][n];
     }
-  for (int a = 0; a < 3; a++) sumV[a] = s[a] / (double)np;
+  sumV[0] = s0 / (double)np;
+  sumV[1] = s1v / (double)np;
+  sumV[2] = s2v / (double)np;
 }
 
 /* periodic_wrap (float version, MEMORY_MODE), src/auxPM.c:649-655 */
@@ -370,6 +424,7 @@ static inline float wrapf(float x, double box) {
 void or_drift(size_t np, float *const pos[3], const float *const vel[3], const float *const d1[3],
               const float *const d2[3], double s1, double s2, int sd, double box, double dyyy, int use_cola,
               const double sumV[3]) {
+#pragma omp parallel for schedule(static)
   for (size_t n = 0; n < np; n++)
     for (int a = 0; a < 3; a++) {
       pos[a][n] += (vel[a][n] - sumV[a]) * dyyy;

@@ -225,35 +225,58 @@ def test_r2c_matches_oracle_and_numpy(n):
     assert rel_err(dk, ref) < 2e-6
 
 
+def check_pofk(got, ref):
+    """Identical mode counts, mean |k| to FP64 rounding and the binned power
+    (no shot-noise constant) within 0.1 % in every bin with k > 0."""
+    pk, kk, nm = got
+    rp, rk, rn = ref
+    assert np.array_equal(nm, rn)
+    m = (rn > 0) & (rk > 0)  # the k = 0 bin holds the mean, zero by construction
+    np.testing.assert_allclose(kk[m], rk[m], rtol=1e-10)  # FP64 sums in another order
+    np.testing.assert_allclose(pk[m], rp[m], rtol=1e-3, atol=0)
+
+
 def test_pofk_matches_oracle(pos):
-    g = M.MGPU(N, NS, BOX, OM, pofk_nbins=NBINS)
+    g = M.MGPU(N, NS, BOX, OM, pofk_nbins=NBINS, pofk_subtract_shotnoise=0)
     g.set_particles(pos)
-    pk, kk, nm = g.pm_density(pofk=True)
+    got = g.pm_density(pofk=True)
     c = O.cfg(N, NS, BOX, OM)
     d, _, _ = oracle_density(c, pos)
-    rp, rk, rn = O.pofk(c, d, NBINS, 0, 0.0, float(N), 1)
-    assert np.array_equal(nm, rn)
-    m = rn > 0
-    np.testing.assert_allclose(kk[m], rk[m], rtol=1e-12)
-    np.testing.assert_allclose(pk[m], rp[m], rtol=1e-3, atol=1e-3 * np.abs(rp).max())
+    check_pofk(got, O.pofk(c, d, NBINS, 0, 0.0, float(N), 0))
+
+
+@pytest.mark.parametrize("nbins", [2000, 20000, 21000])
+def test_pofk_many_bins_matches_oracle(pos, nbins):
+    """Bin counts beyond the LDS of the fused x pass fall back to k_pofk over
+    bin ranges: 20000 bins fit the 8-column density pair of the LCDM step but
+    not the 16-column x pass of the density call, 21000 fit neither; every path
+    agrees with the oracle."""
+    c = O.cfg(N, NS, BOX, OM)
+    d, _, _ = oracle_density(c, pos)
+    ref = O.pofk(c, d, nbins, 0, 0.0, float(N), 0)
+    g = M.MGPU(N, NS, BOX, OM, pofk_nbins=nbins, pofk_subtract_shotnoise=0)
+    g.set_particles(pos)
+    check_pofk(g.pm_density(pofk=True), ref)   # first call: k_pofk, records the geometry
+    check_pofk(g.pm_density(pofk=True), ref)   # fused x pass where the bins fit
+    _, got = g.get_displacements(pofk=True)
+    check_pofk(got, ref)
+    g.close()
 
 
 @pytest.mark.parametrize("bintype,kmin,kmax", [(0, 0.0, float(N)), (1, 1.0, float(N))])
 def test_pofk_binned_in_transform_matches_oracle(pos, bintype, kmin, kmax):
     """From the second call on the power is binned inside the r2c's last pass;
     mode counts and mean |k| come from the first call's full binning."""
-    g = M.MGPU(N, NS, BOX, OM, pofk_nbins=NBINS, pofk_bintype=bintype, pofk_kmin=kmin, pofk_kmax=kmax)
+    g = M.MGPU(N, NS, BOX, OM, pofk_nbins=NBINS, pofk_bintype=bintype, pofk_kmin=kmin, pofk_kmax=kmax,
+               pofk_subtract_shotnoise=0)
     g.set_particles(pos)
     first = g.pm_density(pofk=True)
     second = g.pm_density(pofk=True)
     c = O.cfg(N, NS, BOX, OM)
     d, _, _ = oracle_density(c, pos)
-    rp, rk, rn = O.pofk(c, d, NBINS, bintype, kmin, kmax, 1)
-    for pk, kk, nm in (first, second):
-        assert np.array_equal(nm, rn)
-        m = rn > 0
-        np.testing.assert_allclose(kk[m], rk[m], rtol=1e-12)
-        np.testing.assert_allclose(pk[m], rp[m], rtol=1e-3, atol=1e-3 * np.abs(rp).max())
+    ref = O.pofk(c, d, NBINS, bintype, kmin, kmax, 0)
+    for got in (first, second):
+        check_pofk(got, ref)
 
 
 def test_forces_and_gather_lcdm(pos):

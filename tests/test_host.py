@@ -270,6 +270,89 @@ def test_sigma8_normalisation(tmp_path):
     c.close()
 
 
+def growth_ode(om, mu, a_eval, zini=200.0):
+    """Raw growing mode from D = 1, dD/dln a = 1 at z_ini (the tables' start), independent of the host:
+    D'' + (2 + a H'/H) D' = 1.5 Om mu(a) / (a^3 H^2) D in x = ln a."""
+    def rhs(x, y):
+        a = np.exp(x)
+        H = np.sqrt(om / a ** 3 + 1 - om)
+        dH = -1.5 * om / a ** 4 / H
+        return [y[1], -(2 + a * dH / H) * y[1] + 1.5 * om * mu(a) / (a ** 3 * H * H) * y[0]]
+    x0 = np.log(1 / (1 + zini))
+    s = integrate.solve_ivp(rhs, (x0, 0.0), [1.0, 1.0], t_eval=np.log(a_eval), method="DOP853", rtol=1e-11,
+                            atol=1e-13)
+    return s.y[0]
+
+
+def ic_table(tmp_path, model, **over):
+    hp, c = cosmo_for(tmp_path, model=model, Nmesh=32, Nsample=16, Box=128.0, **over)
+    t = np.zeros(M.n2_table_len(32))
+    M.hlib().mgp_ic_pk_table(c.h, t.ctypes.data)
+    return hp, c, t
+
+
+def test_mg_pofk_ratio_scale_independent(tmp_path):
+    """nDGP, input P(k) for LCDM (src/2LPT.c:329-410, src/cosmo.c:688-756): with Sigma8
+    given for LCDM the IC power is P_LCDM (D_DGP(1)/D_LCDM(1))^2, the ratio of raw growing
+    modes from the same start (independent ODE); with Sigma8 given for the MG run the
+    ratio and the sigma8 enhancement cancel and the table is the unrescaled one."""
+    hp, c0, t0 = ic_table(tmp_path, "DGP", modified_gravity_active=1, input_pofk_is_for_lcdm=0)
+    rc = hp.rcH0_DGP
+    om = hp.Omega
+
+    def mu(a):
+        H = np.sqrt(om / a ** 3 + 1 - om)
+        dH = -1.5 * om / a ** 4 / H
+        return 1 + 1 / (3 * (1 + 2 * rc * (H + a * dH / 3)))
+
+    ratio = (growth_ode(om, mu, [1.0])[0] / growth_ode(om, lambda a: 1.0, [1.0])[0]) ** 2
+    _, c1, t1 = ic_table(tmp_path, "DGP", modified_gravity_active=1, input_pofk_is_for_lcdm=1,
+                         input_sigma8_is_for_lcdm=1)
+    assert ratio > 1.01
+    np.testing.assert_allclose(t1[1:] / t0[1:], ratio, rtol=2e-6)
+    assert c1.mg_pofk_ratio(0.1, 1.0) == pytest.approx(ratio, rel=2e-6)
+    assert c1.mg_sigma8_enhancement(1.0) == pytest.approx(np.sqrt(ratio), rel=2e-6)
+    _, c2, t2 = ic_table(tmp_path, "DGP", modified_gravity_active=1, input_pofk_is_for_lcdm=1,
+                         input_sigma8_is_for_lcdm=0)
+    np.testing.assert_allclose(t2, t0, rtol=1e-12)
+    for c in (c0, c1, c2):
+        c.close()
+
+
+def test_mg_pofk_ratio_scale_dependent(tmp_path):
+    """f(R) (scale-dependent): the ratio is (D(k, 1)/D_LCDM(1))^2 with D(k) the independent
+    ODE solution for mu(k, a) = 1 + k^2 / (3 (k^2 + a^2 m^2)), 1 on large scales; with Sigma8
+    given for the MG run the rescaled spectrum keeps sigma8 = Sigma8 (the enhancement is the
+    sigma8 ratio of the reference's 1000-point integral)."""
+    hp, c, t = ic_table(tmp_path, "FOFR", modified_gravity_active=1, input_pofk_is_for_lcdm=1,
+                        input_sigma8_is_for_lcdm=1)
+    om = hp.Omega
+    dl = growth_ode(om, lambda a: 1.0, [1.0])[0]
+    for k in (0.05, 0.3, 1.0):
+        k2 = (k * 2997.92458) ** 2
+        dk = growth_ode(om, lambda a: 1 + k2 / (3 * (k2 + a * a * c.mass2(a))), [1.0])[0]
+        assert c.mg_pofk_ratio(k, 1.0) == pytest.approx((dk / dl) ** 2, rel=5e-4), k
+    # towards 1 on large scales (the lowest tabulated k is half the fundamental mode)
+    assert 1.0 < c.mg_pofk_ratio(0.025, 1.0) < 1.01 < c.mg_pofk_ratio(0.3, 1.0) < c.mg_pofk_ratio(1.0, 1.0)
+    kf = 2 * np.pi / hp.Box
+    for n2 in (1, 50, 700):
+        k = kf * np.sqrt(n2)
+        assert t[n2] == pytest.approx(c.power_spec(k) * c.mg_pofk_ratio(k, 1.0), rel=1e-12)
+    s8 = c.mg_sigma8_enhancement(1.0)
+    kk = np.exp(np.linspace(np.log(1e-3), np.log(100.0), 1000))
+    w = 3 * (np.sin(8 * kk) - 8 * kk * np.cos(8 * kk)) / (8 * kk) ** 3
+    pw = np.array([c.power_spec(k) for k in kk])
+    rt = np.array([c.mg_pofk_ratio(k, 1.0) for k in kk])
+    s_mg = np.sqrt(np.sum(pw * rt * w * w * kk ** 3) / np.sum(pw * w * w * kk ** 3))
+    assert s8 == pytest.approx(s_mg, rel=1e-4)
+    assert s8 > 1.01
+    c.close()
+    _, c2, t2 = ic_table(tmp_path, "FOFR", modified_gravity_active=1, input_pofk_is_for_lcdm=1,
+                         input_sigma8_is_for_lcdm=0)
+    np.testing.assert_allclose(t2[1:], t[1:] / s8 ** 2, rtol=1e-12)
+    c2.close()
+
+
 # ---------------------------------------------------------------- output formats
 
 def test_gadget_snapshot_layout(tmp_path):
@@ -464,3 +547,24 @@ def test_fofrnu_growth_matches_ode(tmp_path):
         for i, a in enumerate(a_eval):
             assert c.growth_sd(0, k, a) == pytest.approx(d[i], rel=3e-4), (k, a)
     c.close()
+
+
+def test_ascii_snapshot_layout(tmp_path):
+    """The ASCII snapshot of builds without GADGET_STYLE (src/main.c:999-1027): particle
+    count on the first line, then x y z vx vy vz with %12.6f, positions scaled to Mpc/h."""
+    hp, c = cosmo_for(tmp_path, model="FOFR", Nsample=4, Nmesh=8)
+    c.close()
+    L = M.hlib()
+    L.mgp_write_ascii.argtypes = [C.POINTER(M.HostParams), C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_size_t]
+    npart = 50
+    rng = np.random.default_rng(2)
+    pos = rng.uniform(0, hp.Box, size=(3, npart)).astype(np.float32)
+    vel = (rng.normal(size=(3, npart)) * 300).astype(np.float32)
+    assert L.mgp_write_ascii(C.byref(hp), 0.25, 3.0, pos.ctypes.data, vel.ctypes.data, npart) == 0
+    lines = open(tmp_path / f"{hp.FileBase.decode()}_z3p000.0").read().splitlines()
+    assert lines[0] == str(npart) and len(lines) == npart + 1
+    assert len(lines[1]) == 6 * 12 + 5 and lines[1][12] == " "
+    rows = np.array([[float(x) for x in l.split()] for l in lines[1:]])
+    lf = hp.UnitLength_in_cm / 3.085678e24
+    np.testing.assert_allclose(rows[:, :3], (pos.T * lf).astype(np.float32), atol=5e-7)
+    np.testing.assert_allclose(rows[:, 3:], vel.T, atol=5e-7)

@@ -181,6 +181,25 @@ def test_cic_brute_force(n, ns):
     assert abs(d.sum()) < 1e-9
 
 
+@pytest.mark.parametrize("n,ns", [(32, 48), (24, 64), (96, 48)])
+def test_cic_brute_force_threaded_path(n, ns):
+    """More than 65536 particles take the oracle's threaded deposit (x-slab blocks in
+    two phases): clumps across block edges, particles in the last slab and on x = 0."""
+    rng = np.random.default_rng(n + ns)
+    box = 40.0
+    npart = ns ** 3
+    pos = rng.uniform(0, box, size=(3, npart))
+    k = npart // 3
+    pos[:, :k] = rng.normal(box / 2, box / n, size=(3, k))       # a clump over the middle slabs
+    pos[0, k:k + 1000] = rng.uniform(box * (1 - 1.0 / n), box, 1000)  # the last slab
+    pos[0, k + 1000:k + 2000] = 0.0
+    pos = np.mod(pos, box).astype(np.float32)
+    pos[pos >= np.float32(box)] = 0.0
+    d = O.real_view(O.deposit(O.cfg(n, ns, box, 0.3), pos), n)
+    ref = brute_cic(n, ns, box, pos)
+    np.testing.assert_allclose(d, ref, atol=1e-9 * max(1.0, np.abs(ref).max()))
+
+
 def test_gather_is_adjoint_of_deposit():
     n, ns, box = 16, 8, 40.0
     rng = np.random.default_rng(3)
