@@ -32,6 +32,8 @@ import sys
 import tempfile
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "mg-picola-public_amd"))
 
@@ -248,10 +250,12 @@ def ensemble_leg(per_size=4, threads=12, nsteps=20):
                         f"(Box = 2 Np), fixed-amplitude pairs, {nsteps} steps; wall time incl. set-up and ICs"}
 
 
-def cpu_baseline(ns=256, nmesh=512, box=512.0):
-    """One COLA step of the same f(R) + screening + scale-dependent path in the
-    FP64 oracle: deposit, r2c, P(k), fifth force (2 FFTs), Forces (3 c2r),
-    gather, 4 LPT fields (3 c2r each), Kick, Drift."""
+def cpu_sample(ns, nmesh, box):
+    """Phase times [s] of one COLA step of the metric's path (f(R) with screening,
+    scale-dependent LPT fields, P(k) with 512 bins) in the FP64 oracle at ns^3
+    particles on an nmesh^3 mesh: deposit, r2c + P(k), fifth force (2 FFTs and
+    the screening), Forces (3 c2r), gather, the 4 LPT fields (3 c2r each and the
+    lattice read-out), Kick and Drift.  Returns (phases, threads)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import numpy as np
@@ -259,30 +263,94 @@ def cpu_baseline(ns=256, nmesh=512, box=512.0):
     from helpers import lattice_particles
 
     O.lib()
-    pos = lattice_particles(ns, box, amp=2.0, seed=7, clustered=True)
+    pos = lattice_particles(ns, box, amp=2.0 * box / ns / 2, seed=7, clustered=True)
     vel = np.zeros_like(pos)
     c = O.cfg(nmesh, ns, box, 0.3, model=O.FOFR, mg_active=1, include_screening=1)
     mg = O.MG(a=0.5, coupling=1.0 / 3.0, mass2=7e5, phi_crit=5e-6, dgp_fac=0, rsmooth=1, geff=1)
     cd = O.new_mesh(nmesh)
     g = np.ones(3 * (nmesh // 2) ** 2 + 1)
     d1 = np.zeros_like(pos)
+    t = {}
     t0 = time.perf_counter()
     d = O.deposit(c, pos)
+    t["deposit"] = time.perf_counter() - t0
     dx = d.copy()
+    t0 = time.perf_counter()
     O.fft_r2c_3d(nmesh, d)
     O.pofk(c, d, 512, 0, 0.0, float(nmesh), 1)
+    t["r2c_pofk"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
     phik = O.fifth_force(c, mg, d, dx)
+    t["fifth_force"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
     f = O.forces(c, d, phik)
+    t["forces"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
     disp, sD = O.gather(c, pos, f)
+    t["gather"] = time.perf_counter() - t0
+    del f, phik, dx, d
+    t0 = time.perf_counter()
     for _ in range(4):
         O.lpt_field(c, cd, g)
+    t["lpt_fields"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
     sV = O.kick(vel, disp, d1, d1, 0, 0, 1, 0.3, 0.1, 0.5, 1, sD)
     O.drift(pos, vel, d1, d1, 0, 0, 1, box, 0.1, 1, sV)
-    dt = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=ns ** 3 / dt, unit="particle-steps/s", cores=cores, kind="port",
-                sample=f"one f(R)+screening+scale-dependent COLA step of the FP64 oracle at {ns}^3 particles on a "
-                       f"{nmesh}^3 mesh ({dt:.1f} s)")
+    t["kick_drift"] = time.perf_counter() - t0
+    return t, O.num_threads()
+
+
+# phases whose work scales with the particle count; the rest scale with the mesh as G log G
+CPU_PARTICLE_PHASES = ("deposit", "gather", "kick_drift")
+
+
+def cpu_extrapolate(t, ns0, n0, ns1, n1):
+    """Seconds per step at (ns1, n1) from phase times at (ns0, n0): particle phases
+    scale with Ns^3, mesh phases (FFT-dominated) with N^3 log N^3."""
+    mesh = (n1 ** 3 * np.log(n1 ** 3)) / (n0 ** 3 * np.log(n0 ** 3))
+    part = (ns1 / ns0) ** 3
+    return sum(v * (part if k in CPU_PARTICLE_PHASES else mesh) for k, v in t.items())
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(ns=256, nmesh=512, box=512.0, ns1=128, nmesh1=256, target=(512, 1024)):
+    """The oracle on this host's cores: one step at ns^3/nmesh^3 on all the
+    threads OpenMP is given, and (in a child process with OMP_NUM_THREADS=1)
+    one step at ns1^3/nmesh1^3 on one core; both extrapolated phase by phase to
+    the metric's 512^3 particles on a 1024^3 mesh."""
+    import subprocess
+    t, threads = cpu_sample(ns, nmesh, box)
+    sec = cpu_extrapolate(t, ns, nmesh, *target)
+    value = target[0] ** 3 / sec
+    code = (f"import sys, json; sys.path.insert(0, {ROOT!r}); import bench; "
+            f"t, n = bench.cpu_sample({ns1}, {nmesh1}, {float(nmesh1)}); print(json.dumps([t, n]))")
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    one = None
+    try:
+        r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=600)
+        t1, n1 = json.loads(r.stdout.strip().splitlines()[-1])
+        if n1 == 1:
+            one = dict(value=round(target[0] ** 3 / cpu_extrapolate(t1, ns1, nmesh1, *target), 1),
+                       sample_seconds=round(sum(t1.values()), 2),
+                       sample=f"one step at {ns1}^3 particles on a {nmesh1}^3 mesh, 1 thread")
+    except Exception as e:  # the all-core figure stands on its own
+        print(f"[bench] one-core CPU sample failed: {e}", file=sys.stderr, flush=True)
+    return dict(value=round(value, 1), unit="particle-steps/s", cores=threads, kind="port",
+                sample=(f"one f(R)+screening+scale-dependent COLA step of the FP64 oracle (-O3, OpenMP) at {ns}^3 "
+                        f"particles on a {nmesh}^3 mesh ({sum(t.values()):.1f} s), extrapolated phase by phase to "
+                        f"{target[0]}^3 on {target[1]}^3 (particle phases x Ns^3, mesh phases x N^3 log N^3)"),
+                seconds_per_step=round(sec, 2), phases_s={k: round(v, 3) for k, v in t.items()},
+                one_core=one, nproc=os.cpu_count(), cpu_model=cpu_model(), fft="the oracle's own radix FFT "
+                "(no FFTW on the host)")
 
 
 def traffic_of(workload, symbol):

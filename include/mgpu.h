@@ -45,9 +45,10 @@ enum { MGPU_LCDM = 0, MGPU_FOFR = 1, MGPU_DGP = 2, MGPU_BRANSDICKE = 3, MGPU_MBE
 enum { MGPU_FILTER_NONE = 0, MGPU_FILTER_GAUSSIAN = 1, MGPU_FILTER_TOPHAT = 2, MGPU_FILTER_SHARPK = 3 };
 /* Mesh ids for mgpu_get_mesh / mgpu_set_mesh. */
 enum {
-  MGPU_MESH_DENSITY_X = 0,  /* delta(x) after PtoMesh's deposit (real) */
+  MGPU_MESH_DENSITY_X = 0,  /* delta(x) after PtoMesh's deposit (real); the r2c (without MG) or the force
+                               pass (with MG) reuse its memory, so read it before them */
   MGPU_MESH_DENSITY_K = 1,  /* delta(k) after the r2c (complex), P3D */
-  MGPU_MESH_PHI_K = 2,      /* fifth-force potential phi(k) (complex), P3D_mgarray_two */
+  MGPU_MESH_PHI_K = 2,      /* fifth-force potential phi(k) (complex), P3D_mgarray_two; until mgpu_forces */
   MGPU_MESH_FORCE_X = 3,    /* force meshes after the c2r (real), N11 N12 N13 */
   MGPU_MESH_FORCE_Y = 4,
   MGPU_MESH_FORCE_Z = 5,
@@ -98,6 +99,24 @@ typedef struct {
 size_t mgpu_n2_table_len(int nmesh);
 const char *mgpu_last_error(const void *ctx_or_null);
 int mgpu_device_info(char *name, size_t name_len, size_t *global_mem, int *cu_count, size_t *lds_per_block);
+
+/* Device memory of a context, by part: mgpu_create allocates the sum of all
+ * parts but MGPU_MEM_RSD (total), and fails with MGPU_E_NOMEM, before
+ * allocating, when that exceeds the free device memory; the RSD records are
+ * allocated by the first mgpu_rsd_power_spectrum.  The reference allocates and
+ * frees its meshes per step instead (MEMORY_MODE, src/auxPM.c:46-102).
+ * parts: MGPU_MEM_PARTS sizes in bytes, or NULL.  Needs no device. */
+enum {
+  MGPU_MEM_PARTICLES = 0, /* Pos, Vel, Disp and the two LPT slots: 60 bytes per particle */
+  MGPU_MEM_MESHES,        /* the mesh slots of the step */
+  MGPU_MEM_DEPOSIT,       /* tile bins and particle records of the deposit and gather */
+  MGPU_MEM_STORED,        /* scale-dependent: the stored delta(k) and 2LPT source (cdelta_cdm, cdelta_cdm2) */
+  MGPU_MEM_LATTICE,       /* scale-dependent with Nmesh % Nsample == 0: the folded lattice spectra */
+  MGPU_MEM_TABLES,        /* reduction scratch, P(k) bins and windows, per-|n|^2 tables, twiddles */
+  MGPU_MEM_RSD,           /* redshift-space particle records (first RSD call) */
+  MGPU_MEM_PARTS
+};
+int mgpu_memory_plan(const mgpu_params *p, size_t *parts, size_t *total);
 
 /* ---- context; replaces initialize_ffts/initialize_parts + the MEMORY_MODE
  *      allocations of GetDisplacements (src/2LPT.c:47-176, src/auxPM.c:46-102) ---- */

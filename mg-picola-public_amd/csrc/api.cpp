@@ -3,6 +3,7 @@
 #include "mgpu_internal.h"
 
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -140,6 +141,42 @@ int mgpu_device_info(char *name, size_t name_len, size_t *global_mem, int *cu_co
   return MGPU_OK;
 }
 
+// Device bytes of a context, by part (the plan mgpu_create allocates).
+static void memory_plan(const mgpu_params *p, size_t part[MGPU_MEM_PARTS]) {
+  const size_t N = p->nmesh, Np = p->nsample, np = Np * Np * Np;
+  const size_t hp = ((N / 2 + 1) + 15) / 16 * 16, sb = N * N * hp * sizeof(float2);
+  for (int i = 0; i < MGPU_MEM_PARTS; i++) part[i] = 0;
+  part[MGPU_MEM_PARTICLES] = 5 * 3 * np * sizeof(float);
+  part[MGPU_MEM_MESHES] = MGPU_NSLOTS * sb;
+  part[MGPU_MEM_DEPOSIT] = deposit_scratch_bytes((int)N, np);
+  if (p->scale_dependent) {
+    part[MGPU_MEM_STORED] = 2 * sb;
+    if (N % Np == 0 && fft_size_supported((int)Np)) {
+      const size_t hps = ((Np / 2 + 1) + 15) / 16 * 16;
+      part[MGPU_MEM_LATTICE] = 6 * Np * Np * hps * sizeof(float2);
+    }
+  }
+  const size_t nb = (size_t)std::max(1, p->pofk_nbins);
+  part[MGPU_MEM_TABLES] = (3 * std::max<size_t>((np + 255) / 256, N) + 3 * 1024) * sizeof(double) +
+                          4 * nb * sizeof(double) + (N / 2 + 1) * 12 + 2 * N * sizeof(float2) +
+                          mgpu_n2_table_len((int)N) * (2 * sizeof(double) + 2 * sizeof(float) + sizeof(int16_t));
+  part[MGPU_MEM_RSD] = np * sizeof(float4);
+}
+
+int mgpu_memory_plan(const mgpu_params *p, size_t *parts, size_t *total) {
+  if (!p || !fft_size_supported(p->nmesh) || p->nsample < 1 || p->nsample > 2048)
+    return mgpu_fail(nullptr, MGPU_E_INVALID, "bad Nmesh or Nsample");
+  size_t part[MGPU_MEM_PARTS];
+  memory_plan(p, part);
+  size_t t = 0;
+  for (int i = 0; i < MGPU_MEM_PARTS; i++) {
+    if (parts) parts[i] = part[i];
+    if (i != MGPU_MEM_RSD) t += part[i];
+  }
+  if (total) *total = t;
+  return MGPU_OK;
+}
+
 int mgpu_create(const mgpu_params *p, mgpu_ctx **out) {
   if (!p || !out) return mgpu_fail(nullptr, MGPU_E_INVALID, "null argument");
   *out = nullptr;
@@ -152,6 +189,24 @@ int mgpu_create(const mgpu_params *p, mgpu_ctx **out) {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
     (void)hipGetLastError();
     return mgpu_fail(nullptr, MGPU_E_NODEVICE, "no HIP device visible");
+  }
+  {
+    // the plan against the free device memory, before anything is allocated
+    size_t part[MGPU_MEM_PARTS], need = 0, free_b = 0, total_b = 0;
+    mgpu_memory_plan(p, part, &need);
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && need > free_b) {
+      char msg[512];
+      const double G = 1e9;
+      std::snprintf(msg, sizeof msg,
+                    "the run needs %.1f GB of device memory (particles %.1f, %d mesh slots %.1f, deposit %.1f, "
+                    "stored spectra %.1f, lattice spectra %.1f, tables %.2f; RSD records %.1f more on the first "
+                    "RSD call) but %.1f GB are free",
+                    need / G, part[MGPU_MEM_PARTICLES] / G, MGPU_NSLOTS, part[MGPU_MEM_MESHES] / G,
+                    part[MGPU_MEM_DEPOSIT] / G, part[MGPU_MEM_STORED] / G, part[MGPU_MEM_LATTICE] / G,
+                    part[MGPU_MEM_TABLES] / G, part[MGPU_MEM_RSD] / G, free_b / G);
+      return mgpu_fail(nullptr, MGPU_E_NOMEM, msg);
+    }
+    (void)hipGetLastError();
   }
   mgpu_ctx *ctx = new mgpu_ctx();
   ctx->p = *p;
@@ -179,11 +234,8 @@ int mgpu_create(const mgpu_params *p, mgpu_ctx **out) {
   for (int s = 0; s < 2; s++)
     if ((rc = alloc(ctx, (void **)&ctx->lpt[s], pb, "LPT slot"))) return fail(rc);
   const size_t sb = ctx->slot_elems * sizeof(float2);
-  for (int s = 0; s < 3; s++)
+  for (int s = 0; s < MGPU_NSLOTS; s++)
     if ((rc = alloc(ctx, (void **)&ctx->slot[s], sb, "mesh slot"))) return fail(rc);
-  float2 *force = nullptr;
-  if ((rc = alloc(ctx, (void **)&force, 3 * sb, "force meshes"))) return fail(rc);
-  for (int a = 0; a < 3; a++) ctx->slot[3 + a] = force + a * ctx->slot_elems;
   ctx->dep_scratch_bytes = deposit_scratch_bytes(ctx->N, ctx->np);
   if ((rc = alloc(ctx, &ctx->dep_scratch, ctx->dep_scratch_bytes, "deposit scratch"))) return fail(rc);
   if (p->scale_dependent) {
@@ -249,11 +301,10 @@ int mgpu_create(const mgpu_params *p, mgpu_ctx **out) {
       hipMemsetAsync(ctx->disp, 0, pb, ctx->stream) != hipSuccess)
     return fail(mgpu_fail(ctx, MGPU_E_HIP, "memset failed"));
   // meshes too: the padding entries of each row are never written by the passes
-  for (int s = 0; s < 3; s++)
+  for (int s = 0; s < MGPU_NSLOTS; s++)
     if (hipMemsetAsync(ctx->slot[s], 0, sb, ctx->stream) != hipSuccess)
       return fail(mgpu_fail(ctx, MGPU_E_HIP, "memset failed"));
-  if (hipMemsetAsync(ctx->slot[3], 0, 3 * sb, ctx->stream) != hipSuccess ||
-      (ctx->lattice_slot && hipMemsetAsync(ctx->lattice_slot, 0,
+  if ((ctx->lattice_slot && hipMemsetAsync(ctx->lattice_slot, 0,
                                            6 * (size_t)ctx->Np * ctx->Np * ctx->fft_lattice.hp * sizeof(float2),
                                            ctx->stream) != hipSuccess))
     return fail(mgpu_fail(ctx, MGPU_E_HIP, "memset failed"));

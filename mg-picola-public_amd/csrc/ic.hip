@@ -139,9 +139,9 @@ __global__ __launch_bounds__(kBlock) void k_lpt_spectrum(int mode, const float2 
 
 // 2LPT source on padded real meshes: src = d00 (d11 + d22) + d11 d22 (first
 // call), src -= d*d (off-diagonal calls).  src/2LPT.c:1282-1290
-__global__ __launch_bounds__(kBlock) void k_source_diag(const float *__restrict__ d00, const float *__restrict__ d11,
-                                                        const float *__restrict__ d22, float *__restrict__ src,
-                                                        size_t n) {
+// src may be d00 (each element is read before it is written)
+__global__ __launch_bounds__(kBlock) void k_source_diag(const float *d00, const float *__restrict__ d11,
+                                                        const float *__restrict__ d22, float *src, size_t n) {
   size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   const double a = d00[e], b = d11[e], c = d22[e];
@@ -597,29 +597,30 @@ int ic_displacement_fields(mgpu_ctx *ctx, unsigned long seed, int sphere_mode, i
   MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
   MGPU_HIP(ctx, hipFree(dseed));
 
-  // displacement gradient -> 2LPT source (real) in slot2
+  // displacement gradient -> 2LPT source (real), built in slot 1 over the
+  // psi_00 gradient (the diagonal gradients in slots 1-3)
   const size_t nr = 2 * ctx->slot_elems;
   const unsigned gr = cdiv(nr, kBlock);
   static const int ga[3] = {0, 1, 2};
   for (int g = 0; g < 3; g++) {
-    if ((rc = spectrum(ctx, SPEC_GRAD, cd, nullptr, ctx->slot[3 + g], ga[g], ga[g]))) return rc;
-    if ((rc = fft_c2r_3d(ctx, &ctx->fft, ctx->slot[3 + g]))) return rc;
+    if ((rc = spectrum(ctx, SPEC_GRAD, cd, nullptr, ctx->slot[1 + g], ga[g], ga[g]))) return rc;
+    if ((rc = fft_c2r_3d(ctx, &ctx->fft, ctx->slot[1 + g]))) return rc;
   }
-  hipLaunchKernelGGL(k_source_diag, dim3(gr), dim3(kBlock), 0, ctx->stream, (const float *)ctx->slot[3],
-                     (const float *)ctx->slot[4], (const float *)ctx->slot[5], (float *)ctx->slot[2], nr);
+  hipLaunchKernelGGL(k_source_diag, dim3(gr), dim3(kBlock), 0, ctx->stream, (const float *)ctx->slot[1],
+                     (const float *)ctx->slot[2], (const float *)ctx->slot[3], (float *)ctx->slot[1], nr);
   MGPU_HIP(ctx, hipGetLastError());
   static const int oa[3] = {0, 0, 1}, ob[3] = {1, 2, 2};
   for (int g = 0; g < 3; g++) {
     if ((rc = spectrum(ctx, SPEC_GRAD, cd, nullptr, ctx->slot[3], oa[g], ob[g]))) return rc;
     if ((rc = fft_c2r_3d(ctx, &ctx->fft, ctx->slot[3]))) return rc;
     hipLaunchKernelGGL(k_source_off, dim3(gr), dim3(kBlock), 0, ctx->stream, (const float *)ctx->slot[3],
-                       (float *)ctx->slot[2], nr);
+                       (float *)ctx->slot[1], nr);
     MGPU_HIP(ctx, hipGetLastError());
   }
-  if ((rc = fft_r2c_3d(ctx, &ctx->fft, ctx->slot[2]))) return rc;
+  if ((rc = fft_r2c_3d(ctx, &ctx->fft, ctx->slot[1]))) return rc;
 
   if (ctx->p.scale_dependent) {
-    if ((rc = spectrum(ctx, SPEC_NEG, ctx->slot[2], nullptr, ctx->cdelta[1], 0, 0))) return rc;
+    if ((rc = spectrum(ctx, SPEC_NEG, ctx->slot[1], nullptr, ctx->cdelta[1], 0, 0))) return rc;
     return MGPU_OK;
   }
   // ZA and 2LPT displacements at the lattice, mean removed
@@ -628,9 +629,9 @@ int ic_displacement_fields(mgpu_ctx *ctx, unsigned long seed, int sphere_mode, i
     if ((rc = spectrum(ctx, SPEC_DISP, cd, nullptr, ctx->slot[3], a, 0))) return rc;
     if ((rc = fft_c2r_3d(ctx, &ctx->fft, ctx->slot[3]))) return rc;
     if ((rc = readout(ctx, ctx->slot[3], 1.0, ctx->lpt[0] + (size_t)a * ctx->np))) return rc;
-    if ((rc = spectrum(ctx, SPEC_DISP2, ctx->slot[2], nullptr, ctx->slot[4], a, 0))) return rc;
-    if ((rc = fft_c2r_3d(ctx, &ctx->fft, ctx->slot[4]))) return rc;
-    if ((rc = readout(ctx, ctx->slot[4], -3. / 7. * inv_n3, ctx->lpt[1] + (size_t)a * ctx->np))) return rc;
+    if ((rc = spectrum(ctx, SPEC_DISP2, ctx->slot[1], nullptr, ctx->slot[2], a, 0))) return rc;
+    if ((rc = fft_c2r_3d(ctx, &ctx->fft, ctx->slot[2]))) return rc;
+    if ((rc = readout(ctx, ctx->slot[2], -3. / 7. * inv_n3, ctx->lpt[1] + (size_t)a * ctx->np))) return rc;
   }
   MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return MGPU_OK;

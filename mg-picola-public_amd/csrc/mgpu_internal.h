@@ -19,6 +19,8 @@
 
 #define MGPU_PI 3.14159265358979323846
 #define MGPU_INVERSE_H0_MPCH 2997.92458
+// mesh slots of a context
+#define MGPU_NSLOTS 4
 // columns per tile of the strided FFT passes (16: 128-byte row segments, 8: 64-byte)
 #ifndef MGPU_FFT_TCOLS
 #define MGPU_FFT_TCOLS 16
@@ -45,14 +47,19 @@ struct mgpu_ctx {
   float *pos = nullptr, *vel = nullptr, *disp = nullptr;
   float *lpt[2] = {nullptr, nullptr};
 
-  // mesh slots: 0 delta(x) -> delta(k)   1 delta(x) copy (MG)   2 MG potential work, phi(k)
-  //             3..5 force components (x in 3, y in 5, z in 4; deposit accumulator before Forces)
-  float2 *slot[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  // mesh slots (MGPU_NSLOTS, each N*N*HP complex64), reused through the step:
+  //   0  delta(x) -> delta(k) (kept until the next deposit)
+  //   1  delta(x) copy for the MG screening; then the force pair's G -> F_z
+  //   2  MG potential work, phi(k); then i n_y G -> F_y
+  //   3  i n_x G -> F_x
+  // so the force meshes are slots {3, 2, 1} and delta(x) of an MG step and phi(k)
+  // last until the force pass.  The initial conditions use slots 1-3 as work.
+  float2 *slot[MGPU_NSLOTS] = {nullptr, nullptr, nullptr, nullptr};
   float2 *cdelta[2] = {nullptr, nullptr};
   bool have_phik = false;
   int dx_slot = 0;                  // slot holding delta(x) (1 when the MG screening needs it kept)
   int phi_slot = 2;                 // slot holding phi(k) after the fifth force
-  int force_slot[3] = {3, 5, 4};    // slots of the x, y, z force meshes
+  int force_slot[3] = {3, 2, 1};    // slots of the x, y, z force meshes
 
   // reductions
   double *partial = nullptr;  // per-block partial sums

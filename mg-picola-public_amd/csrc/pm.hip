@@ -568,18 +568,20 @@ int pm_fifth_force(mgpu_ctx *ctx, const mgpu_mg *mg) {
 }
 
 // The force tail shared by pm_forces and the fused step: slot 3 holds the x
-// transform of i n_x G and slot 4 that of G; one dual y pass, the y pass of
-// slot 3 and the three z c2r give F_x (slot 3), F_y (slot 5), F_z (slot 4).
+// transform of i n_x G and slot 1 that of G (the delta(x) copy and phi(k) are
+// used up by then); one dual y pass (i n_y G into slot 2), the y pass of slot 3
+// and the three z c2r give F_x (slot 3), F_y (slot 2), F_z (slot 1).
 static int forces_tail(mgpu_ctx *ctx) {
   int rc;
-  if ((rc = fused_dual_y(ctx, ctx->slot[4], ctx->slot[5]))) return rc;
+  ctx->have_phik = false;
+  if ((rc = fused_dual_y(ctx, ctx->slot[1], ctx->slot[2]))) return rc;
   if ((rc = fft_pass_col(ctx, &ctx->fft, ctx->slot[3], 1, 0))) return rc;
   if ((rc = fft_pass_z(ctx, &ctx->fft, ctx->slot[3], 0, nullptr))) return rc;
-  if ((rc = fft_pass_z(ctx, &ctx->fft, ctx->slot[5], 0, nullptr))) return rc;
-  if ((rc = fft_pass_z_ikz(ctx, &ctx->fft, ctx->slot[4]))) return rc;
+  if ((rc = fft_pass_z(ctx, &ctx->fft, ctx->slot[2], 0, nullptr))) return rc;
+  if ((rc = fft_pass_z_ikz(ctx, &ctx->fft, ctx->slot[1]))) return rc;
   ctx->force_slot[0] = 3;
-  ctx->force_slot[1] = 5;
-  ctx->force_slot[2] = 4;
+  ctx->force_slot[1] = 2;
+  ctx->force_slot[2] = 1;
   return MGPU_OK;
 }
 
@@ -620,7 +622,7 @@ int pm_step_fused(mgpu_ctx *ctx, const mgpu_mg *mg, mgpu_pofk *out, double sumD[
   const double cnorm = force_norm(ctx);
   if (!ctx->p.mg_active || ctx->p.model == MGPU_LCDM) {
     PhaseTimer pt(ctx, &ctx->t.forces);
-    if ((rc = fused_x_pair(ctx, FUSED_FWD_DENSITY, FUSED_GREEN_FORCE, dk, dk, nullptr, ctx->slot[3], ctx->slot[4], cnorm,
+    if ((rc = fused_x_pair(ctx, FUSED_FWD_DENSITY, FUSED_GREEN_FORCE, dk, dk, nullptr, ctx->slot[3], ctx->slot[1], cnorm,
                            0.0, 0, 0.0, 0.0, pk)))
       return rc;
   } else {
@@ -642,7 +644,7 @@ int pm_step_fused(mgpu_ctx *ctx, const mgpu_mg *mg, mgpu_pofk *out, double sumD[
       return rc;
     if ((rc = fft_pass_col(ctx, &ctx->fft, phi, 1, 1))) return rc;
     if ((rc = fused_x_pair(ctx, dgp ? FUSED_FWD_PLAIN : FUSED_FWD_EPI, FUSED_GREEN_FORCE, phi, nullptr, dk,
-                           ctx->slot[3], ctx->slot[4], cnorm, 0.0, 0, dgp ? 0.0 : mg->coupling,
+                           ctx->slot[3], ctx->slot[1], cnorm, 0.0, 0, dgp ? 0.0 : mg->coupling,
                            dgp ? 0.0 : fofr_massterm2(ctx, mg), nullptr)))
       return rc;
   }
@@ -665,7 +667,7 @@ int pm_forces(mgpu_ctx *ctx) {
   PhaseTimer pt(ctx, &ctx->t.forces);
   int rc;
   if ((rc = fused_green_x(ctx, FUSED_GREEN_FORCE, ctx->slot[0], ctx->have_phik ? ctx->slot[ctx->phi_slot] : nullptr,
-                          ctx->slot[3], ctx->slot[4], force_norm(ctx), 0.0, 0)))
+                          ctx->slot[3], ctx->slot[1], force_norm(ctx), 0.0, 0)))
     return rc;
   return forces_tail(ctx);
 }

@@ -29,7 +29,7 @@ MGPU_SYMBOLS = [
     "mgpu_get_displacements", "mgpu_kick", "mgpu_drift", "mgpu_ic_displacement_fields", "mgpu_init_particles",
     "mgpu_lpt_fields", "mgpu_get_mesh", "mgpu_set_mesh", "mgpu_get_timers", "mgpu_enable_timers",
     "mgpu_kernel_name", "mgpu_enable_kernel_timing", "mgpu_kernel_timing", "mgpu_lpt_fields_slices",
-    "mgpu_lpt_fields_slices2",
+    "mgpu_lpt_fields_slices2", "mgpu_memory_plan",
 ]
 
 
@@ -135,18 +135,41 @@ def _f32(a, shape):
     return a
 
 
+MEM_PARTS = ("particles", "meshes", "deposit", "stored", "lattice", "tables", "rsd")
+
+
+def make_params(nmesh, nsample, box, omega, model=LCDM, mg_active=0, include_screening=0, filter=FILTER_NONE,
+                scale_dependent=0, use_cola=1, pofk_nbins=0, pofk_bintype=0, pofk_subtract_shotnoise=1,
+                pofk_kmin=0.0, pofk_kmax=0.0):
+    if pofk_nbins <= 0:
+        pofk_nbins = nmesh
+    if pofk_kmax <= pofk_kmin:
+        pofk_kmin, pofk_kmax = (0.0 if pofk_bintype == 0 else 1.0), float(nmesh)
+    return Params(nmesh, nsample, box, omega, model, mg_active, include_screening, filter, scale_dependent,
+                  use_cola, pofk_nbins, pofk_bintype, pofk_subtract_shotnoise, pofk_kmin, pofk_kmax)
+
+
+def memory_plan(nmesh, nsample, scale_dependent=0, pofk_nbins=0):
+    """({part: bytes}, total bytes mgpu_create allocates) without a device."""
+    p = make_params(nmesh, nsample, 1.0, 0.3, scale_dependent=scale_dependent, pofk_nbins=pofk_nbins)
+    parts = (C.c_size_t * len(MEM_PARTS))()
+    total = C.c_size_t()
+    L = lib()
+    L.mgpu_memory_plan.argtypes = [C.POINTER(Params), C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
+    if L.mgpu_memory_plan(C.byref(p), parts, C.byref(total)):
+        raise MGPUError(f"mgpu_memory_plan failed: {L.mgpu_last_error(None).decode()}")
+    return dict(zip(MEM_PARTS, list(parts))), total.value
+
+
 class MGPU:
     """One simulation context on the GPU (owns particles, meshes, FFT tables)."""
 
     def __init__(self, nmesh, nsample, box, omega, model=LCDM, mg_active=0, include_screening=0,
                  filter=FILTER_NONE, scale_dependent=0, use_cola=1, pofk_nbins=0, pofk_bintype=0,
                  pofk_subtract_shotnoise=1, pofk_kmin=0.0, pofk_kmax=0.0):
-        if pofk_nbins <= 0:
-            pofk_nbins = nmesh
-        if pofk_kmax <= pofk_kmin:
-            pofk_kmin, pofk_kmax = (0.0 if pofk_bintype == 0 else 1.0), float(nmesh)
-        self.p = Params(nmesh, nsample, box, omega, model, mg_active, include_screening, filter, scale_dependent,
-                        use_cola, pofk_nbins, pofk_bintype, pofk_subtract_shotnoise, pofk_kmin, pofk_kmax)
+        self.p = make_params(nmesh, nsample, box, omega, model, mg_active, include_screening, filter,
+                             scale_dependent, use_cola, pofk_nbins, pofk_bintype, pofk_subtract_shotnoise, pofk_kmin,
+                             pofk_kmax)
         self.n, self.ns, self.np = nmesh, nsample, nsample ** 3
         self.ctx = C.c_void_p()
         rc = lib().mgpu_create(C.byref(self.p), C.byref(self.ctx))

@@ -38,6 +38,8 @@
 extern "C" {
 #endif
 
+int or_num_threads(void); /* threads of the OpenMP loops (OMP_NUM_THREADS) */
+
 /* model ids (compile-time MODEL= switch of the reference Makefile:33-110) */
 enum { OR_LCDM = 0, OR_FOFR = 1, OR_DGP = 2, OR_BRANSDICKE = 3, OR_MBETA = 4 };
 /* smoothing filters for density screening (src/mg.h:318-338) */

@@ -83,6 +83,8 @@ class using:
 
 def _load(path):
     L = C.CDLL(path)
+    L.or_num_threads.restype = C.c_int
+    L.or_num_threads.argtypes = []
     L.or_real_len.restype = C.c_size_t
     L.or_real_len.argtypes = [C.c_int]
     L.or_fft_c2c_1d.argtypes = [C.c_int, _D, C.c_int]
@@ -115,6 +117,10 @@ def _load(path):
     L.or_ic_displacements.argtypes = [C.POINTER(Cfg), _D, C.c_int, _P3, _P3, C.c_void_p]
     L.or_lpt_field.argtypes = [C.POINTER(Cfg), _D, _D, _P3]
     return L
+
+
+def num_threads():
+    return lib().or_num_threads()
 
 
 def cfg(nmesh, nsample, box, omega, model=LCDM, mg_active=0, include_screening=0, filter=FILTER_NONE):

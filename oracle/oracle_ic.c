@@ -11,6 +11,7 @@
  */
 #include "oracle.h"
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -303,6 +304,7 @@ void or_lpt_field(const or_cfg *c, const double *cd, const double *g_n2, float *
   const size_t len = or_real_len(N);
   double *f[3];
   for (int a = 0; a < 3; a++) f[a] = (double *)calloc(len, sizeof(double));
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < N; i++)
     for (int j = 0; j < N; j++)
       for (int k = 0; k < H; k++) {
@@ -325,6 +327,7 @@ void or_lpt_field(const or_cfg *c, const double *cd, const double *g_n2, float *
     or_fft_c2r_3d(N, f[a]);
     ghost_copy(N, f[a]);
     double s = 0.0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
     for (int n = 0; n < Np; n++)
       for (int m = 0; m < Np; m++)
         for (int p = 0; p < Np; p++) {
@@ -333,6 +336,7 @@ void or_lpt_field(const or_cfg *c, const double *cd, const double *g_n2, float *
           s += tmp[q];
         }
     s /= (double)np;
+#pragma omp parallel for schedule(static)
     for (size_t q = 0; q < np; q++) out[a][q] = (float)(tmp[q] - s);
     free(f[a]);
   }

@@ -20,6 +20,9 @@ static inline size_t cidx(int n, int ix, int iy, int iz) { return ((size_t)ix * 
  * PtoMesh: CIC deposit, density initialised to -1, ghost slab added to
  * slab 0 with +1.  src/auxPM.c:280-356.
  * ---------------------------------------------------------------------- */
+/* threads the oracle's OpenMP loops use */
+int or_num_threads(void) { return omp_get_max_threads(); }
+
 /* one particle's eight CIC updates, in the reference's order */
 static inline void cic_one(int N, double scaleBox, double WPAR, double *density, float px, float py, float pz) {
   double X = px * scaleBox, Y = py * scaleBox, Z = pz * scaleBox;
@@ -128,52 +131,68 @@ void or_pofk(const or_cfg *c, const double *dk, int nbins, int bintype, double k
   const int N = c->nmesh;
   for (int i = 0; i < nbins; i++) pofk[i] = kmean[i] = nmodes[i] = 0.0;
   const double norm = 1.0 / pow((double)N, 6);
-  for (int i = 0; i < N; i++) {
-    int d0 = i > N / 2 ? N - i : i;
-    double gx = d0 == 0 ? 1.0 : sin((OR_PI * d0) / (double)N) / ((OR_PI * d0) / (double)N);
-    for (int j = 0; j < N / 2 + 1; j++) {
-      int d1 = j;
-      double gy = d1 == 0 ? 1.0 : sin((OR_PI * d1) / (double)N) / ((OR_PI * d1) / (double)N);
-      for (int pass = 0; pass < 2; pass++) {
-        int d2 = pass == 0 ? 0 : N / 2;
-        double gz = pass == 0 ? 1.0 : 2.0 / OR_PI;
-        double kmag = sqrt((double)(d0 * d0 + d1 * d1 + d2 * d2));
-        int nk = bin_index(kmag, kmin, kmax, nbins, bintype);
-        if (nk >= 0 && nk < nbins) {
-          double gc = 1.0 / pow(gx * gy * gz, 4.0) * norm;
-          const double *v = dk + 2 * cidx(N, i, j, d2);
-          pofk[nk] += (v[0] * v[0] + v[1] * v[1]) * gc;
-          kmean[nk] += kmag;
-          nmodes[nk] += 1.0;
-          if (j != N / 2 && j != 0) {
-            v = dk + 2 * cidx(N, i, N - j, d2);
-            pofk[nk] += (v[0] * v[0] + v[1] * v[1]) * gc;
-            kmean[nk] += kmag;
-            nmodes[nk] += 1.0;
+  /* each thread bins the x planes it owns into its own sums; the sums are added in thread order */
+  const int nth = omp_get_max_threads();
+  double *acc = (double *)calloc((size_t)nth * 3 * nbins, sizeof(double));
+#pragma omp parallel
+  {
+    double *pofk_t = acc + (size_t)omp_get_thread_num() * 3 * nbins, *kmean_t = pofk_t + nbins,
+           *nmodes_t = kmean_t + nbins;
+#pragma omp for schedule(static)
+    for (int i = 0; i < N; i++) {
+      int d0 = i > N / 2 ? N - i : i;
+      double gx = d0 == 0 ? 1.0 : sin((OR_PI * d0) / (double)N) / ((OR_PI * d0) / (double)N);
+      for (int j = 0; j < N / 2 + 1; j++) {
+        int d1 = j;
+        double gy = d1 == 0 ? 1.0 : sin((OR_PI * d1) / (double)N) / ((OR_PI * d1) / (double)N);
+        for (int pass = 0; pass < 2; pass++) {
+          int d2 = pass == 0 ? 0 : N / 2;
+          double gz = pass == 0 ? 1.0 : 2.0 / OR_PI;
+          double kmag = sqrt((double)(d0 * d0 + d1 * d1 + d2 * d2));
+          int nk = bin_index(kmag, kmin, kmax, nbins, bintype);
+          if (nk >= 0 && nk < nbins) {
+            double gc = 1.0 / pow(gx * gy * gz, 4.0) * norm;
+            const double *v = dk + 2 * cidx(N, i, j, d2);
+            pofk_t[nk] += (v[0] * v[0] + v[1] * v[1]) * gc;
+            kmean_t[nk] += kmag;
+            nmodes_t[nk] += 1.0;
+            if (j != N / 2 && j != 0) {
+              v = dk + 2 * cidx(N, i, N - j, d2);
+              pofk_t[nk] += (v[0] * v[0] + v[1] * v[1]) * gc;
+              kmean_t[nk] += kmag;
+              nmodes_t[nk] += 1.0;
+            }
           }
         }
-      }
-      for (int k = 1; k < N / 2; k++) {
-        double gz = sin((OR_PI * k) / (double)N) / ((OR_PI * k) / (double)N);
-        double kmag = sqrt((double)(d0 * d0 + d1 * d1 + k * k));
-        int nk = bin_index(kmag, kmin, kmax, nbins, bintype);
-        if (nk >= 0 && nk < nbins) {
-          double gc = 1.0 / pow(gx * gy * gz, 4.0) * norm;
-          const double *v = dk + 2 * cidx(N, i, j, k);
-          const double w = OR_PLANT == 3 ? 1.0 : 2.0;
-          pofk[nk] += w * (v[0] * v[0] + v[1] * v[1]) * gc;
-          kmean[nk] += w * kmag;
-          nmodes[nk] += w;
-          if (j != N / 2 && j != 0) {
-            v = dk + 2 * cidx(N, i, N - j, k);
-            pofk[nk] += w * (v[0] * v[0] + v[1] * v[1]) * gc;
-            kmean[nk] += w * kmag;
-            nmodes[nk] += w;
+        for (int k = 1; k < N / 2; k++) {
+          double gz = sin((OR_PI * k) / (double)N) / ((OR_PI * k) / (double)N);
+          double kmag = sqrt((double)(d0 * d0 + d1 * d1 + k * k));
+          int nk = bin_index(kmag, kmin, kmax, nbins, bintype);
+          if (nk >= 0 && nk < nbins) {
+            double gc = 1.0 / pow(gx * gy * gz, 4.0) * norm;
+            const double *v = dk + 2 * cidx(N, i, j, k);
+            const double w = OR_PLANT == 3 ? 1.0 : 2.0;
+            pofk_t[nk] += w * (v[0] * v[0] + v[1] * v[1]) * gc;
+            kmean_t[nk] += w * kmag;
+            nmodes_t[nk] += w;
+            if (j != N / 2 && j != 0) {
+              v = dk + 2 * cidx(N, i, N - j, k);
+              pofk_t[nk] += w * (v[0] * v[0] + v[1] * v[1]) * gc;
+              kmean_t[nk] += w * kmag;
+              nmodes_t[nk] += w;
+            }
           }
         }
       }
     }
   }
+  for (int t = 0; t < nth; t++)
+    for (int b = 0; b < nbins; b++) {
+      pofk[b] += acc[(size_t)t * 3 * nbins + b];
+      kmean[b] += acc[(size_t)t * 3 * nbins + nbins + b];
+      nmodes[b] += acc[(size_t)t * 3 * nbins + 2 * nbins + b];
+    }
+  free(acc);
   for (int i = 0; i < nbins; i++) {
     if (nmodes[i] > 0) {
       pofk[i] = (pofk[i] / nmodes[i]) * pow(c->box, 3);

@@ -94,3 +94,43 @@ def test_no_device_fails_loudly():
     assert rc == -5  # MGPU_E_NODEVICE
     with pytest.raises(M.MGPUError):
         M.MGPU(32, 16, 100.0, 0.3)
+
+
+def hp(n):
+    """Complex entries per mesh row: n/2 + 1 rounded up to 16."""
+    return (n // 2 + 1 + 15) // 16 * 16
+
+
+@pytest.mark.parametrize("n,ns,sd", [(512, 256, 0), (1024, 1024, 1), (2048, 1024, 1), (384, 192, 1), (768, 512, 1)])
+def test_memory_plan_by_part(n, ns, sd):
+    """mgpu_memory_plan needs no device; its parts follow the buffers mgpu_create allocates."""
+    parts, total = M.memory_plan(n, ns, scale_dependent=sd)
+    np_ = ns ** 3
+    sb = n * n * hp(n) * 8
+    assert parts["particles"] == 60 * np_
+    assert parts["meshes"] == 4 * sb
+    assert parts["stored"] == (2 * sb if sd else 0)
+    lat = 6 * ns * ns * hp(ns) * 8 if (sd and n % ns == 0) else 0
+    assert parts["lattice"] == lat
+    # tile bins plus 2 u32 and one float4 record per particle
+    assert 24 * np_ <= parts["deposit"] <= 24 * np_ + 72 * (n // 8) ** 3 + (1 << 20)
+    # per-block partial sums (3 doubles per 256 particles) plus the per-|n|^2 tables (26 bytes an entry)
+    assert 0 < parts["tables"] < 3 * 8 * np_ // 256 + 26 * (3 * (n // 2) ** 2 + 1) + (16 << 20)
+    assert parts["rsd"] == 16 * np_
+    assert total == sum(v for k, v in parts.items() if k != "rsd")
+
+
+def test_memory_plan_of_the_1024_learned_run_fits_one_gpu():
+    """1024^3 particles on a 1024^3 mesh, scale-dependent growth, RSD: within 288 GB of HBM."""
+    parts, total = M.memory_plan(1024, 1024, scale_dependent=1)
+    assert total + parts["rsd"] < 288e9 * 0.9
+
+
+@pytest.mark.parametrize("bad", [dict(nmesh=1000), dict(nsample=0), dict(nsample=4096)])
+def test_memory_plan_rejects_bad_sizes(bad):
+    L = M.lib()
+    args = dict(nmesh=64, nsample=32)
+    args.update(bad)
+    p = M.make_params(args["nmesh"], args["nsample"], 100.0, 0.3)
+    L.mgpu_memory_plan.argtypes = [C.POINTER(M.Params), C.c_void_p, C.c_void_p]
+    assert L.mgpu_memory_plan(C.byref(p), None, None) == -1

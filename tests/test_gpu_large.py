@@ -121,10 +121,11 @@ def test_get_displacements_matches_oracle_large(tmp_path, case):
     g.set_particles(pos)
     s_first, (pk_first, _, n_first) = g.get_displacements(mg=m, pofk=True)  # unpaired: records mode counts
     s_gpu, (pk_gpu, k_gpu, n_gpu) = g.get_displacements(mg=m, pofk=True)  # paired x passes
-    dx_gpu = g.get_mesh(M.MESH_DENSITY_X)
     dk_gpu = g.get_mesh(M.MESH_DENSITY_K)
     f_gpu = [g.get_mesh(q) for q in (M.MESH_FORCE_X, M.MESH_FORCE_Y, M.MESH_FORCE_Z)]
     disp_gpu = g.get_disp()
+    g.pm_deposit()  # delta(x) again: the step's force pass reuses its slot
+    dx_gpu = g.get_mesh(M.MESH_DENSITY_X)
     g.close()
     print("GPU done", flush=True)
 

@@ -40,6 +40,24 @@ def test_device_visible():
     assert info["cu_count"] > 0
 
 
+def test_create_over_the_device_memory_fails_cleanly():
+    """2048^3 particles (515 GB of particle arrays alone) are refused before any allocation, with the plan
+    by part in the message; a context of normal size still works afterwards."""
+    parts, total = M.memory_plan(2048, 2048, scale_dependent=1)
+    assert total > M.device_info()["global_mem"]
+    with pytest.raises(M.MGPUError) as e:
+        M.MGPU(2048, 2048, 4096.0, 0.3, scale_dependent=1)
+    msg = str(e.value)
+    print(msg, flush=True)
+    assert "(-2)" in msg or "-2" in msg
+    assert "particles" in msg and "GB are free" in msg
+    g = M.MGPU(32, 16, 64.0, 0.3)
+    g.set_particles(lattice_particles(16, 64.0, amp=1.0, seed=1))
+    g.pm_deposit()
+    assert np.isfinite(g.get_mesh(M.MESH_DENSITY_X)).all()
+    g.close()
+
+
 @pytest.mark.parametrize("n,ns", [(32, 16), (24, 24), (48, 16)])
 def test_deposit_matches_oracle(n, ns):
     p = lattice_particles(ns, BOX, amp=1.2, seed=5)
