@@ -195,6 +195,17 @@ int mgpu_kick_drift(mgpu_ctx *ctx, double dda, double A, double s1k, double s2k,
  * src/2LPT.c:185-495, src/2LPT.c:1204-1520 */
 int mgpu_ic_displacement_fields(mgpu_ctx *ctx, unsigned long seed, int sphere_mode, int amplitude_fixed,
                                 int inverted, const double *pk_n2);
+/* mgpu_ic_displacement_fields without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * The per-call seed table (Nmesh^2 unsigned) is staged in a 16 MiB scratch array
+ * that each GPU holds until the process ends (at most Nmesh = 2048; larger
+ * tables give MGPU_E_INVALID).  Every call on one GPU shares that GPU's
+ * scratch: calls that overlap on one GPU (contexts on several host threads)
+ * race and must be serialised by the caller.  Results are identical to
+ * mgpu_ic_displacement_fields. */
+int mgpu_ic_displacement_fields_noalloc(mgpu_ctx *ctx, unsigned long seed, int sphere_mode, int amplitude_fixed,
+                                        int inverted, const double *pk_n2);
 /* Particle init: Vel = 0 (COLA) and Pos = q + D*Di + D2*Di2 (scale-dependent:
  * q + slot0), periodic wrap.  src/main.c:226-309 */
 int mgpu_init_particles(mgpu_ctx *ctx, double Di, double Di2);

@@ -220,6 +220,7 @@ int mgpu_create(const mgpu_params *p, mgpu_ctx **out) {
     return code;
   };
   if (hipGetDevice(&ctx->device) != hipSuccess) return fail(mgpu_fail(ctx, MGPU_E_HIP, "hipGetDevice failed"));
+  if ((rc = ic_scratch_init(ctx))) return fail(rc);
   hipDeviceProp_t pr;
   if (hipGetDeviceProperties(&pr, ctx->device) == hipSuccess) ctx->lds_max = pr.sharedMemPerBlock;
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
@@ -627,6 +628,16 @@ int mgpu_ic_displacement_fields(mgpu_ctx *ctx, unsigned long seed, int sphere_mo
                                 const double *pk_n2) {
   if (check_ctx(ctx) || !pk_n2) return MGPU_E_INVALID;
   return ic_displacement_fields(ctx, seed, sphere_mode, amplitude_fixed, inverted, pk_n2);
+}
+
+int mgpu_ic_displacement_fields_noalloc(mgpu_ctx *ctx, unsigned long seed, int sphere_mode, int amplitude_fixed,
+                                        int inverted, const double *pk_n2) {
+  if (check_ctx(ctx) || !pk_n2) return MGPU_E_INVALID;
+  if ((size_t)ctx->N * ctx->N > (size_t)MGPU_IC_SCRATCH_SEEDS)
+    return mgpu_fail(ctx, MGPU_E_INVALID, "Nmesh^2 = " + std::to_string((size_t)ctx->N * ctx->N) +
+                                              " seeds exceed the scratch of " + std::to_string(MGPU_IC_SCRATCH_SEEDS));
+  if (!ctx->ic_scratch) return mgpu_fail(ctx, MGPU_E_STATE, "seed-table scratch not set up");
+  return ic_displacement_fields(ctx, seed, sphere_mode, amplitude_fixed, inverted, pk_n2, ctx->ic_scratch);
 }
 
 int mgpu_init_particles(mgpu_ctx *ctx, double Di, double Di2) {

@@ -14,11 +14,22 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
+
+// Seed table staging of the allocation-free IC entry point: one copy per GPU,
+// held until the process ends.  Host code never passes `scratch` itself to a
+// kernel or a copy (the name is only a host-side stand-in for the symbol); it
+// uses the per-device address that ic_scratch_init obtains with
+// hipGetSymbolAddress.
+static __device__ __align__(256) unsigned scratch[MGPU_IC_SCRATCH_SEEDS];
 
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned *g_scratch_ptr[kMaxDevices] = {};
 
 // one thread per column (i, j) of the N x N grid
 __global__ __launch_bounds__(kBlock) void k_ic_delta(const unsigned *__restrict__ seedtable, int N, int Np, int HP,
@@ -578,15 +589,37 @@ int upload_n2(mgpu_ctx *ctx, int which, const double *h) {
 
 }  // namespace
 
+// The device address of this GPU's seed-table scratch, looked up once per
+// device (with that device current) and kept for the process.
+int ic_scratch_init(mgpu_ctx *ctx) {
+  if (ctx->device < 0 || ctx->device >= kMaxDevices)
+    return mgpu_fail(ctx, MGPU_E_INVALID, "device index beyond the seed-table scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[ctx->device]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return mgpu_fail(ctx, MGPU_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
+    g_scratch_ptr[ctx->device] = (unsigned *)p;
+  }
+  ctx->ic_scratch = g_scratch_ptr[ctx->device];
+  return MGPU_OK;
+}
+
+// displacement_fields.  seed_dev == nullptr: the seed table goes to a buffer
+// allocated for the call; otherwise to seed_dev (the per-device scratch), and
+// nothing is allocated on the device.
 int ic_displacement_fields(mgpu_ctx *ctx, unsigned long seed, int sphere_mode, int amp_fixed, int inverted,
-                           const double *pk_n2) {
+                           const double *pk_n2, unsigned *seed_dev) {
   PhaseTimer pt(ctx, &ctx->t.ic);
   const int N = ctx->N;
   int rc;
   std::vector<unsigned> st;
   seedtable_host(N, seed, st);
-  unsigned *dseed = nullptr;
-  MGPU_HIP(ctx, hipMalloc(&dseed, st.size() * sizeof(unsigned)));
+  unsigned *dseed = seed_dev;
+  if (!dseed) MGPU_HIP(ctx, hipMalloc(&dseed, st.size() * sizeof(unsigned)));
   MGPU_HIP(ctx, hipMemcpyAsync(dseed, st.data(), st.size() * sizeof(unsigned), hipMemcpyHostToDevice, ctx->stream));
   if ((rc = upload_n2(ctx, 0, pk_n2))) return rc;
   float2 *cd = ctx->p.scale_dependent ? ctx->cdelta[0] : ctx->slot[0];
@@ -595,7 +628,7 @@ int ic_displacement_fields(mgpu_ctx *ctx, unsigned long seed, int sphere_mode, i
                      ctx->HP, ctx->p.box, sphere_mode, amp_fixed, inverted, ctx->n2tab[0], cd);
   MGPU_HIP(ctx, hipGetLastError());
   MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  MGPU_HIP(ctx, hipFree(dseed));
+  if (!seed_dev) MGPU_HIP(ctx, hipFree(dseed));
 
   // displacement gradient -> 2LPT source (real), built in slot 1 over the
   // psi_00 gradient (the diagonal gradients in slots 1-3)

@@ -21,6 +21,8 @@
 #define MGPU_INVERSE_H0_MPCH 2997.92458
 // mesh slots of a context
 #define MGPU_NSLOTS 4
+// seed-table entries of the allocation-free IC scratch: the Nmesh^2 columns of Nmesh = 2048
+#define MGPU_IC_SCRATCH_SEEDS (2048 * 2048)
 // columns per tile of the strided FFT passes (16: 128-byte row segments, 8: 64-byte)
 #ifndef MGPU_FFT_TCOLS
 #define MGPU_FFT_TCOLS 16
@@ -102,6 +104,7 @@ struct mgpu_ctx {
   float4 *dep_rec = nullptr;       // particle records (x, y, z, index bits) in bin order (inside dep_scratch)
   unsigned *dep_off = nullptr;     // bin offsets of dep_rec, 8 bins (face classes) per tile
   bool sorted_valid = false;       // dep_rec holds the current positions
+  unsigned *ic_scratch = nullptr;  // this device's seed-table scratch (allocation-free IC)
 
   // state
   bool have_deposit = false, have_density = false, have_forces = false, have_disp = false;
@@ -235,8 +238,11 @@ int pm_kick_drift(mgpu_ctx *ctx, double dda, double A, double s1k, double s2k, c
 int pm_reduce3(mgpu_ctx *ctx, int nblocks, double out[3]);
 
 // ---------------------------------------------------------------- IC / LPT (ic.hip)
+// seed_dev: nullptr allocates the seed table for the call; else the per-device scratch
 int ic_displacement_fields(mgpu_ctx *ctx, unsigned long seed, int sphere_mode, int amp_fixed, int inverted,
-                           const double *pk_n2);
+                           const double *pk_n2, unsigned *seed_dev = nullptr);
+// looks up this device's seed-table scratch (ctx->ic_scratch); called by mgpu_create
+int ic_scratch_init(mgpu_ctx *ctx);
 int ic_init_particles(mgpu_ctx *ctx, double Di, double Di2);
 int ic_lpt_fields(mgpu_ctx *ctx, int slot, const double *g1_n2, const double *g2_n2);
 int ic_lpt_fields_slices(mgpu_ctx *ctx, int slot, const mgpu_kslice *o1, const mgpu_kslice *o2);

@@ -29,7 +29,7 @@ MGPU_SYMBOLS = [
     "mgpu_get_displacements", "mgpu_kick", "mgpu_drift", "mgpu_ic_displacement_fields", "mgpu_init_particles",
     "mgpu_lpt_fields", "mgpu_get_mesh", "mgpu_set_mesh", "mgpu_get_timers", "mgpu_enable_timers",
     "mgpu_kernel_name", "mgpu_enable_kernel_timing", "mgpu_kernel_timing", "mgpu_lpt_fields_slices",
-    "mgpu_lpt_fields_slices2", "mgpu_memory_plan",
+    "mgpu_lpt_fields_slices2", "mgpu_ic_displacement_fields_noalloc", "mgpu_memory_plan",
 ]
 
 
