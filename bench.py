@@ -43,7 +43,7 @@ HBM_PEAK_GBS = 8000.0
 # kernel-timing class -> the kernel (rocprofv3 name) it times, for the f(R)
 # scale-dependent workload's mesh size N and lattice size Ns
 KERNEL_SYMBOL = {
-    "deposit": "k_dep_tile", "mean": "k_sum3", "dep_sort": "k_dep_bin + k_dep_scatter",
+    "deposit": "k_dep_tiles<false, 16>", "mean": "k_sum3", "dep_sort": "k_dep_bin + k_dep_scatter",
     "fft_r2c": "k_fft_z_r2c<{N}>", "fft_c2r": "k_fft_z_c2r<{N}, *>", "fft_col_fwd": "k_fft_col<{N}, 16, -1, 2>",
     "fft_col_inv": "k_fft_col<{N}, 16, 1, 2>", "fft_green": "k_col_green<{N}, 16, 0>",
     "fft_pot": "k_col_green<{N}, 16, 1>", "fft_dual": "k_col_dual<{N}, 16>", "fft_screen": "k_z_screen<{N}, 0>",
@@ -211,6 +211,96 @@ def learned_leg(np_eval=512, box_eval=1024.0, nsteps=30, np_train=128, box_train
             "train": f"{len(pairs)} pairs at {np_train}^3/Box {box_train:g} (coarse {np_train} vs {3 * np_train}), "
                      f"{iterations} fixed-point iterations (best kept), {t_train:.0f} s",
             "train_history": [[h[0], round(h[2], 4)] for h in hist]}
+
+
+def fft_leg(n=1024, reps=3):
+    """Our 3D r2c of an n^3 real mesh (z r2c pass, y pass, x pass:
+    mgpu_pm_transform without P(k); device time of its kernels by HIP events)
+    against rocFFT's out-of-place single-precision r2c plan of the same mesh
+    (librocfft of the same ROCm, called through its C API; HIP events around
+    `reps` executions after one untimed one), and the largest difference of the
+    two spectra (rocFFT's from its first execution) relative to the largest
+    mode."""
+    import ctypes as C
+
+    import mgpicola as M
+
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal((n, n, n), dtype=np.float32)
+    g = M.MGPU(n, 8, float(n), 0.3)
+    ours = []
+    for r in range(reps + 1):
+        g.set_mesh(M.MESH_DENSITY_X, x)
+        M.enable_kernel_timing(g, True)
+        M.kernel_timing(g)
+        g.pm_transform()
+        g.sync()
+        kt = M.kernel_timing(g)
+        M.enable_kernel_timing(g, False)
+        if r:
+            ours.append(sum(v[1] for k, v in kt.items() if k.startswith("fft")))
+    got = g.get_mesh(M.MESH_DENSITY_K)
+    g.close()
+
+    hip = C.CDLL("libamdhip64.so")
+    fft = C.CDLL("librocfft.so")
+    vp = C.c_void_p
+    nin, nout = 4 * n ** 3, 8 * n * n * (n // 2 + 1)
+    din, dout, work = vp(), vp(), vp()
+    plan, info = vp(), vp()
+
+    def ck(rc, what):
+        if rc:
+            raise RuntimeError(f"{what} returned {rc}")
+
+    ck(hip.hipMalloc(C.byref(din), C.c_size_t(nin)), "hipMalloc")
+    ck(hip.hipMalloc(C.byref(dout), C.c_size_t(nout)), "hipMalloc")
+    try:
+        ck(hip.hipMemcpy(din, x.ctypes.data_as(vp), C.c_size_t(nin), 1), "hipMemcpy")  # host to device
+        ck(fft.rocfft_setup(), "rocfft_setup")
+        lengths = (C.c_size_t * 3)(n, n, n)
+        # not in place, real forward, single precision, 3 dimensions, one transform, default layout
+        ck(fft.rocfft_plan_create(C.byref(plan), 1, 2, 0, C.c_size_t(3), lengths, C.c_size_t(1), vp()),
+           "rocfft_plan_create")
+        wsz = C.c_size_t()
+        ck(fft.rocfft_plan_get_work_buffer_size(plan, C.byref(wsz)), "rocfft_plan_get_work_buffer_size")
+        ck(fft.rocfft_execution_info_create(C.byref(info)), "rocfft_execution_info_create")
+        if wsz.value:
+            ck(hip.hipMalloc(C.byref(work), wsz), "hipMalloc")
+            ck(fft.rocfft_execution_info_set_work_buffer(info, work, wsz), "rocfft_execution_info_set_work_buffer")
+        ins, outs = (vp * 1)(din), (vp * 1)(dout)
+        ck(fft.rocfft_execute(plan, ins, outs, info), "rocfft_execute")
+        ck(hip.hipDeviceSynchronize(), "hipDeviceSynchronize")
+        # the spectrum of the first execution (a transform may use its input as scratch)
+        ref = np.empty((n, n, n // 2 + 1), np.complex64)
+        ck(hip.hipMemcpy(ref.ctypes.data_as(vp), dout, C.c_size_t(nout), 2), "hipMemcpy")  # device to host
+        e0, e1 = vp(), vp()
+        hip.hipEventCreate(C.byref(e0))
+        hip.hipEventCreate(C.byref(e1))
+        hip.hipEventRecord(e0, vp())
+        for _ in range(reps):
+            ck(fft.rocfft_execute(plan, ins, outs, info), "rocfft_execute")
+        hip.hipEventRecord(e1, vp())
+        ck(hip.hipEventSynchronize(e1), "hipEventSynchronize")
+        ms = C.c_float()
+        hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+        roc_ms = ms.value / reps
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+    finally:
+        if info:
+            fft.rocfft_execution_info_destroy(info)
+        if plan:
+            fft.rocfft_plan_destroy(plan)
+        for b in (din, dout, work):
+            if b:
+                hip.hipFree(b)
+    err = float(np.abs(got - ref).max() / np.abs(ref).max())
+    ours_ms = float(np.mean(ours))
+    return {"n": n, "ours_ms": round(ours_ms, 3), "rocfft_ms": round(roc_ms, 3),
+            "rocfft_over_ours": round(roc_ms / ours_ms, 3), "max_rel_diff": float(f"{err:.2e}"),
+            "what": f"3D r2c of a {n}^3 FP32 mesh: libmgpu in place (padded rows) vs a rocFFT out-of-place plan, "
+                    "device time per transform"}
 
 
 def ensemble_leg(per_size=4, threads=12, nsteps=20):
@@ -391,6 +481,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-multiplier P(k) error leg")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the concurrent-ensemble leg")
+    ap.add_argument("--no-fft-compare", action="store_true", help="skip the rocFFT comparison")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy this many MiB device to device (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -428,6 +519,12 @@ def main():
             ens = ensemble_leg()
         except Exception as e:  # the throughput line still prints; the failure is reported in it
             ens = {"error": f"{type(e).__name__}: {e}"}
+    fft = None
+    if not args.no_fft_compare:
+        try:
+            fft = fft_leg()
+        except Exception as e:  # the throughput line still prints; the failure is reported in it
+            fft = {"error": f"{type(e).__name__}: {e}"}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline()
@@ -438,7 +535,8 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": args.workload, "particles": r["npart"], "nmesh": r["nmesh"], "box": r["box"],
                    "model": "f(R) Hu-Sawicki fR0=1e-5 n=1, scale-dependent, screening, P(k) every step"},
-        "roofline": roof, "cpu_baseline": cpu, "learned": lrn, "ensemble": ens, "kernels_per_step": per_kernel,
+        "roofline": roof, "cpu_baseline": cpu, "learned": lrn, "ensemble": ens, "fft_vs_rocfft": fft,
+        "kernels_per_step": per_kernel,
     }
     print(json.dumps(line), flush=True)
 

@@ -53,11 +53,6 @@ __device__ __forceinline__ unsigned tile_of(const TileGeom &g, unsigned IX, unsi
 
 __device__ __forceinline__ float to_delta(unsigned long long v) { return (float)((double)v * kInvFix - 1.0); }
 
-// 1. bin key of every particle, 8 tile + face class (class bits: x, y, z local
-// coordinate = T - 1), and its rank inside the bin.  The lanes of a wave with
-// equal keys are grouped first (ballots only); then the group leaders add
-// their group sizes in one atomic instruction, so a wave waits for one memory
-// round trip however many bins it touches.
 // Mesh coordinates of a particle.  Plain: from the SoA positions.  Redshift
 // space (RSD): from a record (x, y in box units, Z in mesh units as FP64 in
 // the last two slots), written by k_rsd_coords.
@@ -79,46 +74,74 @@ __device__ __forceinline__ void particle_xyz(const float *__restrict__ pos, cons
   }
 }
 
+// Bin key of a particle: 8 tile + face class.
+template <bool RSD>
+__device__ __forceinline__ unsigned bin_key(const float *__restrict__ pos, const float4 *__restrict__ crd, size_t np,
+                                            size_t i, const TileGeom &g, double scale) {
+  unsigned IX, IY, IZ;
+  double D, X, Y, Z;
+  particle_xyz<RSD>(pos, crd, np, i, scale, X, Y, Z);
+  cell_of(X, g.N, IX, D);
+  cell_of(Y, g.N, IY, D);
+  cell_of(Z, g.N, IZ, D);
+  const unsigned m = (unsigned)g.tx - 1;
+  const unsigned cls = ((IX & m) == m ? 4u : 0u) | ((IY & m) == m ? 2u : 0u) | ((IZ & m) == m ? 1u : 0u);
+  return 8u * tile_of(g, IX, IY, IZ) + cls;
+}
+
+// 1. bin key of every particle, 8 tile + face class (class bits: x, y, z local
+// coordinate = T - 1), and its rank inside the bin.  The lanes of a wave with
+// equal keys are grouped first (ballots only); then the group leaders add
+// their group sizes in one atomic instruction.  Each thread takes kBinU
+// particles (block chunks of kBinU kTB, coalesced per u) whose groups are all
+// formed before their atomics are issued back to back, so a wave waits for one
+// atomic round trip per kBinU particles.  Only the rank is stored: the scatter
+// forms the key again from the position.
+constexpr int kBinU = 4;
+
 template <bool RSD>
 __global__ __launch_bounds__(kTB) void k_dep_bin(const float *__restrict__ pos, const float4 *__restrict__ crd,
                                                  size_t np, TileGeom g, double scale, unsigned *__restrict__ count,
-                                                 unsigned *__restrict__ key, unsigned *__restrict__ rank) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = i < np;
-  unsigned t = 0;
-  if (valid) {
-    unsigned IX, IY, IZ;
-    double D, X, Y, Z;
-    particle_xyz<RSD>(pos, crd, np, i, scale, X, Y, Z);
-    cell_of(X, g.N, IX, D);
-    cell_of(Y, g.N, IY, D);
-    cell_of(Z, g.N, IZ, D);
-    const unsigned m = (unsigned)g.tx - 1;
-    const unsigned cls = ((IX & m) == m ? 4u : 0u) | ((IY & m) == m ? 2u : 0u) | ((IZ & m) == m ? 1u : 0u);
-    t = 8u * tile_of(g, IX, IY, IZ) + cls;
-  }
+                                                 unsigned *__restrict__ rank) {
+  const size_t i0 = (size_t)blockIdx.x * (kBinU * kTB) + threadIdx.x;
   const int lane = threadIdx.x % warpSize;
-  bool pending = valid;
-  int leader_of = lane;
-  unsigned off_in = 0, gsize = 0;
-  for (unsigned long long rem = __ballot(pending); rem;) {
-    const int leader = __ffsll((long long)rem) - 1;
-    const unsigned tl = __shfl(t, leader);
-    const unsigned long long same = __ballot(pending && t == tl);
-    if (pending && t == tl) {
-      leader_of = leader;
-      off_in = (unsigned)__popcll(same & ((1ull << lane) - 1ull));
-      pending = false;
-    }
-    if (lane == leader) gsize = (unsigned)__popcll(same);
-    rem &= ~same;
+  unsigned t[kBinU], off_in[kBinU], gsize[kBinU], base[kBinU];
+  int leader_of[kBinU];
+#pragma unroll
+  for (int u = 0; u < kBinU; u++) {
+    const size_t i = i0 + (size_t)u * kTB;
+    t[u] = i < np ? bin_key<RSD>(pos, crd, np, i, g, scale) : 0u;
   }
-  unsigned base = 0;
-  if (valid && lane == leader_of) base = atomicAdd(&count[t], gsize);
-  base = __shfl(base, leader_of);
-  if (valid) {
-    key[i] = t;
-    rank[i] = base + off_in;
+#pragma unroll
+  for (int u = 0; u < kBinU; u++) {
+    const bool valid = i0 + (size_t)u * kTB < np;
+    bool pending = valid;
+    leader_of[u] = lane;
+    off_in[u] = 0;
+    gsize[u] = 0;
+    for (unsigned long long rem = __ballot(pending); rem;) {
+      const int leader = __ffsll((long long)rem) - 1;
+      const unsigned tl = __shfl(t[u], leader);
+      const unsigned long long same = __ballot(pending && t[u] == tl);
+      if (pending && t[u] == tl) {
+        leader_of[u] = leader;
+        off_in[u] = (unsigned)__popcll(same & ((1ull << lane) - 1ull));
+        pending = false;
+      }
+      if (lane == leader) gsize[u] = (unsigned)__popcll(same);
+      rem &= ~same;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < kBinU; u++) {
+    base[u] = 0;
+    if (i0 + (size_t)u * kTB < np && lane == leader_of[u]) base[u] = atomicAdd(&count[t[u]], gsize[u]);
+  }
+#pragma unroll
+  for (int u = 0; u < kBinU; u++) {
+    const size_t i = i0 + (size_t)u * kTB;
+    const unsigned b = __shfl(base[u], leader_of[u]);
+    if (i < np) rank[i] = b + off_in[u];
   }
 }
 
@@ -168,49 +191,62 @@ __global__ __launch_bounds__(1024) void k_scan_add(unsigned *__restrict__ out, s
   if (i == 0) out[ntail] = total_tail;
 }
 
-// 3. particle records in bin order: (x, y, z, index as bits), or the RSD record
+// 3. particle records in bin order: (x, y, z, index as bits), or the RSD
+// record; the bin key is formed again from the position (the same function as
+// the binning, so the same key)
 template <bool RSD>
-__global__ __launch_bounds__(kTB) void k_dep_scatter(const unsigned *__restrict__ key,
-                                                     const unsigned *__restrict__ rank,
+__global__ __launch_bounds__(kTB) void k_dep_scatter(const unsigned *__restrict__ rank,
                                                      const unsigned *__restrict__ off, const float *__restrict__ pos,
-                                                     const float4 *__restrict__ crd, size_t np,
-                                                     float4 *__restrict__ rec) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
-  rec[off[key[i]] + rank[i]] =
-      RSD ? crd[i] : make_float4(pos[i], pos[np + i], pos[2 * np + i], __uint_as_float((unsigned)i));
+                                                     const float4 *__restrict__ crd, size_t np, TileGeom g,
+                                                     double scale, float4 *__restrict__ rec) {
+  const size_t i0 = (size_t)blockIdx.x * (kBinU * kTB) + threadIdx.x;
+  unsigned o[kBinU];
+#pragma unroll
+  for (int u = 0; u < kBinU; u++) {
+    const size_t i = i0 + (size_t)u * kTB;
+    o[u] = i < np ? off[bin_key<RSD>(pos, crd, np, i, g, scale)] + rank[i] : 0u;
+  }
+#pragma unroll
+  for (int u = 0; u < kBinU; u++) {
+    const size_t i = i0 + (size_t)u * kTB;
+    if (i < np)
+      rec[o[u]] = RSD ? crd[i] : make_float4(pos[i], pos[np + i], pos[2 * np + i], __uint_as_float((unsigned)i));
+  }
 }
 
-// 4. one workgroup per tile.  The particle lists it reads are up to 64 bin
-// ranges: for each lower-neighbour shift (sx, sy, sz) in {0, 1}^3 the bins of
-// the neighbour tile whose face class contains the shift's bits.  A particle
-// read through shift s has local coordinate (I mod T) - s T, so its corners at
-// local coordinate -1 (s = 1) or T (s = 0) fall outside the tile and are
-// skipped; every corner is added by exactly one tile.
-template <bool RSD>
-__global__ __launch_bounds__(kDepTileNT) void k_dep_tile(const float4 *__restrict__ rec, const unsigned *__restrict__ off,
-                                                  TileGeom g, double scale, double wpar, int HP2,
-                                                  float *__restrict__ d) {
-  extern __shared__ unsigned long long blk[];
-  __shared__ unsigned rstart[64], rpre[65];
-  const int T = g.tx, T3 = T * T * T;
-  const unsigned t = blockIdx.x;
+// 4. one workgroup per tile at a time, by a persistent grid: each workgroup
+// walks its share of the tiles.  The particle lists of a tile are up to 64
+// bin ranges: for each lower-neighbour shift (sx, sy, sz) in {0, 1}^3 the bins
+// of the neighbour tile whose face class contains the shift's bits.  A
+// particle read through shift s has local coordinate (I mod T) - s T, so its
+// corners at local coordinate -1 (s = 1) or T (s = 0) fall outside the tile
+// and are skipped; every corner is added by exactly one tile.  The next
+// tile's first records and the bin offsets of the one after are loaded while
+// the current tile's cells are read out and stored (a barrier waits for LDS
+// operations only, not for loads or stores in flight).  Tiles of one XCD are
+// consecutive, so the two z neighbours sharing a 128-byte row segment write
+// it through one L2.
+struct RangeRegs {
+  unsigned a, e;
+};
+__device__ __forceinline__ RangeRegs tile_ranges_issue(const unsigned *__restrict__ off, const TileGeom &g, unsigned t) {
+  // lane q < 64: range q = (shift q / 8, face class q % 8) of tile t
+  RangeRegs r{0u, 0u};
   const int ti = t / (g.nty * g.ntz), tj = (t / g.ntz) % g.nty, tk = t % g.ntz;
-  for (int e = threadIdx.x; e < T3; e += blockDim.x) blk[e] = 0ull;
+  const int sh = threadIdx.x >> 3, cls = threadIdx.x & 7;
+  if (threadIdx.x < 64 && (cls & sh) == sh) {
+    const int ni = (ti - (sh >> 2) + g.ntx) % g.ntx, nj = (tj - ((sh >> 1) & 1) + g.nty) % g.nty,
+              nk = (tk - (sh & 1) + g.ntz) % g.ntz;
+    const unsigned b = 8u * (unsigned)((ni * g.nty + nj) * g.ntz + nk) + (unsigned)cls;
+    r.a = off[b];
+    r.e = off[b + 1];
+  }
+  return r;
+}
+__device__ __forceinline__ void tile_ranges_store(const RangeRegs &r, unsigned *rstart, unsigned *rpre) {
   if (threadIdx.x < 64) {
-    // range q: shift bits q / 8 (x: 4, y: 2, z: 1), face class q % 8
-    const int sh = threadIdx.x >> 3, cls = threadIdx.x & 7;
-    unsigned a = 0, n = 0;
-    if ((cls & sh) == sh) {
-      const int ni = (ti - (sh >> 2) + g.ntx) % g.ntx, nj = (tj - ((sh >> 1) & 1) + g.nty) % g.nty,
-                nk = (tk - (sh & 1) + g.ntz) % g.ntz;
-      const unsigned b = 8u * (unsigned)((ni * g.nty + nj) * g.ntz + nk) + (unsigned)cls;
-      a = off[b];
-      n = off[b + 1] - a;
-    }
-    rstart[threadIdx.x] = a;
-    // inclusive prefix sum of the lengths over the 64 lanes of the first wave
-    unsigned v = n;
+    rstart[threadIdx.x] = r.a;
+    unsigned v = r.e - r.a;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
       const unsigned u = __shfl_up(v, o);
@@ -219,19 +255,19 @@ __global__ __launch_bounds__(kDepTileNT) void k_dep_tile(const float4 *__restric
     rpre[threadIdx.x + 1] = v;
     if (threadIdx.x == 0) rpre[0] = 0;
   }
-  __syncthreads();
-  const unsigned total = rpre[64];
-  const int tm = T - 1;
-  // U virtual indices per thread and round: all record loads are in flight
-  // before the first deposit
-  constexpr int U = 4;
-  for (unsigned base = 0; base < total; base += U * blockDim.x) {
-    int sh[U];
-    float4 pr[U];
+}
+
+constexpr int kTileU = 4;  // records per thread and round
+
+// round `base` of a tile's records (ranges in rstart/rpre) into registers
+__device__ __forceinline__ void tile_records_issue(const float4 *__restrict__ rec, const unsigned *rstart,
+                                                   const unsigned *rpre, unsigned base, unsigned total,
+                                                   float4 (&pr)[kTileU], int (&sh)[kTileU]) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const unsigned v = min(base + u * blockDim.x + threadIdx.x, total - 1);
-      // range holding v: the last q with rpre[q] <= v (binary search over 64)
+  for (int u = 0; u < kTileU; ++u) {
+    const unsigned v = base + u * blockDim.x + threadIdx.x;
+    sh[u] = -1;
+    if (v < total) {
       int q = 0;
 #pragma unroll
       for (int stp = 32; stp > 0; stp >>= 1)
@@ -239,40 +275,130 @@ __global__ __launch_bounds__(kDepTileNT) void k_dep_tile(const float4 *__restric
       sh[u] = q >> 3;
       pr[u] = rec[rstart[q] + (v - rpre[q])];
     }
+  }
+}
+
+template <bool RSD>
+__device__ __forceinline__ void tile_records_add(unsigned long long *blk, const TileGeom &g, double scale,
+                                                 double wpar, const float4 (&pr)[kTileU], const int (&sh)[kTileU]) {
+  const int T = g.tx, tm = T - 1;
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (base + u * blockDim.x + threadIdx.x >= total) break;
-      unsigned IX, IY, IZ;
-      double DX, DY, DZ;
-      cell_of((double)pr[u].x * scale, g.N, IX, DX);
-      cell_of((double)pr[u].y * scale, g.N, IY, DY);
-      cell_of(RSD ? rec_z(pr[u]) : (double)pr[u].z * scale, g.N, IZ, DZ);
-      const double TX = 1.0 - DX, TZ = 1.0 - DZ;
-      const double TY = (1.0 - DY) * wpar;
-      DY *= wpar;
-      const int lx = (int)(IX & tm) - ((sh[u] >> 2) ? T : 0), ly = (int)(IY & tm) - (((sh[u] >> 1) & 1) ? T : 0),
-                lz = (int)(IZ & tm) - ((sh[u] & 1) ? T : 0);
-      // corner a of an axis is in the tile iff 0 <= l + a < T
-      const bool x0 = lx >= 0, x1 = lx + 1 < T, y0 = ly >= 0, y1 = ly + 1 < T, z0 = lz >= 0, z1 = lz + 1 < T;
-      const int b00 = (lx * T + ly) * T + lz, b01 = b00 + T, b10 = b00 + T * T, b11 = b10 + T;
+  for (int u = 0; u < kTileU; ++u) {
+    if (sh[u] < 0) continue;
+    unsigned IX, IY, IZ;
+    double DX, DY, DZ;
+    cell_of((double)pr[u].x * scale, g.N, IX, DX);
+    cell_of((double)pr[u].y * scale, g.N, IY, DY);
+    cell_of(RSD ? rec_z(pr[u]) : (double)pr[u].z * scale, g.N, IZ, DZ);
+    const double TX = 1.0 - DX, TZ = 1.0 - DZ;
+    const double TY = (1.0 - DY) * wpar;
+    DY *= wpar;
+    const int lx = (int)(IX & tm) - ((sh[u] >> 2) ? T : 0), ly = (int)(IY & tm) - (((sh[u] >> 1) & 1) ? T : 0),
+              lz = (int)(IZ & tm) - ((sh[u] & 1) ? T : 0);
+    const bool x0 = lx >= 0, x1 = lx + 1 < T, y0 = ly >= 0, y1 = ly + 1 < T, z0 = lz >= 0, z1 = lz + 1 < T;
+    const int b00 = (lx * T + ly) * T + lz, b01 = b00 + T, b10 = b00 + T * T, b11 = b10 + T;
 #define DEP(c, b, w) \
   if (c) atomicAdd(&blk[(b)], (unsigned long long)__double2ll_rn((w) * kFix))
-      DEP(x0 && y0 && z0, b00, TX * TY * TZ);
-      DEP(x0 && y0 && z1, b00 + 1, TX * TY * DZ);
-      DEP(x0 && y1 && z0, b01, TX * DY * TZ);
-      DEP(x0 && y1 && z1, b01 + 1, TX * DY * DZ);
-      DEP(x1 && y0 && z0, b10, DX * TY * TZ);
-      DEP(x1 && y0 && z1, b10 + 1, DX * TY * DZ);
-      DEP(x1 && y1 && z0, b11, DX * DY * TZ);
-      DEP(x1 && y1 && z1, b11 + 1, DX * DY * DZ);
+    DEP(x0 && y0 && z0, b00, TX * TY * TZ);
+    DEP(x0 && y0 && z1, b00 + 1, TX * TY * DZ);
+    DEP(x0 && y1 && z0, b01, TX * DY * TZ);
+    DEP(x0 && y1 && z1, b01 + 1, TX * DY * DZ);
+    DEP(x1 && y0 && z0, b10, DX * TY * TZ);
+    DEP(x1 && y0 && z1, b10 + 1, DX * TY * DZ);
+    DEP(x1 && y1 && z0, b11, DX * DY * TZ);
+    DEP(x1 && y1 && z1, b11 + 1, DX * DY * DZ);
 #undef DEP
-    }
   }
+}
+
+// k-th tile of this workgroup, or false past its share.  With nt and the grid
+// multiples of 8, XCD x (blocks b = x mod 8) owns tiles [x nt/8, (x+1) nt/8)
+// and walks them in order; else tile b + k grid.
+__device__ __forceinline__ bool tile_of_block(unsigned nt, unsigned k, unsigned &t) {
+  const unsigned G = gridDim.x, b = blockIdx.x;
+  if (nt % 8 == 0 && G % 8 == 0) {
+    const unsigned j = (b >> 3) + k * (G >> 3);
+    t = (b & 7) * (nt >> 3) + j;
+    return j < (nt >> 3);
+  }
+  t = b + k * G;
+  return t < nt;
+}
+
+template <bool RSD, int T>
+__global__ __launch_bounds__(kDepTileNT) void k_dep_tiles(const float4 *__restrict__ rec,
+                                                          const unsigned *__restrict__ off, TileGeom g, double scale,
+                                                          double wpar, int HP2, float *__restrict__ d) {
+  // The loop is rotated so that each tile's first records are loaded and
+  // consumed inside one iteration with a fixed sequence of memory operations
+  // between (the next tile's offsets, then this readout's stores): the wait
+  // before the records' use then leaves those stores in flight.
+  constexpr int T3 = T * T * T, tm = T - 1, NST = (T3 / 4 + kDepTileNT - 1) / kDepTileNT;
+  extern __shared__ unsigned long long blk[];
+  __shared__ unsigned rstart[2][64], rpre[2][65];
+  const unsigned nt = (unsigned)g.ntx * g.nty * g.ntz;
+  const size_t N = g.N;
+  unsigned t, tn = 0;
+  if (!tile_of_block(nt, 0, t)) return;  // uniform over the workgroup
+  // prologue: tile 0's ranges stored and its records added; tile 1's offsets in flight
+  tile_ranges_store(tile_ranges_issue(off, g, t), rstart[0], rpre[0]);
+  bool have_next = tile_of_block(nt, 1, tn);
+  RangeRegs rn{0u, 0u};
+  if (have_next) rn = tile_ranges_issue(off, g, tn);
+  for (int e = threadIdx.x; e < T3; e += blockDim.x) blk[e] = 0ull;
   __syncthreads();
-  const size_t N = g.N, X0 = (size_t)ti * T, Y0 = (size_t)tj * T, Z0 = (size_t)tk * T;
-  for (int e = threadIdx.x; e < T3; e += blockDim.x) {
-    const int lz = e & tm, ly = (e / T) & tm, lx = e / (T * T);
-    d[((X0 + lx) * N + Y0 + ly) * HP2 + Z0 + lz] = to_delta(blk[e]);
+  unsigned total = rpre[0][64];
+  for (unsigned base = 0; base < total; base += kTileU * blockDim.x) {
+    float4 xr[kTileU];
+    int xs[kTileU];
+    tile_records_issue(rec, rstart[0], rpre[0], base, total, xr, xs);
+    tile_records_add<RSD>(blk, g, scale, wpar, xr, xs);
+  }
+  for (unsigned k = 0;; ++k) {
+    const int nb = (k & 1) ^ 1;
+    if (have_next) tile_ranges_store(rn, rstart[nb], rpre[nb]);
+    __syncthreads();  // tile k's corners added; tile k+1's ranges visible
+    float4 pr[kTileU];
+    int sh[kTileU];
+    bool have_next2 = false;
+    unsigned t2 = 0;
+    if (have_next) {
+      total = rpre[nb][64];
+      tile_records_issue(rec, rstart[nb], rpre[nb], 0, total, pr, sh);
+      have_next2 = tile_of_block(nt, k + 2, t2);
+      if (have_next2) rn = tile_ranges_issue(off, g, t2);
+    }
+    {
+      // tile k's cells, four of a row per thread: 16-byte stores (rows start at multiples of 4 floats)
+      const int ti = t / (g.nty * g.ntz), tj = (t / g.ntz) % g.nty, tk = t % g.ntz;
+      const size_t X0 = (size_t)ti * T, Y0 = (size_t)tj * T, Z0 = (size_t)tk * T;
+#pragma unroll
+      for (int j = 0; j < NST; j++) {
+        const int e4 = j * kDepTileNT + threadIdx.x;
+        if (NST * kDepTileNT == T3 / 4 || e4 < T3 / 4) {
+          const int e = 4 * e4;
+          const int lz = e & tm, ly = (e / T) & tm, lx = e / (T * T);
+          const float4 v =
+              make_float4(to_delta(blk[e]), to_delta(blk[e + 1]), to_delta(blk[e + 2]), to_delta(blk[e + 3]));
+          *(float4 *)&d[((X0 + lx) * N + Y0 + ly) * HP2 + Z0 + lz] = v;
+        }
+      }
+    }
+    if (!have_next) break;
+    __syncthreads();  // cells read out before the clear
+    for (int e = threadIdx.x; e < T3; e += blockDim.x) blk[e] = 0ull;
+    __syncthreads();
+    const int cb = nb;
+    tile_records_add<RSD>(blk, g, scale, wpar, pr, sh);
+    for (unsigned base = kTileU * blockDim.x; base < total; base += kTileU * blockDim.x) {
+      float4 xr[kTileU];
+      int xs[kTileU];
+      tile_records_issue(rec, rstart[cb], rpre[cb], base, total, xr, xs);
+      tile_records_add<RSD>(blk, g, scale, wpar, xr, xs);
+    }
+    t = tn;
+    tn = t2;
+    have_next = have_next2;
   }
 }
 
@@ -375,12 +501,12 @@ TileGeom make_geom(int N) {
 
 }  // namespace
 
-// scratch sizes for the tiled deposit: bin counters + offsets, scan sums, bin keys, ranks, records
+// scratch sizes for the tiled deposit: bin counters + offsets, scan sums, ranks, records
 size_t deposit_scratch_bytes(int N, size_t np) {
   TileGeom g = make_geom(N);
   const size_t nb = 8 * (size_t)g.ntx * g.nty * g.ntz;
   const size_t nsum = (nb + 1023) / 1024;
-  return sizeof(unsigned) * (2 * (nb + 1) + nsum + 2 * np) + sizeof(float4) * np + 256 * 8;
+  return sizeof(unsigned) * (2 * (nb + 1) + nsum + np) + sizeof(float4) * np + 256 * 8;
 }
 
 int deposit_tiled(mgpu_ctx *ctx, float *d, const float4 *rsd) {
@@ -401,7 +527,6 @@ int deposit_tiled(mgpu_ctx *ctx, float *d, const float4 *rsd) {
   unsigned *count = (unsigned *)take(sizeof(unsigned) * (nb + 1));
   unsigned *off = (unsigned *)take(sizeof(unsigned) * (nb + 1));
   unsigned *sums = (unsigned *)take(sizeof(unsigned) * nsum);
-  unsigned *key = (unsigned *)take(sizeof(unsigned) * np);
   unsigned *rank = (unsigned *)take(sizeof(unsigned) * np);
   float4 *rec = (float4 *)take(sizeof(float4) * np);
   if ((size_t)(b - (char *)ctx->dep_scratch) > ctx->dep_scratch_bytes)
@@ -415,9 +540,9 @@ int deposit_tiled(mgpu_ctx *ctx, float *d, const float4 *rsd) {
   const double wpar = std::pow((double)N / (double)ctx->Np, 3);
   MGPU_HIP(ctx, hipMemsetAsync(count, 0, sizeof(unsigned) * (nb + 1), ctx->stream));
   if (np) {
-    KScope ks(ctx, MGPU_K_DEP_SORT, 20.0 * np);
-    hipLaunchKernelGGL(rsd ? k_dep_bin<true> : k_dep_bin<false>, dim3(cdiv(np, kTB)), dim3(kTB), 0, ctx->stream,
-                       ctx->pos, rsd, np, g, scale, count, key, rank);
+    KScope ks(ctx, MGPU_K_DEP_SORT, 16.0 * np);
+    hipLaunchKernelGGL(rsd ? k_dep_bin<true> : k_dep_bin<false>, dim3(cdiv(np, kBinU * kTB)), dim3(kTB), 0,
+                       ctx->stream, ctx->pos, rsd, np, g, scale, count, rank);
   }
   MGPU_HIP(ctx, hipGetLastError());
   hipLaunchKernelGGL(k_scan_block, dim3(nsum), dim3(1024), 0, ctx->stream, count, nb, off, sums);
@@ -425,9 +550,9 @@ int deposit_tiled(mgpu_ctx *ctx, float *d, const float4 *rsd) {
   hipLaunchKernelGGL(k_scan_add, dim3(nsum), dim3(1024), 0, ctx->stream, off, nb, sums, (unsigned)np, nb);
   MGPU_HIP(ctx, hipGetLastError());
   if (np) {
-    KScope ks(ctx, MGPU_K_DEP_SORT, 12.0 * np);
-    hipLaunchKernelGGL(rsd ? k_dep_scatter<true> : k_dep_scatter<false>, dim3(cdiv(np, kTB)), dim3(kTB), 0,
-                       ctx->stream, key, rank, off, ctx->pos, rsd, np, rec);
+    KScope ks(ctx, MGPU_K_DEP_SORT, 32.0 * np);
+    hipLaunchKernelGGL(rsd ? k_dep_scatter<true> : k_dep_scatter<false>, dim3(cdiv(np, kBinU * kTB)), dim3(kTB), 0,
+                       ctx->stream, rank, off, ctx->pos, rsd, np, g, scale, rec);
   }
   MGPU_HIP(ctx, hipGetLastError());
   const size_t shmem = sizeof(unsigned long long) * g.tx * g.ty * g.tz;
@@ -435,8 +560,19 @@ int deposit_tiled(mgpu_ctx *ctx, float *d, const float4 *rsd) {
   {
     // compulsory: the particle records read once, delta written
     KScope ks(ctx, MGPU_K_DEPOSIT, 16.0 * np + 4.0 * cells);
-    hipLaunchKernelGGL(rsd ? k_dep_tile<true> : k_dep_tile<false>, dim3(nt), dim3(kDepTileNT), shmem, ctx->stream, rec, off,
-                       g, scale, wpar, 2 * ctx->HP, d);
+    if (g.tx != 16 && g.tx != 8 && g.tx != 4) return mgpu_fail(ctx, MGPU_E_INVALID, "deposit tile side not 4, 8 or 16");
+    auto kern = rsd ? (g.tx == 16 ? k_dep_tiles<true, 16> : g.tx == 8 ? k_dep_tiles<true, 8> : k_dep_tiles<true, 4>)
+                    : (g.tx == 16 ? k_dep_tiles<false, 16> : g.tx == 8 ? k_dep_tiles<false, 8> : k_dep_tiles<false, 4>);
+    // persistent grid: the workgroups that fit the device at once, a multiple of 8 when there are enough tiles
+    if (!ctx->dep_tiles_grid) {
+      int per_cu = 0, cus = 0;
+      MGPU_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kDepTileNT, shmem));
+      MGPU_HIP(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+      ctx->dep_tiles_grid = (unsigned)std::max(1, per_cu * cus);
+    }
+    unsigned grid = (unsigned)std::min<size_t>(ctx->dep_tiles_grid, nt);
+    if (grid >= 8) grid -= grid % 8;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kDepTileNT), shmem, ctx->stream, rec, off, g, scale, wpar, 2 * ctx->HP, d);
   }
   MGPU_HIP(ctx, hipGetLastError());
   return MGPU_OK;

@@ -104,6 +104,7 @@ struct mgpu_ctx {
   float4 *dep_rec = nullptr;       // particle records (x, y, z, index bits) in bin order (inside dep_scratch)
   unsigned *dep_off = nullptr;     // bin offsets of dep_rec, 8 bins (face classes) per tile
   bool sorted_valid = false;       // dep_rec holds the current positions
+  unsigned dep_tiles_grid = 0;     // workgroups of the persistent tile deposit (0: not yet measured)
   unsigned *ic_scratch = nullptr;  // this device's seed-table scratch (allocation-free IC)
 
   // state

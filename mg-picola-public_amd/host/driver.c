@@ -194,6 +194,10 @@ static int output(sim *s, double A, double AF, double AFF, double z_label) {
   /* RSD multipoles at the output, with the velocities synchronised (src/main.c:875-879) */
   const int first_step = fabs(p->Init_Redshift - z_label) / p->Init_Redshift <= 1.0E-6;
   if (p->compute_pofk && p->pofk_compute_rsd_pofk >= 1 && !first_step && rsd_pofk(s, A)) return -1;
+  if (!p->write_particles) {
+    if (p->scale_dependent && lpt_fields(s, 1, MGP_FIELD_deltaD, A, AFF)) return -1;
+    return 0;
+  }
   float *pos = malloc(sizeof(float) * 3 * np), *vel = malloc(sizeof(float) * 3 * np);
   float *l0 = malloc(sizeof(float) * 3 * np), *l1 = malloc(sizeof(float) * 3 * np);
   int rc = mgpu_get_particles(s->g, pos, vel) || mgpu_get_lpt(s->g, 0, l0) || mgpu_get_lpt(s->g, 1, l1);

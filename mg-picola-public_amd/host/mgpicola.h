@@ -78,6 +78,10 @@ typedef struct {
    * the kick up to rounding (the kick term has zero mean); 0 runs the reference's
    * separate Kick and Drift with the mean of the kicked velocities (src/main.c:771). */
   int fuse_kick_drift;
+  /* host option: write the particle snapshot at each output (default 1); 0 keeps
+   * the outputs' P(k) and RSD files but skips the particle download and files
+   * (a 1024^3 snapshot is 26 GB) */
+  int write_particles;
 } mgp_params;
 
 /* ---- paramfile (src/read_param.c) ---- */

@@ -112,6 +112,7 @@ void mgp_params_defaults(mgp_params *p, int model) {
   p->gadget_style = 1;
   p->growth_on_gpu = 1;
   p->fuse_kick_drift = 1;
+  p->write_particles = 1;
 }
 
 int mgp_param_tag_count(const mgp_params *p) {

@@ -404,7 +404,7 @@ class HostParams(C.Structure):
         + [("Noutputs", C.c_int), ("out_z", C.c_double * 512), ("out_nsteps", C.c_int * 512)]
         + [("nu_include_massive_neutrinos", C.c_int), ("nu_SumMassNuEV", C.c_double),
            ("nu_FilenameTransferInfofile", C.c_char * 500), ("OmegaNu", C.c_double), ("OmegaCDM", C.c_double),
-           ("growth_on_gpu", C.c_int), ("fuse_kick_drift", C.c_int)]
+           ("growth_on_gpu", C.c_int), ("fuse_kick_drift", C.c_int), ("write_particles", C.c_int)]
     )
 
 

@@ -112,8 +112,8 @@ def test_memory_plan_by_part(n, ns, sd):
     assert parts["stored"] == (2 * sb if sd else 0)
     lat = 6 * ns * ns * hp(ns) * 8 if (sd and n % ns == 0) else 0
     assert parts["lattice"] == lat
-    # tile bins plus 2 u32 and one float4 record per particle
-    assert 24 * np_ <= parts["deposit"] <= 24 * np_ + 72 * (n // 8) ** 3 + (1 << 20)
+    # tile bins plus a u32 rank and a float4 record per particle
+    assert 20 * np_ <= parts["deposit"] <= 20 * np_ + 72 * (n // 8) ** 3 + (1 << 20)
     # per-block partial sums (3 doubles per 256 particles) plus the per-|n|^2 tables (26 bytes an entry)
     assert 0 < parts["tables"] < 3 * 8 * np_ // 256 + 26 * (3 * (n // 2) ** 2 + 1) + (16 << 20)
     assert parts["rsd"] == 16 * np_
