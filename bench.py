@@ -124,19 +124,22 @@ def gpu_bench(steps, warmup, workload):
                 steps_per_run=nsteps)
 
 
-def learned_leg(np_eval=512, box_eval=1024.0, nsteps=30, np_train=128, box_train=256.0, iterations=20):
+def learned_leg(np_eval=512, box_eval=1024.0, nsteps=30, np_train=128, box_train=256.0, iterations=30):
     """The metric's second part: the learned Green's-function multiplier on the
     benchmark's model (f(R) fR0 = 1e-5, scale-dependent, screening, 30 steps).
     A network is trained on the GPU on coarse (Nmesh = Np) / 3x-fine pairs at
     128^3 particles in 256 Mpc/h (the same 2 Mpc/h particle spacing and coarse
-    Nyquist k as the evaluation; seeds 101-103, LCDM and f(R)), then at 512^3 in
-    1024 Mpc/h the plain fine run (Nmesh 1536), the plain coarse run and the
-    learned coarse run (Nmesh 512) give max |P/P_fine - 1| over the bins with
-    k < 1 h/Mpc at z = 0 (64 linear bins up to the coarse Nyquist k, shot noise
-    subtracted) and their time per step (steps 2-30, as in the throughput
-    measurement: the first step carries one-time costs and the output with its
-    snapshot write is not a step); the same network's errors on the LCDM run of
-    the same initial conditions go in "lcdm"."""
+    Nyquist k as the evaluation; seeds 101-103, LCDM and f(R) with f_R0 = 1e-6,
+    1e-5 and 1e-4; the iterate kept is the best on held-out seed-104 pairs).
+    Then at 512^3 in 1024 Mpc/h the plain fine run (Nmesh 1536), the plain
+    coarse run and the learned coarse run (Nmesh 512) give max |P/P_fine - 1|
+    over the bins with k < 1 h/Mpc at z = 0 (64 linear bins up to the coarse
+    Nyquist k, shot noise subtracted) and their time per step (steps 2-30, as
+    in the throughput measurement: the first step carries one-time costs and
+    the output with its snapshot write is not a step).  The same network's
+    errors on the LCDM run of the same initial conditions (a model in its
+    training set) go in "lcdm", and on f(R) with f_R0 = 3e-5 (in no training or
+    validation pair) in "fofr_3e-5"."""
     import math
 
     import learned as LR
@@ -162,11 +165,19 @@ def learned_leg(np_eval=512, box_eval=1024.0, nsteps=30, np_train=128, box_train
         return M.read_params(pf, model=model)
 
     t0 = time.time()
-    pairs = [(params(f"c{s}_{f}", np_train, np_train, box_train, s, f),
-              params(f"f{s}_{f}", np_train, 3 * np_train, box_train, s, f)) for s in (101, 102, 103)
-             for f in (0.0, 1e-5)]
+
+    def pair(s, f):
+        return (params(f"c{s}_{f}", np_train, np_train, box_train, s, f),
+                params(f"f{s}_{f}", np_train, 3 * np_train, box_train, s, f))
+
+    # LCDM and f(R) with f_R0 spread over two decades (log-uniform nodes); sample weights sqrt(modes), and
+    # ln M = 0 anchors at k = 0 (the coarse mesh is exact at the largest scales)
+    pairs = [pair(s, f) for s in (101, 102, 103) for f in (0.0, 1e-6, 1e-5, 1e-4)]
+    # the kept iterate is chosen on held-out pairs (seed 104), never sampled
+    val = [pair(104, f) for f in (0.0, 1e-5)]
     mlp = M.MLP(seed=7)
     hist = LR.train(pairs, mlp, iterations=iterations, iters=1500, lr=3e-3, eta=0.6, eta_decay=0.93,
+                    kmax_select=1.0, val_pairs=val, wpow=0.5, anchor=4.0,
                     log=lambda m: print(f"[bench] learned: {m}", file=sys.stderr, flush=True))
     t_train = time.time() - t0
 
@@ -195,21 +206,32 @@ def learned_leg(np_eval=512, box_eval=1024.0, nsteps=30, np_train=128, box_train
     lrn, ms_lrn = timed_run(hc, mlp)
     e_plain, e_lrn = LR.pofk_error(fine, plain)[0], LR.pofk_error(fine, lrn)[0]
     print(f"[bench] learned: P(k) error plain {e_plain:.4f}, learned {e_lrn:.4f}", file=sys.stderr, flush=True)
-    # the same network on the LCDM run of the same initial conditions
+    # the same network on the LCDM run of the same initial conditions (a model it was trained on) and on f(R)
+    # with f_R0 = 3e-5, which no training or validation pair has
     fine_l, _ = timed_run(params("eval_fine_lcdm", np_eval, 3 * np_eval, box_eval, 5001, 0.0))
     hcl = params("eval_coarse_lcdm", np_eval, np_eval, box_eval, 5001, 0.0)
     el_plain = LR.pofk_error(fine_l, timed_run(hcl)[0])[0]
     el_lrn = LR.pofk_error(fine_l, timed_run(hcl, mlp)[0])[0]
     print(f"[bench] learned: LCDM P(k) error plain {el_plain:.4f}, learned {el_lrn:.4f}", file=sys.stderr, flush=True)
+    fine_o, _ = timed_run(params("eval_fine_3e-5", np_eval, 3 * np_eval, box_eval, 5001, 3e-5))
+    hco = params("eval_coarse_3e-5", np_eval, np_eval, box_eval, 5001, 3e-5)
+    eo_plain = LR.pofk_error(fine_o, timed_run(hco)[0])[0]
+    eo_lrn = LR.pofk_error(fine_o, timed_run(hco, mlp)[0])[0]
+    print(f"[bench] learned: f_R0 = 3e-5 P(k) error plain {eo_plain:.4f}, learned {eo_lrn:.4f}", file=sys.stderr,
+          flush=True)
     mlp.close()
     return {"pofk_err_learned": round(e_lrn, 4), "pofk_err_plain_coarse": round(e_plain, 4), "kmax": 1.0,
-            "lcdm": {"pofk_err_learned": round(el_lrn, 4), "pofk_err_plain_coarse": round(el_plain, 4)},
+            "lcdm": {"pofk_err_learned": round(el_lrn, 4), "pofk_err_plain_coarse": round(el_plain, 4),
+                     "note": "in distribution: LCDM pairs are in the training set"},
+            "fofr_3e-5": {"pofk_err_learned": round(eo_lrn, 4), "pofk_err_plain_coarse": round(eo_plain, 4),
+                          "note": "out of distribution: no training or validation pair has f_R0 = 3e-5"},
             "ms_per_step_fine": round(ms_fine, 2), "ms_per_step_coarse_learned": round(ms_lrn, 2),
             "ms_per_step_coarse_plain": round(ms_plain, 2), "speedup_vs_fine": round(ms_fine / ms_lrn, 2),
             "eval": f"{np_eval}^3 particles, Box {box_eval:g}, coarse Nmesh {np_eval} vs fine {3 * np_eval}, "
                     f"f(R) fR0=1e-5 scale-dependent + screening, {nsteps} steps, seed 5001",
             "train": f"{len(pairs)} pairs at {np_train}^3/Box {box_train:g} (coarse {np_train} vs {3 * np_train}), "
-                     f"{iterations} fixed-point iterations (best kept), {t_train:.0f} s",
+                     f"{iterations} fixed-point iterations, the iterate with the smallest k < 1 error on 2 "
+                     f"held-out pairs (seed 104) kept, {t_train:.0f} s",
             "train_history": [[h[0], round(h[2], 4)] for h in hist]}
 
 
@@ -303,10 +325,10 @@ def fft_leg(n=1024, reps=3):
                     "device time per transform"}
 
 
-def ensemble_leg(per_size=4, threads=12, nsteps=20):
-    """A slice of the batched nDGP ensemble (rc H0 = 1, density screening,
-    Box = 2 Np, Nmesh = 2 Np, 20 steps): per_size realizations at each of 96^3,
-    128^3 and 192^3 particles, in fixed-amplitude pairs (delta and -delta),
+def ensemble_leg(per_size=32, threads=12, nsteps=20):
+    """The batched nDGP ensemble (rc H0 = 1, density screening, Box = 2 Np,
+    Nmesh = 2 Np, 20 steps): per_size realizations at each of 96^3, 128^3 and
+    192^3 particles (all 96 by default), in fixed-amplitude pairs (delta and -delta),
     run one after the other and then concurrently on `threads` host threads
     (one libmgpu context and stream each).  Whole-ensemble wall time, set-up
     and initial conditions included, and the particle-steps/s it gives."""
@@ -326,7 +348,9 @@ def ensemble_leg(per_size=4, threads=12, nsteps=20):
                 f.write(f"0, {nsteps}\n")
             pf = os.path.join(d, "params.txt")
             M.write_paramfile(pf, vals, model=model)
-            hps.append(M.read_params(pf, model=model))
+            hp = M.read_params(pf, model=model)
+            hp.write_particles = 0  # no snapshot files at the z = 0 output
+            hps.append(hp)
     work = sum(h.Nsample ** 3 for h in hps) * nsteps
     seq, t_seq = E.run_ensemble([M.HostParams.from_buffer_copy(h) for h in hps], threads=1)
     print(f"[bench] ensemble: {len(hps)} runs one after the other {t_seq:.2f} s", file=sys.stderr, flush=True)

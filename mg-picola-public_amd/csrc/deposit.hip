@@ -5,7 +5,7 @@
 // binned by its T^3-cell tile and by its face class -- which of its cell's
 // local coordinates equal T - 1, the cells whose +1 corners reach into the
 // upper neighbours -- with wave-aggregated counters, a prefix sum and a
-// scatter of particle indices.  One workgroup then owns one tile: it reads
+// scatter of particle records.  One workgroup then owns one tile: it reads
 // the tile's particles and, from its seven lower neighbours, only the
 // particles of the face classes that reach it, adds every corner that lands
 // in the tile with 64-bit integer atomics on a T^3 block in LDS, and writes

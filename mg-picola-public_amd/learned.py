@@ -26,8 +26,9 @@ Training is a fixed-point iteration, all of it on the GPU through libmgpu:
      next iteration repeats 2-4;
   5. the iteration is not monotonic (a refit can overshoot into a coarse run
      whose boosted grid-scale forces heat the particles), so every iterate's
-     error on the pairs is measured, the last one's too, and the best network
-     is kept.
+     error is measured, the last one's too, and the best network is kept; the
+     error is taken on held-out pairs (seeds never sampled) when they are
+     given, else on the training pairs.
 The plain fine-mesh run stays the reference of the P(k) error.
 """
 import math
@@ -64,8 +65,11 @@ def run_logged(hp, mlp=None):
         sim.close()
 
 
-def samples(fine, coarse, hp_coarse, mlp, kmax, eta=0.3, clip=0.3):
-    """Training samples (x, t, w) of one coarse/fine pair of logs."""
+def samples(fine, coarse, hp_coarse, mlp, kmax, eta=0.3, clip=0.3, wpow=1.0, anchor=0.0):
+    """Training samples (x, t, w) of one coarse/fine pair of logs; w = modes**wpow.
+    anchor > 0 adds, per step, a sample at k = 0 with target ln M = 0 (the
+    coarse mesh is exact at the largest scales) and `anchor` times the mean
+    weight of the step's samples."""
     kny = math.pi * hp_coarse.Nmesh / hp_coarse.Box
     th = theta_of(hp_coarse)
     X, R, W = [], [], []
@@ -75,13 +79,21 @@ def samples(fine, coarse, hp_coarse, mlp, kmax, eta=0.3, clip=0.3):
     ok = (nc1 > 0) & (pc1 > 0) & (pf1 > 0) & (kc1 > 0) & (kc1 < kmax)
     k = kc1[ok]
     corr = np.clip(eta * np.log(pf1[ok] / pc1[ok]), -clip, clip)
+    wk = nc1[ok] ** wpow
+    anchored = []
     for (a, s, _, _, _), (af, _, _, _, _) in zip(coarse, fine):
         assert abs(a - af) < 1e-12
         X.append(np.stack([k / kny, np.full(k.size, a), np.full(k.size, th), np.full(k.size, s)], axis=1))
         R.append(corr)
-        W.append(nc1[ok])
+        W.append(wk)
+        if anchor > 0:
+            X.append(np.array([[0.0, a, th, s]]))
+            R.append(np.zeros(1))
+            W.append(np.array([anchor * wk.mean()]))
+            anchored.append(sum(len(x) for x in X) - 1)
     X = np.concatenate(X).astype(np.float32)
     base = mlp.forward(X).astype(np.float64) if mlp is not None else np.zeros(len(X))
+    base[anchored] = 0.0  # the anchors' target is ln M = 0 itself, not a correction of the current network
     return X, (base + np.concatenate(R)).astype(np.float32), np.concatenate(W).astype(np.float32)
 
 
@@ -94,12 +106,19 @@ def pofk_error(fine, coarse, kmax=1.0):
     return float(np.max(np.abs(pc[ok] / pf[ok] - 1.0))), kc[ok]
 
 
-def train(pairs, mlp, iterations=6, iters=1500, lr=3e-3, kmax=None, eta=0.6, eta_decay=1.0, log=None):
+def train(pairs, mlp, iterations=6, iters=1500, lr=3e-3, kmax=None, eta=0.6, eta_decay=1.0, log=None,
+          kmax_select=None, wpow=1.0, val_pairs=None, anchor=0.0):
     """pairs: [(hp_coarse, hp_fine)]; returns the training history
-    [(iteration, loss, mean P(k) error over the pairs before the refit)].
-    Iteration i uses the gain eta * eta_decay**i."""
+    [(iteration, loss, mean P(k) error before the refit)].
+    Iteration i uses the gain eta * eta_decay**i.  Samples cover k < kmax
+    (default: the coarse Nyquist k) with weights modes**wpow (and k = 0
+    anchors of relative weight `anchor`, see samples).  The error that
+    picks the kept iterate covers k < kmax_select (default: kmax) and is
+    measured on val_pairs (held out: never sampled) when given, else on the
+    training pairs."""
     keep = dict(pofk_subtract_shotnoise=0)
     fine_logs = [run_logged(clone(f, **keep)) for _, f in pairs]
+    val_logs = [run_logged(clone(f, **keep)) for _, f in val_pairs] if val_pairs else None
     hist = []
     best = (math.inf, None, -1)
     for it in range(iterations + 1):
@@ -108,12 +127,18 @@ def train(pairs, mlp, iterations=6, iters=1500, lr=3e-3, kmax=None, eta=0.6, eta
         for (c, _), fl in zip(pairs, fine_logs):
             cl = run_logged(clone(c, **keep), mlp)
             km = kmax if kmax is not None else math.pi * c.Nmesh / c.Box
-            errs.append(pofk_error(fl, cl, km)[0])
+            if val_logs is None:
+                errs.append(pofk_error(fl, cl, kmax_select if kmax_select is not None else km)[0])
             if it < iterations:
-                x, t, w = samples(fl, cl, c, mlp, km, eta=e, clip=0.5 * e)
+                x, t, w = samples(fl, cl, c, mlp, km, eta=e, clip=0.5 * e, wpow=wpow, anchor=anchor)
                 Xs.append(x)
                 Ts.append(t)
                 Ws.append(w)
+        if val_logs is not None:
+            for (c, _), fl in zip(val_pairs, val_logs):
+                km = kmax if kmax is not None else math.pi * c.Nmesh / c.Box
+                errs.append(pofk_error(fl, run_logged(clone(c, **keep), mlp),
+                                       kmax_select if kmax_select is not None else km)[0])
         err = float(np.mean(errs))
         if err < best[0]:
             best = (err, mlp.params.copy(), it)
