@@ -170,14 +170,14 @@ def learned_leg(np_eval=512, box_eval=1024.0, nsteps=30, np_train=128, box_train
         return (params(f"c{s}_{f}", np_train, np_train, box_train, s, f),
                 params(f"f{s}_{f}", np_train, 3 * np_train, box_train, s, f))
 
-    # LCDM and f(R) with f_R0 spread over two decades (log-uniform nodes); sample weights sqrt(modes), and
+    # LCDM and f(R) with f_R0 spread over two decades (log-uniform nodes); sample weights modes^(1/4), and
     # ln M = 0 anchors at k = 0 (the coarse mesh is exact at the largest scales)
     pairs = [pair(s, f) for s in (101, 102, 103) for f in (0.0, 1e-6, 1e-5, 1e-4)]
     # the kept iterate is chosen on held-out pairs (seed 104), never sampled
     val = [pair(104, f) for f in (0.0, 1e-5)]
     mlp = M.MLP(seed=7)
     hist = LR.train(pairs, mlp, iterations=iterations, iters=1500, lr=3e-3, eta=0.6, eta_decay=0.93,
-                    kmax_select=1.0, val_pairs=val, wpow=0.5, anchor=4.0,
+                    kmax_select=1.0, val_pairs=val, wpow=0.25, anchor=4.0,
                     log=lambda m: print(f"[bench] learned: {m}", file=sys.stderr, flush=True))
     t_train = time.time() - t0
 

@@ -133,7 +133,7 @@ def test_learned_coarse_1024_fofr_nu_rsd_run(tmp_path):
     mlp = M.MLP(seed=7)
     t0 = time.time()
     LR.train([train_pair(tmp_path, s, f) for s in (101, 102, 103) for f in (0.0, 1e-6, 1e-5, 1e-4)], mlp,
-             iterations=30, iters=1500, lr=3e-3, eta=0.6, eta_decay=0.93, kmax_select=1.0, wpow=0.5, anchor=4.0,
+             iterations=30, iters=1500, lr=3e-3, eta=0.6, eta_decay=0.93, kmax_select=1.0, wpow=0.25, anchor=4.0,
              val_pairs=[train_pair(tmp_path, 104, f) for f in (0.0, 1e-5)])
     print(f"network trained in {time.time() - t0:.0f} s", flush=True)
     lrn = run(tmp_path, "learned", mlp)
