@@ -55,3 +55,28 @@ def test_theta_and_pofk_error():
     sel = (k > 0) & (k < 1.0)
     assert np.allclose(ks, k[sel])
     assert abs(err - np.max(np.abs(1 / r[sel] - 1))) < 1e-12
+
+
+def test_samples_weight_power_and_k0_anchors():
+    """wpow sets the weights to modes**wpow; anchor adds, after each step's bins, one sample at k = 0 with
+    target ln M = 0 (not a correction of the current network) and anchor times the step's mean weight."""
+    r = np.array([1.0, 1.1, 1.2, 1.3, 1.5, 2.0, 3.0, 9.0])
+    fine, coarse, k, modes = logs(r)
+    kmax = math.pi * 32 / 64.0
+    ok = (modes > 0) & (k > 0) & (k < kmax)
+    nk = ok.sum()
+
+    class Net:  # a network whose output is 0.7 everywhere: targets are 0.7 + correction, anchors stay 0
+        def forward(self, x):
+            return np.full(len(x), 0.7, np.float32)
+
+    x, t, w = LR.samples(fine, coarse, HP, Net(), kmax, eta=0.6, clip=0.3, wpow=0.25, anchor=4.0)
+    assert x.shape == (3 * (nk + 1), 4)
+    for s, a in enumerate([0.05, 0.5, 1.0]):
+        blk = slice(s * (nk + 1), s * (nk + 1) + nk)
+        anc = s * (nk + 1) + nk
+        np.testing.assert_allclose(w[blk], modes[ok] ** 0.25, rtol=1e-6)
+        np.testing.assert_allclose(t[blk], 0.7 + np.clip(0.6 * np.log(r[ok]), -0.3, 0.3), rtol=1e-6)
+        assert x[anc, 0] == 0.0 and x[anc, 1] == np.float32(a) and t[anc] == 0.0
+        np.testing.assert_allclose(x[anc, 3], 0.3 + a, rtol=1e-6)
+        np.testing.assert_allclose(w[anc], 4.0 * np.mean(modes[ok] ** 0.25), rtol=1e-6)
