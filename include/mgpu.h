@@ -185,6 +185,26 @@ int mgpu_drift(mgpu_ctx *ctx, double dyyy, double s1, double s2, const double su
  * reduced by sumD), so a further mgpu_kick needs a new mgpu_mesh_to_particles. */
 int mgpu_kick_drift(mgpu_ctx *ctx, double dda, double A, double s1k, double s2k, const double sumD[3], double dyyy,
                     double s1d, double s2d, double sumV[3]);
+/* One COLA step between outputs that never waits for the device: the
+ * GetDisplacements of mgpu_get_displacements (without P(k)) and the Kick and
+ * Drift of mgpu_kick_drift, with the means of Disp and of the last kick's
+ * velocities kept in device memory (the reference's sumDxyz and sumxyz,
+ * src/auxPM.c:635-644, src/main.c:735-740) instead of coming back to the host.
+ * With graph != 0 the step's launches are captured in a hipGraph on each call
+ * (after one eager call) and the context's executable graph is updated in place
+ * with this step's scalars and launched: one launch per step.  Capture needs
+ * the process to itself: while another host thread makes HIP calls (several
+ * contexts stepping at once), the runtime invalidates the capture and the call
+ * fails with MGPU_E_HIP; use graph = 0 there.  Results are bit-identical to
+ * mgpu_get_displacements + mgpu_kick_drift.  For runs whose
+ * step is one fused sequence: not scale-dependent, no P(k) that step, no
+ * learned multiplier or neutrino term, timers off; the velocity sums must be on
+ * the device (any mgpu_kick or mgpu_kick_drift since the particles were set).
+ * Other cases return MGPU_E_STATE.  Read the means with mgpu_get_means. */
+int mgpu_step(mgpu_ctx *ctx, const mgpu_mg *mg, double dda, double A, double s1k, double s2k, double dyyy,
+              double s1d, double s2d, int graph);
+/* The device-resident means: Disp of the last gather, velocities of the last kick. */
+int mgpu_get_means(mgpu_ctx *ctx, double sumD[3], double sumV[3]);
 
 /* ---- initial conditions ---- */
 /* displacement_fields (Gaussian): seed table + per-column RANLUX stream,

@@ -297,6 +297,7 @@ int mgpu_create(const mgpu_params *p, mgpu_ctx **out) {
     return fail(mgpu_fail(ctx, MGPU_E_HIP, "event creation failed"));
   // defined contents everywhere
   if (hipMemsetAsync(ctx->vel, 0, pb, ctx->stream) != hipSuccess ||
+      hipMemsetAsync(ctx->dsum, 0, 8 * sizeof(double), ctx->stream) != hipSuccess ||
       hipMemsetAsync(ctx->lpt[0], 0, pb, ctx->stream) != hipSuccess ||
       hipMemsetAsync(ctx->lpt[1], 0, pb, ctx->stream) != hipSuccess ||
       hipMemsetAsync(ctx->disp, 0, pb, ctx->stream) != hipSuccess)
@@ -318,6 +319,7 @@ void mgpu_destroy(mgpu_ctx *ctx) {
   if (!ctx) return;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   flush_ktimers(ctx);
+  if (ctx->step_exec) (void)hipGraphExecDestroy(ctx->step_exec);
   void *ptrs[] = {ctx->pos,      ctx->vel,       ctx->disp,      ctx->lpt[0],    ctx->lpt[1],
                   ctx->slot[0],  ctx->slot[1],   ctx->slot[2],   ctx->slot[3],   ctx->cdelta[0],
                   ctx->cdelta[1], ctx->partial,  ctx->dsum,      ctx->pk,        ctx->n2tab[0],
@@ -344,6 +346,7 @@ int mgpu_sync(mgpu_ctx *ctx) {
 
 int mgpu_set_particles(mgpu_ctx *ctx, const float *pos, const float *vel) {
   if (ctx) ctx->sorted_valid = false;
+  if (ctx && vel) ctx->vsum_valid = false;
   if (check_ctx(ctx)) return MGPU_E_INVALID;
   const size_t b = 3 * ctx->np * sizeof(float);
   if (pos) MGPU_HIP(ctx, hipMemcpyAsync(ctx->pos, pos, b, hipMemcpyHostToDevice, ctx->stream));
@@ -624,6 +627,82 @@ int mgpu_kick_drift(mgpu_ctx *ctx, double dda, double A, double s1k, double s2k,
   return rc;
 }
 
+// The step body of mgpu_step: the fused GetDisplacements (its Disp sums left in
+// dsum[0..2]) and Kick + Drift reading the means from dsum; no host wait.
+static int device_step_body(mgpu_ctx *ctx, const mgpu_mg *mg, double dda, double A, double s1k, double s2k,
+                            double dyyy, double s1d, double s2d) {
+  ctx->have_deposit = ctx->have_density = ctx->have_forces = false;
+  ctx->have_phik = false;
+  int rc = pm_step_fused(ctx, mg, nullptr, nullptr);
+  if (rc) return rc;
+  return pm_kick_drift(ctx, dda, A, s1k, s2k, nullptr, dyyy, s1d, s2d, nullptr);
+}
+
+int mgpu_step(mgpu_ctx *ctx, const mgpu_mg *mg, double dda, double A, double s1k, double s2k, double dyyy,
+              double s1d, double s2d, int graph) {
+  if (check_ctx(ctx)) return MGPU_E_INVALID;
+  if (ctx->p.mg_active && ctx->p.model != MGPU_LCDM && !mg)
+    return mgpu_fail(ctx, MGPU_E_INVALID, "modified gravity active but no mgpu_mg coefficients given");
+  if (ctx->p.scale_dependent)
+    return mgpu_fail(ctx, MGPU_E_STATE,
+                     "scale-dependent runs refill the LPT fields between the force and the kick: use the separate calls");
+  if (!pm_step_fusable(ctx, mg, nullptr) || ctx->learn_on || ctx->nu_on)
+    return mgpu_fail(ctx, MGPU_E_STATE, "this model's step is not one fused sequence: use the separate calls");
+  if (ctx->timers_on || ctx->ktimers_on)
+    return mgpu_fail(ctx, MGPU_E_STATE, "timers wait for the device inside a step: disable them for mgpu_step");
+  if (!ctx->vsum_valid)
+    return mgpu_fail(ctx, MGPU_E_STATE, "no velocity sums on the device: the first step takes mgpu_kick or "
+                                        "mgpu_kick_drift");
+  int rc;
+  if (graph && ctx->step_warm) {
+    hipGraph_t g = nullptr;
+    MGPU_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+    rc = device_step_body(ctx, mg, dda, A, s1k, s2k, dyyy, s1d, s2d);
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc || e != hipSuccess || !g) {
+      if (g) (void)hipGraphDestroy(g);
+      (void)hipGetLastError();
+      return rc ? rc : mgpu_fail(ctx, MGPU_E_HIP, std::string("step capture failed: ") + hipGetErrorString(e));
+    }
+    if (ctx->step_exec) {
+      // same kernels and shapes every step: only the scalar arguments change
+      hipGraphNode_t en = nullptr;
+      hipGraphExecUpdateResult ur = hipGraphExecUpdateSuccess;
+      if (hipGraphExecUpdate(ctx->step_exec, g, &en, &ur) != hipSuccess || ur != hipGraphExecUpdateSuccess) {
+        (void)hipGetLastError();
+        (void)hipGraphExecDestroy(ctx->step_exec);
+        ctx->step_exec = nullptr;
+      }
+    }
+    if (!ctx->step_exec && hipGraphInstantiate(&ctx->step_exec, g, nullptr, nullptr, 0) != hipSuccess) {
+      ctx->step_exec = nullptr;
+      (void)hipGraphDestroy(g);
+      return mgpu_fail(ctx, MGPU_E_HIP, "step graph instantiation failed");
+    }
+    (void)hipGraphDestroy(g);
+    MGPU_HIP(ctx, hipGraphLaunch(ctx->step_exec, ctx->stream));
+  } else {
+    if ((rc = device_step_body(ctx, mg, dda, A, s1k, s2k, dyyy, s1d, s2d))) return rc;
+    ctx->step_warm = true;
+  }
+  ctx->have_density = ctx->have_forces = true;
+  ctx->have_disp = false;  // Disp was not reduced by the mean
+  return MGPU_OK;
+}
+
+int mgpu_get_means(mgpu_ctx *ctx, double sumD[3], double sumV[3]) {
+  if (check_ctx(ctx)) return MGPU_E_INVALID;
+  double s[6];
+  MGPU_HIP(ctx, hipMemcpyAsync(s, ctx->dsum, sizeof s, hipMemcpyDeviceToHost, ctx->stream));
+  MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const double n = (double)std::max<size_t>(ctx->np, 1);
+  for (int a = 0; a < 3; a++) {
+    if (sumD) sumD[a] = s[a] / n;
+    if (sumV) sumV[a] = s[3 + a] / n;
+  }
+  return MGPU_OK;
+}
+
 int mgpu_ic_displacement_fields(mgpu_ctx *ctx, unsigned long seed, int sphere_mode, int amplitude_fixed, int inverted,
                                 const double *pk_n2) {
   if (check_ctx(ctx) || !pk_n2) return MGPU_E_INVALID;
@@ -641,7 +720,7 @@ int mgpu_ic_displacement_fields_noalloc(mgpu_ctx *ctx, unsigned long seed, int s
 }
 
 int mgpu_init_particles(mgpu_ctx *ctx, double Di, double Di2) {
-  if (ctx) ctx->sorted_valid = false;
+  if (ctx) ctx->sorted_valid = false, ctx->vsum_valid = false;
   if (check_ctx(ctx)) return MGPU_E_INVALID;
   int rc = ic_init_particles(ctx, Di, Di2);
   if (rc) return rc;

@@ -145,6 +145,11 @@ __global__ __launch_bounds__(kTB) void k_dep_bin(const float *__restrict__ pos, 
   }
 }
 
+// the bin counters cleared (grid-stride)
+__global__ __launch_bounds__(256) void k_zero_u32(unsigned *__restrict__ a, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] = 0u;
+}
+
 // 2. exclusive prefix sum of the tile counts (three launches)
 __global__ __launch_bounds__(1024) void k_scan_block(const unsigned *__restrict__ in, size_t n,
                                                      unsigned *__restrict__ out, unsigned *__restrict__ sums) {
@@ -538,7 +543,9 @@ int deposit_tiled(mgpu_ctx *ctx, float *d, const float4 *rsd) {
   ctx->sorted_valid = np > 0 && !rsd;
   const double scale = (double)N / ctx->p.box;
   const double wpar = std::pow((double)N / (double)ctx->Np, 3);
-  MGPU_HIP(ctx, hipMemsetAsync(count, 0, sizeof(unsigned) * (nb + 1), ctx->stream));
+  // cleared by a kernel, not hipMemsetAsync, so that the step can be captured in a hipGraph
+  hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)std::min<size_t>(cdiv(nb + 1, 1024), 65535)), dim3(256), 0,
+                     ctx->stream, count, nb + 1);
   if (np) {
     KScope ks(ctx, MGPU_K_DEP_SORT, 16.0 * np);
     hipLaunchKernelGGL(rsd ? k_dep_bin<true> : k_dep_bin<false>, dim3(cdiv(np, kBinU * kTB)), dim3(kTB), 0,

@@ -105,6 +105,11 @@ struct mgpu_ctx {
   unsigned *dep_off = nullptr;     // bin offsets of dep_rec, 8 bins (face classes) per tile
   bool sorted_valid = false;       // dep_rec holds the current positions
   unsigned dep_tiles_grid = 0;     // workgroups of the persistent tile deposit (0: not yet measured)
+  // device-resident step (mgpu_step): dsum[0..2] the Disp sums of the last gather, dsum[3..5] the velocity
+  // sums of the last kick; vsum_valid when dsum[3..5] matches the velocities
+  bool vsum_valid = false;
+  bool step_warm = false;           // one eager step has run (kernel attributes and launch shapes settled)
+  hipGraphExec_t step_exec = nullptr;  // the per-step graph, updated in place each call
   unsigned *ic_scratch = nullptr;  // this device's seed-table scratch (allocation-free IC)
 
   // state
@@ -236,7 +241,7 @@ int pm_kick(mgpu_ctx *ctx, double dda, double A, double s1, double s2, const dou
 int pm_drift(mgpu_ctx *ctx, double dyyy, double s1, double s2, const double sumV[3]);
 int pm_kick_drift(mgpu_ctx *ctx, double dda, double A, double s1k, double s2k, const double sumD[3], double dyyy,
                   double s1d, double s2d, double sumV[3]);
-int pm_reduce3(mgpu_ctx *ctx, int nblocks, double out[3]);
+int pm_reduce3(mgpu_ctx *ctx, int nblocks, double out[3], double *dst);
 
 // ---------------------------------------------------------------- IC / LPT (ic.hip)
 // seed_dev: nullptr allocates the seed table for the call; else the per-device scratch

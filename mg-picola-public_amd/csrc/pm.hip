@@ -336,11 +336,21 @@ __global__ __launch_bounds__(kBlock) void k_kick_drift(float *__restrict__ vel, 
                                                        double s2k, int sdd, double s1d, double s2d, double omega,
                                                        double dda, double A, double dyyy, float box, int use_cola,
                                                        double sD0, double sD1, double sD2, double sV0, double sV1,
-                                                       double sV2, double *__restrict__ partial) {
+                                                       double sV2, const double *__restrict__ dsum,
+                                                       double *__restrict__ partial) {
 #pragma clang fp contract(off)  // same rounding sequence as the separate passes
   const size_t base = (size_t)blockIdx.x * kChunk + threadIdx.x;
   double s[3] = {0, 0, 0};
-  const double sD[3] = {sD0, sD1, sD2}, sV[3] = {sV0, sV1, sV2};
+  double sD[3] = {sD0, sD1, sD2}, sV[3] = {sV0, sV1, sV2};
+  if (dsum) {
+    // the means from the device-resident sums (Disp in dsum[0..2], the last kick's velocities in
+    // dsum[3..5]), divided as the host divides them
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      sD[a] = dsum[a] / (double)np;
+      sV[a] = dsum[3 + a] / (double)np;
+    }
+  }
 #pragma unroll 2
   for (int r = 0; r < kRed; r++) {
     const size_t i = base + (size_t)r * kBlock;
@@ -366,10 +376,14 @@ __global__ __launch_bounds__(kBlock) void k_kick_drift(float *__restrict__ vel, 
 
 // =================================================================== host side
 
-int pm_reduce3(mgpu_ctx *ctx, int nblocks, double out[3]) {
-  hipLaunchKernelGGL(k_reduce3, dim3(1), dim3(1024), 0, ctx->stream, ctx->partial, nblocks, ctx->dsum);
+// the per-block partials reduced into dst (3 doubles on the device: ctx->dsum
+// for Disp, ctx->dsum + 3 for the velocities), and copied to out unless out is
+// nullptr (then nothing waits for the device)
+int pm_reduce3(mgpu_ctx *ctx, int nblocks, double out[3], double *dst) {
+  hipLaunchKernelGGL(k_reduce3, dim3(1), dim3(1024), 0, ctx->stream, ctx->partial, nblocks, dst);
   MGPU_HIP(ctx, hipGetLastError());
-  MGPU_HIP(ctx, hipMemcpyAsync(out, ctx->dsum, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  if (!out) return MGPU_OK;
+  MGPU_HIP(ctx, hipMemcpyAsync(out, dst, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return MGPU_OK;
 }
@@ -675,7 +689,7 @@ int pm_forces(mgpu_ctx *ctx) {
 int pm_gather(mgpu_ctx *ctx, double sumD[3]) {
   PhaseTimer pt(ctx, &ctx->t.mtoparticles);
   if (ctx->np == 0) {
-    sumD[0] = sumD[1] = sumD[2] = 0;
+    if (sumD) sumD[0] = sumD[1] = sumD[2] = 0;
     return MGPU_OK;
   }
   const int N = ctx->N;
@@ -703,8 +717,8 @@ int pm_gather(mgpu_ctx *ctx, double sumD[3]) {
   }
   MGPU_HIP(ctx, hipGetLastError());
   double s[3];
-  int rc = pm_reduce3(ctx, (int)nb, s);
-  if (rc) return rc;
+  int rc = pm_reduce3(ctx, (int)nb, sumD ? s : nullptr, ctx->dsum);
+  if (rc || !sumD) return rc;
   for (int a = 0; a < 3; a++) sumD[a] = s[a] / (double)ctx->np;
   return MGPU_OK;
 }
@@ -726,7 +740,8 @@ int pm_kick(mgpu_ctx *ctx, double dda, double A, double s1, double s2, const dou
   }
   MGPU_HIP(ctx, hipGetLastError());
   double s[3];
-  int rc = pm_reduce3(ctx, (int)nb, s);
+  ctx->vsum_valid = true;
+  int rc = pm_reduce3(ctx, (int)nb, s, ctx->dsum + 3);
   if (rc) return rc;
   for (int a = 0; a < 3; a++) sumV[a] = s[a] / (double)ctx->np;
   return MGPU_OK;
@@ -748,14 +763,19 @@ int pm_drift(mgpu_ctx *ctx, double dyyy, double s1, double s2, const double sumV
   return MGPU_OK;
 }
 
+// sumD and sumV nullptr: the means are read from the device-resident sums and
+// the new velocity sums stay on the device (nothing waits for the device)
 int pm_kick_drift(mgpu_ctx *ctx, double dda, double A, double s1k, double s2k, const double sumD[3], double dyyy,
                   double s1d, double s2d, double sumV[3]) {
   PhaseTimer pt(ctx, &ctx->t.kick);
   ctx->sorted_valid = false;
+  const bool dev = !sumD;
   if (ctx->np == 0) {
-    sumV[0] = sumV[1] = sumV[2] = 0;
+    if (sumV) sumV[0] = sumV[1] = sumV[2] = 0;
     return MGPU_OK;
   }
+  const double z3[3] = {0.0, 0.0, 0.0};
+  const double *hD = dev ? z3 : sumD, *hV = dev ? z3 : sumV;
   const unsigned nb = cdiv(ctx->np, kChunk);
   const bool sdep = ctx->p.scale_dependent;
   {
@@ -764,13 +784,14 @@ int pm_kick_drift(mgpu_ctx *ctx, double dda, double A, double s1k, double s2k, c
     hipLaunchKernelGGL(k_kick_drift, dim3(nb), dim3(kBlock), 0, ctx->stream, ctx->vel, ctx->pos,
                        (const float *)ctx->disp, (const float *)ctx->lpt[0], (const float *)ctx->lpt[1], ctx->np,
                        sdep ? 1 : 0, s1k, s2k, sdep ? 2 : 0, s1d, s2d, ctx->p.omega, dda, A, dyyy,
-                       (float)ctx->p.box, ctx->p.use_cola, sumD[0], sumD[1], sumD[2], sumV[0], sumV[1], sumV[2],
-                       ctx->partial);
+                       (float)ctx->p.box, ctx->p.use_cola, hD[0], hD[1], hD[2], hV[0], hV[1], hV[2],
+                       dev ? (const double *)ctx->dsum : nullptr, ctx->partial);
   }
   MGPU_HIP(ctx, hipGetLastError());
   double s[3];
-  int rc = pm_reduce3(ctx, (int)nb, s);
-  if (rc) return rc;
+  ctx->vsum_valid = true;
+  int rc = pm_reduce3(ctx, (int)nb, dev ? nullptr : s, ctx->dsum + 3);
+  if (rc || dev) return rc;
   for (int a = 0; a < 3; a++) sumV[a] = s[a] / (double)ctx->np;
   return MGPU_OK;
 }

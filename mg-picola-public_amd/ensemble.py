@@ -45,6 +45,11 @@ def run_ensemble(hps, threads=8, keep_particles=False):
     (per-realization results in the order of hps, wall seconds of the whole
     ensemble)."""
     t0 = time.perf_counter()
+    if threads > 1:
+        # a hipGraph capture needs the process to itself: concurrent runs keep the device-resident step
+        # without graphs
+        for hp in hps:
+            hp.device_step = min(hp.device_step, 1)
     if threads <= 1:
         res = [run_one(hp, keep_particles) for hp in hps]
     else:

@@ -33,6 +33,7 @@ struct mgp_sim {
   /* learned Green's-function multiplier (host option) and the per-step log of
    * (a, density feature, CDM P(k)) that its training reads */
   int vmean_valid; /* sumV holds the mean of the current velocities (after a kick) */
+  int means_on_device; /* the last steps went through mgpu_step: sumD and sumV are on the device */
   mgpu_mlp *mlp;
   int learn_track;
   double learn_theta;
@@ -131,6 +132,37 @@ static int drift(sim *s, double A, double AFF, double AF, double Di, double Di2)
 /* Kick then Drift in one GPU pass (the steps with no output between them): the
  * same time factors as kick() and drift(); the drift's mean velocity is the one
  * before the kick (s->sumV, valid once a kick has run). */
+/* the coefficients of mgpu_kick_drift / mgpu_step for one step */
+static void kick_drift_coeffs(sim *s, double AI, double AF, double A, double AFF, double Di, double Di2, double c[7]) {
+  double dda, dyyy;
+  if (s->StdDA == 0) {
+    dda = mgp_Sphi(s->c, AI, AF, A);
+    dyyy = mgp_Sq(s->c, A, AFF, AF);
+  } else {
+    extern double mgp_SphiStd(const mgp_cosmo *, double, double);
+    extern double mgp_SqStd(const mgp_cosmo *, double, double);
+    dda = mgp_SphiStd(s->c, AI, AF);
+    dyyy = mgp_SqStd(s->c, A, AFF);
+  }
+  double s1k = 0, s2k = 0, s1d = 0, s2d = 0;
+  if (!s->p->scale_dependent) {
+    s1k = mgp_growth(s->c, MGP_G_ddDddy, A);
+    s2k = mgp_growth(s->c, MGP_G_ddD2ddy, A);
+    s1d = mgp_growth(s->c, MGP_G_D, AFF) - Di;
+    s2d = mgp_growth(s->c, MGP_G_D2, AFF) - Di2;
+  }
+  c[0] = dda, c[1] = s1k, c[2] = s2k, c[3] = dyyy, c[4] = s1d, c[5] = s2d, c[6] = A;
+}
+
+/* mgpu_step may apply: nothing of this step is read back on the host (mgpu_step
+ * itself refuses, with MGPU_E_STATE, the models whose step is not one fused
+ * sequence, and those steps take the separate calls) */
+static int device_step_ok(const sim *s) {
+  const mgp_params *p = s->p;
+  return p->device_step && !p->scale_dependent && !(p->compute_pofk && p->pofk_compute_every_step) &&
+         !(p->compute_pofk && p->pofk_compute_rsd_pofk == 1) && !s->mlp && !s->learn_track;
+}
+
 static int kick_drift(sim *s, double AI, double AF, double A, double AFF, double Di, double Di2) {
   double dda, dyyy;
   if (s->StdDA == 0) {
@@ -480,11 +512,34 @@ int mgp_sim_advance(mgp_sim *s, int *counted) {
     if (s->i != p->Noutputs) printf("[mgpicola] step a = %f -> %f\n", A, s->AFF);
     else printf("[mgpicola] final half timestep\n");
   }
+  const int output_now = s->ts == 0 && s->i != s->NoutputStart;
+  int stepped = 0;
+  if (!output_now && s->vmean_valid && p->fuse_kick_drift && device_step_ok(s)) {
+    /* the whole step on the device: the means stay there */
+    mgpu_mg mg;
+    mgp_mg_coeffs(s->c, A, &mg);
+    double c[7];
+    kick_drift_coeffs(s, s->AI, s->AF, A, s->AFF, s->Di, s->Di2, c);
+    int rc = mgpu_step(s->g, p->modified_gravity_active ? &mg : NULL, c[0], c[6], c[1], c[2], c[3], c[4], c[5],
+                       p->device_step == 2);
+    if (rc == 0) {
+      stepped = 1;
+      s->means_on_device = 1;
+    } else if (rc != MGPU_E_STATE) {
+      snprintf(s->err, s->errlen, "mgpu_step failed (%d): %s", rc, mgpu_last_error(s->g));
+      return -1;
+    }
+  }
+  if (!stepped && s->means_on_device) {
+    /* back to the separate calls: the last kick's velocity mean from the device */
+    GPU(mgpu_get_means(s->g, NULL, s->sumV));
+    s->means_on_device = 0;
+  }
+  if (!stepped) {
   if (get_displacements(s, A)) return -1;
   if (p->scale_dependent) {
     if (lpt_fields2(s, MGP_FIELD_ddDddy, MGP_FIELD_deltaD, A, s->AFF)) return -1;
   }
-  const int output_now = s->ts == 0 && s->i != s->NoutputStart;
   if (!output_now && s->vmean_valid && p->fuse_kick_drift) {
     if (kick_drift(s, s->AI, s->AF, A, s->AFF, s->Di, s->Di2)) return -1;
   } else {
@@ -503,11 +558,12 @@ int mgp_sim_advance(mgp_sim *s, int *counted) {
     }
     if (drift(s, A, s->AFF, s->AF, s->Di, s->Di2)) return -1;
   }
+  }
   s->A = s->AFF;
   s->AI = s->AF;
   s->Di = mgp_growth(s->c, MGP_G_D, s->A);
   s->Di2 = mgp_growth(s->c, MGP_G_D2, s->A);
-  GPU(mgpu_sync(s->g));
+  if (!stepped) GPU(mgpu_sync(s->g)); /* device steps are left in flight: the next call queues behind them */
   s->loop_time += now_s() - t0;
   s->steps_done++;
   *counted = 1;

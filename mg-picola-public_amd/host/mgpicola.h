@@ -82,6 +82,12 @@ typedef struct {
    * the outputs' P(k) and RSD files but skips the particle download and files
    * (a 1024^3 snapshot is 26 GB) */
   int write_particles;
+  /* host option: the steps between outputs through mgpu_step, with the means kept
+   * on the device and no wait for the device inside a step (1), each step also
+   * captured and launched as one hipGraph (2: only for a run that has the process
+   * to itself), or the separate calls (0, default).  Used where the step is one
+   * fused sequence (see mgpu_step); other steps take the separate calls. */
+  int device_step;
 } mgp_params;
 
 /* ---- paramfile (src/read_param.c) ---- */

@@ -29,7 +29,8 @@ MGPU_SYMBOLS = [
     "mgpu_get_displacements", "mgpu_kick", "mgpu_drift", "mgpu_ic_displacement_fields", "mgpu_init_particles",
     "mgpu_lpt_fields", "mgpu_get_mesh", "mgpu_set_mesh", "mgpu_get_timers", "mgpu_enable_timers",
     "mgpu_kernel_name", "mgpu_enable_kernel_timing", "mgpu_kernel_timing", "mgpu_lpt_fields_slices",
-    "mgpu_lpt_fields_slices2", "mgpu_ic_displacement_fields_noalloc", "mgpu_memory_plan",
+    "mgpu_lpt_fields_slices2", "mgpu_ic_displacement_fields_noalloc", "mgpu_memory_plan", "mgpu_step",
+    "mgpu_get_means",
 ]
 
 
@@ -305,6 +306,22 @@ class MGPU:
         self._ck(L.mgpu_kick_drift(self.ctx, dda, A, s1k, s2k, sd, dyyy, s1d, s2d, sv), "kick_drift")
         return np.array(sv[:])
 
+    def step(self, dda, A, s1k, s2k, dyyy, s1d, s2d, mg=None, graph=False):
+        """One whole step with the means on the device (mgpu_step); nothing comes back."""
+        L = lib()
+        d = C.c_double
+        L.mgpu_step.argtypes = [C.c_void_p, C.c_void_p, d, d, d, d, d, d, d, C.c_int]
+        self._ck(L.mgpu_step(self.ctx, None if mg is None else C.byref(mg), dda, A, s1k, s2k, dyyy, s1d, s2d,
+                             1 if graph else 0), "step")
+
+    def get_means(self):
+        """(mean Disp of the last gather, mean velocity of the last kick) from the device."""
+        L = lib()
+        sd, sv = (C.c_double * 3)(), (C.c_double * 3)()
+        L.mgpu_get_means.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        self._ck(L.mgpu_get_means(self.ctx, sd, sv), "get_means")
+        return np.array(sd[:]), np.array(sv[:])
+
     def fifth_force(self, mg):
         self._ck(lib().mgpu_fifth_force(self.ctx, C.byref(mg)), "fifth_force")
 
@@ -404,7 +421,8 @@ class HostParams(C.Structure):
         + [("Noutputs", C.c_int), ("out_z", C.c_double * 512), ("out_nsteps", C.c_int * 512)]
         + [("nu_include_massive_neutrinos", C.c_int), ("nu_SumMassNuEV", C.c_double),
            ("nu_FilenameTransferInfofile", C.c_char * 500), ("OmegaNu", C.c_double), ("OmegaCDM", C.c_double),
-           ("growth_on_gpu", C.c_int), ("fuse_kick_drift", C.c_int), ("write_particles", C.c_int)]
+           ("growth_on_gpu", C.c_int), ("fuse_kick_drift", C.c_int), ("write_particles", C.c_int),
+           ("device_step", C.c_int)]
     )
 
 

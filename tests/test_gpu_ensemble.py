@@ -73,3 +73,67 @@ def test_ensemble_member_matches_oracle(tmp_path, inverted):
     d = np.abs(periodic_diff(gpos, rpos, hp.Box))
     print(f"inverted {inverted}: max |dpos| / Box = {d.max() / hp.Box:.2e}", flush=True)
     assert d.max() <= 1e-4 * hp.Box
+
+
+@pytest.mark.parametrize("model", ["lcdm", "dgp"])
+def test_device_step_equals_separate_calls(model):
+    """mgpu_step (the whole step with the means on the device), eager and as a per-step hipGraph replayed with
+    new scalars each step, gives the particles of mgpu_get_displacements + mgpu_kick_drift bit for bit over five
+    steps, and its device-resident means equal the host ones."""
+    from helpers import lattice_particles
+    n, ns, box = 48, 24, 48.0
+    pos0 = lattice_particles(ns, box, amp=1.0, seed=9)
+    rng = np.random.default_rng(3)
+    vel0 = (rng.normal(size=pos0.shape) * 2.0).astype(np.float32)
+    kw = dict(model=M.DGP, mg_active=1, include_screening=1, filter=1) if model == "dgp" else {}
+    mg = M.MG(a=0.6, coupling=0.4, mass2=0.0, phi_crit=0.0, dgp_fac=1.7, rsmooth=1.0, geff=1.0) if model == "dgp" \
+        else None
+    coeffs = [(0.1 + 0.01 * s, 0.5 + 0.05 * s, 0.3, -0.2, 0.07 + 0.01 * s, 0.02, -0.01) for s in range(5)]
+    out = []
+    for mode in ("host", "device", "graph"):
+        g = M.MGPU(n, ns, box, 0.3, **kw)
+        g.set_particles(pos0, vel0)
+        sD, _ = g.get_displacements(mg=mg)
+        sV = g.kick(0.1, 0.5, 0.3, -0.2, sD)  # the velocity sums land on the device
+        g.drift(0.05, 0.01, 0.0, sV)
+        for dda, A, s1k, s2k, dyyy, s1d, s2d in coeffs:
+            if mode == "host":
+                sD, _ = g.get_displacements(mg=mg)
+                sV = g.kick_drift(dda, A, s1k, s2k, sD, dyyy, s1d, s2d, sV)
+            else:
+                g.step(dda, A, s1k, s2k, dyyy, s1d, s2d, mg=mg, graph=mode == "graph")
+        p, v = g.get_particles()
+        means = (sD, sV) if mode == "host" else g.get_means()
+        out.append((p, v, means))
+        g.close()
+    for p, v, (md, mv) in out[1:]:
+        np.testing.assert_array_equal(p, out[0][0])
+        np.testing.assert_array_equal(v, out[0][1])
+        np.testing.assert_array_equal(md, out[0][2][0])
+        np.testing.assert_array_equal(mv, out[0][2][1])
+    assert not np.array_equal(out[0][0], pos0)
+
+
+def test_device_step_runs_through_the_host(tmp_path):
+    """The C host with device_step = 1 (steps on the device) and 2 (each step one hipGraph) runs the
+    ensemble's workload to the same particles as the separate calls, bit for bit."""
+    res = []
+    for mode in (0, 1, 2):
+        hp = params(tmp_path, f"d{mode}", 32, 401, 1, nsteps=6)
+        hp.device_step = mode
+        r, _ = E.run_ensemble([hp], threads=1, keep_particles=True)
+        res.append(r[0])
+    for r in res[1:]:
+        np.testing.assert_array_equal(r["pos"], res[0]["pos"])
+        np.testing.assert_array_equal(r["vel"], res[0]["vel"])
+
+
+def test_device_step_refuses_what_it_cannot_do():
+    g = M.MGPU(32, 16, 32.0, 0.3, scale_dependent=1)
+    with pytest.raises(M.MGPUError):
+        g.step(0.1, 0.5, 0.0, 0.0, 0.05, 0.0, 0.0)
+    g.close()
+    g = M.MGPU(32, 16, 32.0, 0.3)
+    with pytest.raises(M.MGPUError):  # no kick yet: no velocity sums on the device
+        g.step(0.1, 0.5, 0.0, 0.0, 0.05, 0.0, 0.0)
+    g.close()
