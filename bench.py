@@ -357,13 +357,30 @@ def ensemble_leg(per_size=32, threads=12, nsteps=20):
     print(f"[bench] ensemble: {len(hps)} runs one after the other {t_seq:.2f} s", file=sys.stderr, flush=True)
     con, t_con = E.run_ensemble([M.HostParams.from_buffer_copy(h) for h in hps], threads=threads)
     print(f"[bench] ensemble: concurrently on {threads} threads {t_con:.2f} s", file=sys.stderr, flush=True)
+    bag, t_bag = E.run_batch([M.HostParams.from_buffer_copy(h) for h in hps], graph=True)
+    print(f"[bench] ensemble: batched (one launch per kernel per size class), one hipGraph per step {t_bag:.2f} s",
+          file=sys.stderr, flush=True)
+    bat, t_bat = E.run_batch([M.HostParams.from_buffer_copy(h) for h in hps], threads=3)
+    print(f"[bench] ensemble: batched, the size classes on 3 threads {t_bat:.2f} s", file=sys.stderr, flush=True)
+    # time loops per size class: the sum over its realizations run alone, its batch's lockstep loop
+    loops = {}
+    for c, name in enumerate(("96^3/192", "128^3/256", "192^3/384")):
+        sl = slice(c * per_size, (c + 1) * per_size)
+        loops[name] = {"sequential": round(sum(r["seconds"] for r in seq[sl]), 3),
+                       "batched_graph": round(bag[sl][0]["seconds"], 3)}
     return {"realizations": len(hps), "steps": nsteps, "threads": threads,
             "seconds_sequential": round(t_seq, 3), "seconds_concurrent": round(t_con, 3),
+            "seconds_batched_graph": round(t_bag, 3), "seconds_batched_threads3": round(t_bat, 3),
+            "time_loop_seconds_by_size_class": loops,
             "particle_steps_per_s_sequential": round(work / t_seq, 1),
-            "particle_steps_per_s_concurrent": round(work / t_con, 1), "speedup": round(t_seq / t_con, 2),
+            "particle_steps_per_s_concurrent": round(work / t_con, 1),
+            "particle_steps_per_s_batched": round(work / min(t_bat, t_bag), 1),
+            "speedup": round(t_seq / t_con, 2), "speedup_batched": round(t_seq / min(t_bat, t_bag), 2),
             "workload": f"nDGP rcH0=1 + screening, {per_size} each of 96^3/192, 128^3/256, 192^3/384 "
                         f"(Box = 2 Np), fixed-amplitude pairs, {nsteps} steps, device-resident steps; wall time "
-                        f"incl. set-up and ICs"}
+                        f"incl. set-up and ICs; batched: the realizations of one size class as one batch "
+                        f"(mgpu_batch_step), the classes one after the other with a hipGraph per step, or on 3 "
+                        f"host threads without graphs"}
 
 
 def cpu_sample(ns, nmesh, box):

@@ -205,6 +205,21 @@ int mgpu_step(mgpu_ctx *ctx, const mgpu_mg *mg, double dda, double A, double s1k
               double s1d, double s2d, int graph);
 /* The device-resident means: Disp of the last gather, velocities of the last kick. */
 int mgpu_get_means(mgpu_ctx *ctx, double sumD[3], double sumV[3]);
+/* A batch of count realizations of one run configuration (the paired,
+ * fixed-amplitude ICs of an ensemble: seeds and inversion differ, the mesh, the
+ * lattice, the box, the model and the time steps do not).  members[0..count-1]
+ * are full contexts, each usable with every call of this header and destroyed
+ * with mgpu_destroy; they share one HIP stream and one device allocation, in
+ * which member r's buffers lie at a fixed stride from member 0's.  Each member
+ * runs the main loop of src/main.c:394-611 as a solo run would. */
+int mgpu_batch_create(const mgpu_params *p, int count, mgpu_ctx **members);
+/* mgpu_step for every member of a batch at once (members in the order of
+ * mgpu_batch_create): each kernel of the step is one launch over all members
+ * (grid z = member), so small meshes fill the device.  Every member must meet
+ * mgpu_step's conditions; the result of each member is bit-identical to
+ * mgpu_step on it alone. */
+int mgpu_batch_step(mgpu_ctx *const *members, int count, const mgpu_mg *mg, double dda, double A, double s1k,
+                    double s2k, double dyyy, double s1d, double s2d, int graph);
 
 /* ---- initial conditions ---- */
 /* displacement_fields (Gaussian): seed table + per-column RANLUX stream,

@@ -12,7 +12,27 @@ thread_local std::string g_err;
 
 int check_ctx(mgpu_ctx *ctx) { return ctx ? MGPU_OK : MGPU_E_INVALID; }
 
+// one hold on a batch allocation dropped; the last frees it and the shared stream
+void arena_release(mgpu_arena *a) {
+  if (--a->refs > 0) return;
+  if (a->base) (void)hipFree(a->base);
+  if (a->stream) (void)hipStreamDestroy(a->stream);
+  delete a;
+}
+
 int alloc(mgpu_ctx *ctx, void **p, size_t bytes, const char *what) {
+  const size_t aligned = (bytes + 255) / 256 * 256;
+  if (ctx->arena_cur) {
+    // a batch member: carved from its share of the batch allocation
+    if (aligned > (size_t)(ctx->arena_end - ctx->arena_cur)) {
+      *p = nullptr;
+      return mgpu_fail(ctx, MGPU_E_NOMEM, std::string("batch share exhausted at ") + what);
+    }
+    *p = ctx->arena_cur;
+    ctx->arena_cur += aligned;
+    return MGPU_OK;
+  }
+  if (ctx->arena_measure) *ctx->arena_measure += aligned;
   if (hipMalloc(p, bytes) != hipSuccess) {
     (void)hipGetLastError();
     *p = nullptr;
@@ -177,7 +197,17 @@ int mgpu_memory_plan(const mgpu_params *p, size_t *parts, size_t *total) {
   return MGPU_OK;
 }
 
-int mgpu_create(const mgpu_params *p, mgpu_ctx **out) {
+namespace {
+// arena null: a context of its own (memory plan checked, own stream); else a batch
+// member whose create-time buffers are carved from share (the arena's stream);
+// measure not null: the aligned bytes of those buffers are added to *measure
+int create_ctx(const mgpu_params *p, mgpu_ctx **out, mgpu_arena *arena, char *share, size_t *measure);
+}  // namespace
+
+int mgpu_create(const mgpu_params *p, mgpu_ctx **out) { return create_ctx(p, out, nullptr, nullptr, nullptr); }
+
+namespace {
+int create_ctx(const mgpu_params *p, mgpu_ctx **out, mgpu_arena *arena, char *share, size_t *measure) {
   if (!p || !out) return mgpu_fail(nullptr, MGPU_E_INVALID, "null argument");
   *out = nullptr;
   if (!fft_size_supported(p->nmesh))
@@ -190,7 +220,7 @@ int mgpu_create(const mgpu_params *p, mgpu_ctx **out) {
     (void)hipGetLastError();
     return mgpu_fail(nullptr, MGPU_E_NODEVICE, "no HIP device visible");
   }
-  {
+  if (!arena) {
     // the plan against the free device memory, before anything is allocated
     size_t part[MGPU_MEM_PARTS], need = 0, free_b = 0, total_b = 0;
     mgpu_memory_plan(p, part, &need);
@@ -219,11 +249,20 @@ int mgpu_create(const mgpu_params *p, mgpu_ctx **out) {
     mgpu_destroy(ctx);
     return code;
   };
+  if (arena) {
+    ctx->arena = arena;
+    ctx->arena_cur = share;
+    ctx->arena_end = share + arena->stride;
+    ctx->zs = arena->stride;
+    ctx->stream = arena->stream;
+    arena->refs++;
+  }
+  ctx->arena_measure = measure;
   if (hipGetDevice(&ctx->device) != hipSuccess) return fail(mgpu_fail(ctx, MGPU_E_HIP, "hipGetDevice failed"));
   if ((rc = ic_scratch_init(ctx))) return fail(rc);
   hipDeviceProp_t pr;
   if (hipGetDeviceProperties(&pr, ctx->device) == hipSuccess) ctx->lds_max = pr.sharedMemPerBlock;
-  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+  if (!ctx->stream && hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(mgpu_fail(ctx, MGPU_E_HIP, "stream creation failed"));
   if ((rc = fft_plan_init(ctx, &ctx->fft, ctx->N, MGPU_FFT_TCOLS))) return fail(rc);
   ctx->HP = ctx->fft.hp;
@@ -311,13 +350,19 @@ int mgpu_create(const mgpu_params *p, mgpu_ctx **out) {
                                            ctx->stream) != hipSuccess))
     return fail(mgpu_fail(ctx, MGPU_E_HIP, "memset failed"));
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(mgpu_fail(ctx, MGPU_E_HIP, "sync failed"));
+  // later allocations (RSD records, learned tables, slices) are the member's own
+  ctx->arena_cur = ctx->arena_end = nullptr;
+  ctx->arena_measure = nullptr;
   *out = ctx;
   return MGPU_OK;
 }
+}  // namespace
 
 void mgpu_destroy(mgpu_ctx *ctx) {
   if (!ctx) return;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  mgpu_arena *arena = ctx->arena;
+  const char *a0 = arena ? arena->base : nullptr, *a1 = arena ? arena->base + arena->stride * arena->count : nullptr;
   flush_ktimers(ctx);
   if (ctx->step_exec) (void)hipGraphExecDestroy(ctx->step_exec);
   void *ptrs[] = {ctx->pos,      ctx->vel,       ctx->disp,      ctx->lpt[0],    ctx->lpt[1],
@@ -327,14 +372,19 @@ void mgpu_destroy(mgpu_ctx *ctx) {
                   ctx->dep_scratch, ctx->winv,    ctx->winvf,     ctx->pk_bin,    ctx->rsd_crd,
                   ctx->nu_slice, ctx->nu_fac,  ctx->feat_part, ctx->feat,     ctx->green_mult};
   for (void *q : ptrs)
-    if (q) (void)hipFree(q);
+    if (q && !((const char *)q >= a0 && (const char *)q < a1)) (void)hipFree(q);
   if (ctx->slice_host) (void)hipHostFree(ctx->slice_host);
   if (ctx->slice_ev) (void)hipEventDestroy(ctx->slice_ev);
   fft_plan_free(&ctx->fft);
   fft_plan_free(&ctx->fft_lattice);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
-  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (arena) {
+    // the last member frees the batch allocation and the shared stream
+    arena_release(arena);
+  } else if (ctx->stream) {
+    (void)hipStreamDestroy(ctx->stream);
+  }
   delete ctx;
 }
 
@@ -638,9 +688,9 @@ static int device_step_body(mgpu_ctx *ctx, const mgpu_mg *mg, double dda, double
   return pm_kick_drift(ctx, dda, A, s1k, s2k, nullptr, dyyy, s1d, s2d, nullptr);
 }
 
-int mgpu_step(mgpu_ctx *ctx, const mgpu_mg *mg, double dda, double A, double s1k, double s2k, double dyyy,
-              double s1d, double s2d, int graph) {
-  if (check_ctx(ctx)) return MGPU_E_INVALID;
+namespace {
+// what mgpu_step needs of a context (MGPU_OK or the refusal)
+int step_check(mgpu_ctx *ctx, const mgpu_mg *mg) {
   if (ctx->p.mg_active && ctx->p.model != MGPU_LCDM && !mg)
     return mgpu_fail(ctx, MGPU_E_INVALID, "modified gravity active but no mgpu_mg coefficients given");
   if (ctx->p.scale_dependent)
@@ -653,6 +703,13 @@ int mgpu_step(mgpu_ctx *ctx, const mgpu_mg *mg, double dda, double A, double s1k
   if (!ctx->vsum_valid)
     return mgpu_fail(ctx, MGPU_E_STATE, "no velocity sums on the device: the first step takes mgpu_kick or "
                                         "mgpu_kick_drift");
+  return MGPU_OK;
+}
+
+// the step body on ctx's stream, eagerly or (graph, after one eager step) captured
+// into ctx's executable graph, updated in place, and launched once
+int run_step(mgpu_ctx *ctx, const mgpu_mg *mg, double dda, double A, double s1k, double s2k, double dyyy,
+             double s1d, double s2d, int graph) {
   int rc;
   if (graph && ctx->step_warm) {
     hipGraph_t g = nullptr;
@@ -687,6 +744,123 @@ int mgpu_step(mgpu_ctx *ctx, const mgpu_mg *mg, double dda, double A, double s1k
   }
   ctx->have_density = ctx->have_forces = true;
   ctx->have_disp = false;  // Disp was not reduced by the mean
+  return MGPU_OK;
+}
+}  // namespace
+
+int mgpu_step(mgpu_ctx *ctx, const mgpu_mg *mg, double dda, double A, double s1k, double s2k, double dyyy,
+              double s1d, double s2d, int graph) {
+  if (check_ctx(ctx)) return MGPU_E_INVALID;
+  int rc = step_check(ctx, mg);
+  if (rc) return rc;
+  return run_step(ctx, mg, dda, A, s1k, s2k, dyyy, s1d, s2d, graph);
+}
+
+int mgpu_batch_create(const mgpu_params *p, int count, mgpu_ctx **members) {
+  if (!p || !members || count < 1) return mgpu_fail(nullptr, MGPU_E_INVALID, "bad batch arguments");
+  for (int r = 0; r < count; r++) members[r] = nullptr;
+  // one context made and destroyed to measure a member's share
+  size_t share = 0;
+  {
+    mgpu_ctx *probe = nullptr;
+    int rc = create_ctx(p, &probe, nullptr, nullptr, &share);
+    if (rc) return rc;
+    mgpu_destroy(probe);
+  }
+  share = (share + 255) / 256 * 256;
+  if (share > SIZE_MAX / (size_t)count) return mgpu_fail(nullptr, MGPU_E_NOMEM, "batch too large");
+  const size_t total = share * (size_t)count;
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total > free_b)
+    return mgpu_fail(nullptr, MGPU_E_NOMEM, "the batch needs " + std::to_string(total) + " bytes of device memory but " +
+                                                std::to_string(free_b) + " are free");
+  (void)hipGetLastError();
+  mgpu_arena *arena = new mgpu_arena();
+  arena->stride = share;
+  arena->count = count;
+  arena->refs = 1;  // held by this call until the members exist
+  if (hipMalloc((void **)&arena->base, total) != hipSuccess) {
+    (void)hipGetLastError();
+    arena->base = nullptr;
+    arena_release(arena);
+    return mgpu_fail(nullptr, MGPU_E_NOMEM, "batch allocation failed (" + std::to_string(total) + " bytes)");
+  }
+  if (hipStreamCreateWithFlags(&arena->stream, hipStreamNonBlocking) != hipSuccess) {
+    (void)hipGetLastError();
+    arena->stream = nullptr;
+    arena_release(arena);
+    return mgpu_fail(nullptr, MGPU_E_HIP, "stream creation failed");
+  }
+  int rc = MGPU_OK;
+  for (int r = 0; r < count && !rc; r++) rc = create_ctx(p, &members[r], arena, arena->base + share * r, nullptr);
+  if (!rc) {
+    // every member's buffers at its share's offset: realization r = realization 0 + r * share
+    mgpu_ctx *m0 = members[0];
+    for (int r = 1; r < count && !rc; r++) {
+      const mgpu_ctx *m = members[r];
+      const void *a[] = {m->pos, m->vel, m->disp, m->lpt[0], m->lpt[1], m->slot[0], m->slot[1], m->slot[2], m->slot[3],
+                         m->partial, m->dsum, m->dep_scratch};
+      const void *b[] = {m0->pos, m0->vel, m0->disp, m0->lpt[0], m0->lpt[1], m0->slot[0], m0->slot[1], m0->slot[2],
+                         m0->slot[3], m0->partial, m0->dsum, m0->dep_scratch};
+      for (size_t q = 0; q < sizeof a / sizeof a[0]; q++)
+        if ((const char *)a[q] != (const char *)b[q] + share * r)
+          rc = mgpu_fail(nullptr, MGPU_E_HIP, "batch members' buffers are not at a fixed stride");
+      if (m->fft.tcols != m0->fft.tcols) rc = mgpu_fail(nullptr, MGPU_E_HIP, "batch members' FFT tiles differ");
+    }
+  }
+  if (rc) {
+    const std::string e = g_err;
+    for (int r = 0; r < count; r++) {
+      mgpu_destroy(members[r]);
+      members[r] = nullptr;
+    }
+    arena_release(arena);
+    g_err = e;
+    return rc;
+  }
+  arena_release(arena);
+  return MGPU_OK;
+}
+
+int mgpu_batch_step(mgpu_ctx *const *members, int count, const mgpu_mg *mg, double dda, double A, double s1k,
+                    double s2k, double dyyy, double s1d, double s2d, int graph) {
+  if (!members || count < 1 || !members[0]) return mgpu_fail(nullptr, MGPU_E_INVALID, "bad batch arguments");
+  mgpu_ctx *m0 = members[0];
+  mgpu_arena *arena = m0->arena;
+  if (!arena || arena->count != count)
+    return mgpu_fail(m0, MGPU_E_INVALID, "the members must be all the contexts of one mgpu_batch_create, in order");
+  int rc;
+  for (int r = 0; r < count; r++) {
+    mgpu_ctx *m = members[r];
+    if (!m || m->arena != arena || m->pos != (float *)((char *)m0->pos + arena->stride * r))
+      return mgpu_fail(m0, MGPU_E_INVALID, "the members must be all the contexts of one mgpu_batch_create, in order");
+    if ((rc = step_check(m, mg))) return mgpu_fail(m0, rc, "member " + std::to_string(r) + ": " + m->err);
+    if (m->fft.tcols != m0->fft.tcols) return mgpu_fail(m0, MGPU_E_STATE, "the members' FFT tile widths differ");
+  }
+  // realization 0's launches with grid z = count reach every member
+  m0->bz = (unsigned)count;
+  rc = run_step(m0, mg, dda, A, s1k, s2k, dyyy, s1d, s2d, graph);
+  m0->bz = 1;
+  if (rc) return rc;
+  const size_t rec_off = (const char *)m0->dep_rec - (const char *)m0->dep_scratch;
+  const size_t off_off = (const char *)m0->dep_off - (const char *)m0->dep_scratch;
+  for (int r = 1; r < count; r++) {
+    // the host state the step left in realization 0, for each member
+    mgpu_ctx *m = members[r];
+    m->have_deposit = m0->have_deposit;
+    m->have_density = m0->have_density;
+    m->have_forces = m0->have_forces;
+    m->have_disp = m0->have_disp;
+    m->have_phik = m0->have_phik;
+    m->dx_slot = m0->dx_slot;
+    m->phi_slot = m0->phi_slot;
+    for (int a = 0; a < 3; a++) m->force_slot[a] = m0->force_slot[a];
+    m->dep_rec = (float4 *)((char *)m->dep_scratch + rec_off);
+    m->dep_off = (unsigned *)((char *)m->dep_scratch + off_off);
+    m->sorted_valid = m0->sorted_valid;
+    m->vsum_valid = m0->vsum_valid;
+    m->dep_tiles_grid = m0->dep_tiles_grid;
+  }
   return MGPU_OK;
 }
 

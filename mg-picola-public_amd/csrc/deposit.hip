@@ -102,7 +102,11 @@ constexpr int kBinU = 4;
 template <bool RSD>
 __global__ __launch_bounds__(kTB) void k_dep_bin(const float *__restrict__ pos, const float4 *__restrict__ crd,
                                                  size_t np, TileGeom g, double scale, unsigned *__restrict__ count,
-                                                 unsigned *__restrict__ rank) {
+                                                 unsigned *__restrict__ rank, size_t zs) {
+  pos = zbuf(pos, zs);
+  crd = zbuf(crd, zs);
+  count = zbuf(count, zs);
+  rank = zbuf(rank, zs);
   const size_t i0 = (size_t)blockIdx.x * (kBinU * kTB) + threadIdx.x;
   const int lane = threadIdx.x % warpSize;
   unsigned t[kBinU], off_in[kBinU], gsize[kBinU], base[kBinU];
@@ -146,13 +150,18 @@ __global__ __launch_bounds__(kTB) void k_dep_bin(const float *__restrict__ pos, 
 }
 
 // the bin counters cleared (grid-stride)
-__global__ __launch_bounds__(256) void k_zero_u32(unsigned *__restrict__ a, size_t n) {
+__global__ __launch_bounds__(256) void k_zero_u32(unsigned *__restrict__ a, size_t n, size_t zs) {
+  a = zbuf(a, zs);
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] = 0u;
 }
 
 // 2. exclusive prefix sum of the tile counts (three launches)
 __global__ __launch_bounds__(1024) void k_scan_block(const unsigned *__restrict__ in, size_t n,
-                                                     unsigned *__restrict__ out, unsigned *__restrict__ sums) {
+                                                     unsigned *__restrict__ out, unsigned *__restrict__ sums,
+                                                     size_t zs) {
+  in = zbuf(in, zs);
+  out = zbuf(out, zs);
+  sums = zbuf(sums, zs);
   __shared__ unsigned s[1024];
   const size_t i = (size_t)blockIdx.x * 1024 + threadIdx.x;
   const unsigned v = i < n ? in[i] : 0u;
@@ -167,7 +176,8 @@ __global__ __launch_bounds__(1024) void k_scan_block(const unsigned *__restrict_
   if (i < n) out[i] = s[threadIdx.x] - v;
   if (threadIdx.x == 1023) sums[blockIdx.x] = s[1023];
 }
-__global__ __launch_bounds__(1024) void k_scan_sums(unsigned *__restrict__ sums, int nb) {
+__global__ __launch_bounds__(1024) void k_scan_sums(unsigned *__restrict__ sums, int nb, size_t zs) {
+  sums = zbuf(sums, zs);
   // nb <= 1024 * 1024 handled by a serial carry over 1024-chunks
   __shared__ unsigned s[1024];
   unsigned carry = 0;
@@ -190,7 +200,9 @@ __global__ __launch_bounds__(1024) void k_scan_sums(unsigned *__restrict__ sums,
 }
 __global__ __launch_bounds__(1024) void k_scan_add(unsigned *__restrict__ out, size_t n,
                                                    const unsigned *__restrict__ sums, unsigned total_tail,
-                                                   size_t ntail) {
+                                                   size_t ntail, size_t zs) {
+  out = zbuf(out, zs);
+  sums = zbuf(sums, zs);
   const size_t i = (size_t)blockIdx.x * 1024 + threadIdx.x;
   if (i < n) out[i] += sums[blockIdx.x];
   if (i == 0) out[ntail] = total_tail;
@@ -203,7 +215,12 @@ template <bool RSD>
 __global__ __launch_bounds__(kTB) void k_dep_scatter(const unsigned *__restrict__ rank,
                                                      const unsigned *__restrict__ off, const float *__restrict__ pos,
                                                      const float4 *__restrict__ crd, size_t np, TileGeom g,
-                                                     double scale, float4 *__restrict__ rec) {
+                                                     double scale, float4 *__restrict__ rec, size_t zs) {
+  rank = zbuf(rank, zs);
+  off = zbuf(off, zs);
+  pos = zbuf(pos, zs);
+  crd = zbuf(crd, zs);
+  rec = zbuf(rec, zs);
   const size_t i0 = (size_t)blockIdx.x * (kBinU * kTB) + threadIdx.x;
   unsigned o[kBinU];
 #pragma unroll
@@ -333,7 +350,10 @@ __device__ __forceinline__ bool tile_of_block(unsigned nt, unsigned k, unsigned 
 template <bool RSD, int T>
 __global__ __launch_bounds__(kDepTileNT) void k_dep_tiles(const float4 *__restrict__ rec,
                                                           const unsigned *__restrict__ off, TileGeom g, double scale,
-                                                          double wpar, int HP2, float *__restrict__ d) {
+                                                          double wpar, int HP2, float *__restrict__ d, size_t zs) {
+  rec = zbuf(rec, zs);
+  off = zbuf(off, zs);
+  d = zbuf(d, zs);
   // The loop is rotated so that each tile's first records are loaded and
   // consumed inside one iteration with a fixed sequence of memory operations
   // between (the next tile's offsets, then this readout's stores): the wait
@@ -415,7 +435,13 @@ __global__ __launch_bounds__(kGatherNT) void k_gather_tile(const float4 *__restr
                                                      const unsigned *__restrict__ off, int N, int nt1, int HP2,
                                                      double scale, const float *__restrict__ g0,
                                                      const float *__restrict__ g1, const float *__restrict__ g2,
-                                                     float *__restrict__ disp) {
+                                                     float *__restrict__ disp, size_t zs) {
+  rec = zbuf(rec, zs);
+  off = zbuf(off, zs);
+  g0 = zbuf(g0, zs);
+  g1 = zbuf(g1, zs);
+  g2 = zbuf(g2, zs);
+  disp = zbuf(disp, zs);
   constexpr int B = T + 1, B2 = B * B, B3 = B2 * B;
   __shared__ float f[3][B3];
   const unsigned nt = (unsigned)nt1 * nt1 * nt1;
@@ -544,22 +570,24 @@ int deposit_tiled(mgpu_ctx *ctx, float *d, const float4 *rsd) {
   const double scale = (double)N / ctx->p.box;
   const double wpar = std::pow((double)N / (double)ctx->Np, 3);
   // cleared by a kernel, not hipMemsetAsync, so that the step can be captured in a hipGraph
-  hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)std::min<size_t>(cdiv(nb + 1, 1024), 65535)), dim3(256), 0,
-                     ctx->stream, count, nb + 1);
+  const unsigned bz = ctx->bz;
+  const size_t zs = ctx->zs;
+  hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)std::min<size_t>(cdiv(nb + 1, 1024), 65535), 1, bz), dim3(256), 0,
+                     ctx->stream, count, nb + 1, zs);
   if (np) {
     KScope ks(ctx, MGPU_K_DEP_SORT, 16.0 * np);
-    hipLaunchKernelGGL(rsd ? k_dep_bin<true> : k_dep_bin<false>, dim3(cdiv(np, kBinU * kTB)), dim3(kTB), 0,
-                       ctx->stream, ctx->pos, rsd, np, g, scale, count, rank);
+    hipLaunchKernelGGL(rsd ? k_dep_bin<true> : k_dep_bin<false>, dim3(cdiv(np, kBinU * kTB), 1, bz), dim3(kTB), 0,
+                       ctx->stream, ctx->pos, rsd, np, g, scale, count, rank, zs);
   }
   MGPU_HIP(ctx, hipGetLastError());
-  hipLaunchKernelGGL(k_scan_block, dim3(nsum), dim3(1024), 0, ctx->stream, count, nb, off, sums);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, ctx->stream, sums, (int)nsum);
-  hipLaunchKernelGGL(k_scan_add, dim3(nsum), dim3(1024), 0, ctx->stream, off, nb, sums, (unsigned)np, nb);
+  hipLaunchKernelGGL(k_scan_block, dim3(nsum, 1, bz), dim3(1024), 0, ctx->stream, count, nb, off, sums, zs);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1, 1, bz), dim3(1024), 0, ctx->stream, sums, (int)nsum, zs);
+  hipLaunchKernelGGL(k_scan_add, dim3(nsum, 1, bz), dim3(1024), 0, ctx->stream, off, nb, sums, (unsigned)np, nb, zs);
   MGPU_HIP(ctx, hipGetLastError());
   if (np) {
     KScope ks(ctx, MGPU_K_DEP_SORT, 32.0 * np);
-    hipLaunchKernelGGL(rsd ? k_dep_scatter<true> : k_dep_scatter<false>, dim3(cdiv(np, kBinU * kTB)), dim3(kTB), 0,
-                       ctx->stream, rank, off, ctx->pos, rsd, np, g, scale, rec);
+    hipLaunchKernelGGL(rsd ? k_dep_scatter<true> : k_dep_scatter<false>, dim3(cdiv(np, kBinU * kTB), 1, bz),
+                       dim3(kTB), 0, ctx->stream, rank, off, ctx->pos, rsd, np, g, scale, rec, zs);
   }
   MGPU_HIP(ctx, hipGetLastError());
   const size_t shmem = sizeof(unsigned long long) * g.tx * g.ty * g.tz;
@@ -577,9 +605,12 @@ int deposit_tiled(mgpu_ctx *ctx, float *d, const float4 *rsd) {
       MGPU_HIP(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
       ctx->dep_tiles_grid = (unsigned)std::max(1, per_cu * cus);
     }
-    unsigned grid = (unsigned)std::min<size_t>(ctx->dep_tiles_grid, nt);
+    // a batched launch shares the resident workgroups among its realizations (the integer
+    // tile sums do not depend on which workgroup adds a tile)
+    unsigned grid = (unsigned)std::min<size_t>(std::max<unsigned>(8u, ctx->dep_tiles_grid / bz), nt);
     if (grid >= 8) grid -= grid % 8;
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kDepTileNT), shmem, ctx->stream, rec, off, g, scale, wpar, 2 * ctx->HP, d);
+    hipLaunchKernelGGL(kern, dim3(grid, 1, bz), dim3(kDepTileNT), shmem, ctx->stream, rec, off, g, scale, wpar,
+                       2 * ctx->HP, d, zs);
   }
   MGPU_HIP(ctx, hipGetLastError());
   return MGPU_OK;
@@ -594,8 +625,8 @@ int gather_tiled(mgpu_ctx *ctx, const float *f0, const float *f1, const float *f
   const unsigned nt = (unsigned)nt1 * nt1 * nt1;
   const double scale = (double)N / ctx->p.box;
   auto kern = T == 16 ? k_gather_tile<16> : T == 8 ? k_gather_tile<8> : k_gather_tile<4>;
-  hipLaunchKernelGGL(kern, dim3(nt), dim3(kGatherNT), 0, ctx->stream, ctx->dep_rec, ctx->np, ctx->dep_off, N, nt1,
-                     2 * ctx->HP, scale, f0, f1, f2, ctx->disp);
+  hipLaunchKernelGGL(kern, dim3(nt, 1, ctx->bz), dim3(kGatherNT), 0, ctx->stream, ctx->dep_rec, ctx->np, ctx->dep_off,
+                     N, nt1, 2 * ctx->HP, scale, f0, f1, f2, ctx->disp, ctx->zs);
   MGPU_HIP(ctx, hipGetLastError());
   return MGPU_OK;
 }

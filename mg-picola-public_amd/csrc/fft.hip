@@ -32,7 +32,8 @@ using namespace fftc;
 template <int L, int T, int SIGN, int TPB>
 __global__ __launch_bounds__(col_threads(L, T)) void k_fft_col(float2 *__restrict__ buf, size_t line_stride,
                                                                  size_t outer_stride,
-                                                                 const float2 *__restrict__ tw) {
+                                                                 const float2 *__restrict__ tw, size_t zs) {
+  buf = zbuf(buf, zs);
   extern __shared__ float2 lds[];
   constexpr int NT = col_threads(L, T);
   constexpr int E = col_ept(L, T);
@@ -77,7 +78,9 @@ __global__ __launch_bounds__(col_threads(L, T)) void k_fft_col(float2 *__restric
 // Entries N/2 < k < HP of the row are padding and are left untouched.
 template <int N>
 __global__ __launch_bounds__(RowCfg<N>::NT) void k_fft_z_r2c(const float2 *in, float2 *buf, int HP, size_t nrows,
-                                                              const float2 *__restrict__ tw) {
+                                                              const float2 *__restrict__ tw, size_t zs) {
+  in = zbuf(in, zs);
+  buf = zbuf(buf, zs);
   using C = RowCfg<N>;
   extern __shared__ float2 lds[];
   const int tid = threadIdx.x;
@@ -102,7 +105,10 @@ __global__ __launch_bounds__(RowCfg<N>::NT) void k_fft_z_r2c(const float2 *in, f
 // (the z force component, k the kz index).
 template <int N, bool IKZ>
 __global__ __launch_bounds__(RowCfg<N>::NT) void k_fft_z_c2r(const float2 *buf, int HP, size_t nrows,
-                                                              const float2 *__restrict__ tw, float2 *out, int orow) {
+                                                              const float2 *__restrict__ tw, float2 *out, int orow,
+                                                              size_t zs) {
+  buf = zbuf(buf, zs);
+  out = zbuf(out, zs);
   using C = RowCfg<N>;
   extern __shared__ float2 lds[];
   const int tid = threadIdx.x;
@@ -132,8 +138,8 @@ int launch_col(mgpu_ctx *ctx, float2 *buf, size_t line_stride, size_t outer_stri
   auto kern = k_fft_col<L, T, SIGN, TPB>;
   static_assert(L % TPB == 0, "outer count must be a multiple of the tiles per block");
   if (lds > 65536) MGPU_HIP(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  dim3 grid(ntiles, nouter / TPB);
-  hipLaunchKernelGGL(kern, grid, dim3(col_threads(L, T)), lds, ctx->stream, buf, line_stride, outer_stride, tw);
+  dim3 grid(ntiles, nouter / TPB, ctx->bz);
+  hipLaunchKernelGGL(kern, grid, dim3(col_threads(L, T)), lds, ctx->stream, buf, line_stride, outer_stride, tw, ctx->zs);
   MGPU_HIP(ctx, hipGetLastError());
   return MGPU_OK;
 }
@@ -144,16 +150,17 @@ int launch_z(mgpu_ctx *ctx, float2 *buf, int HP, const float2 *tw, int forward, 
   using C = RowCfg<N>;
   size_t nrows = (size_t)N * N;
   size_t lds = (size_t)C::RB * C::LS * sizeof(float2);
-  dim3 grid(cdiv(nrows, C::RB));
+  dim3 grid(cdiv(nrows, C::RB), 1, ctx->bz);
   if (forward) {
     auto kern = k_fft_z_r2c<N>;
     if (lds > 65536) MGPU_HIP(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(kern, grid, dim3(C::NT), lds, ctx->stream, in ? in : (const float2 *)buf, buf, HP, nrows, tw);
+    hipLaunchKernelGGL(kern, grid, dim3(C::NT), lds, ctx->stream, in ? in : (const float2 *)buf, buf, HP, nrows, tw,
+                       ctx->zs);
   } else {
     auto kern = k_fft_z_c2r<N, false>;
     if (lds > 65536) MGPU_HIP(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(kern, grid, dim3(C::NT), lds, ctx->stream, (const float2 *)buf, HP, nrows, tw,
-                       out ? out : buf, out ? N / 2 : HP);
+                       out ? out : buf, out ? N / 2 : HP, ctx->zs);
   }
   MGPU_HIP(ctx, hipGetLastError());
   return MGPU_OK;
@@ -294,8 +301,8 @@ int fft_pass_z_ikz(mgpu_ctx *ctx, const FFTPlan *plan, float2 *buf) {
     auto kern = k_fft_z_c2r<S, true>;                                                                           \
     if (lds > 65536)                                                                                            \
       MGPU_HIP(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
-    hipLaunchKernelGGL(kern, dim3(cdiv(nrows, C::RB)), dim3(C::NT), lds, ctx->stream, (const float2 *)buf,       \
-                       plan->hp, nrows, plan->tw, buf, plan->hp);                                               \
+    hipLaunchKernelGGL(kern, dim3(cdiv(nrows, C::RB), 1, ctx->bz), dim3(C::NT), lds, ctx->stream,              \
+                       (const float2 *)buf, plan->hp, nrows, plan->tw, buf, plan->hp, ctx->zs);                 \
     MGPU_HIP(ctx, hipGetLastError());                                                                           \
     return MGPU_OK;                                                                                             \
   }

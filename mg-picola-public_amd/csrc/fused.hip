@@ -126,7 +126,9 @@ __global__ __launch_bounds__(col_threads(L, T)) void k_col_green(const float2 *_
 // Tile (bx, by) of xcd_tile: columns kz, outer index kx = by, line index ky.
 template <int L, int T>
 __global__ __launch_bounds__(col_threads(L, T)) void k_col_dual(float2 *buf, float2 *__restrict__ oy, size_t ls,
-                                                                  size_t os, const float2 *__restrict__ tw) {
+                                                                  size_t os, const float2 *__restrict__ tw, size_t zs) {
+  buf = zbuf(buf, zs);
+  oy = zbuf(oy, zs);
   extern __shared__ float2 lds[];
   constexpr int NT = col_threads(L, T), E = col_ept(L, T);
   constexpr int R0 = pick_radix(L, E), NB0 = E / R0, M0 = L / R0;
@@ -241,7 +243,13 @@ template <int L, int T, int FWD, int MODE>
 __global__ __launch_bounds__(col_threads(L, T)) void k_col_fused(
     const float2 *in, float2 *kout, const float2 *__restrict__ dk, float2 *out0, float2 *out1, size_t ls, size_t os,
     const float2 *__restrict__ tw, GreenArgs ga, float coupling, float m2, const float *__restrict__ winv,
-    const int16_t *__restrict__ pbin, int nbins, double *__restrict__ pk) {
+    const int16_t *__restrict__ pbin, int nbins, double *__restrict__ pk, size_t zs) {
+  in = zbuf(in, zs);
+  kout = zbuf(kout, zs);
+  dk = zbuf(dk, zs);
+  out0 = zbuf(out0, zs);
+  out1 = zbuf(out1, zs);
+  pk = zbuf(pk, zs);
   extern __shared__ float2 lds[];
   constexpr int NT = col_threads(L, T), E = col_ept(L, T);
   constexpr int R0 = pick_radix(L, E), NB0 = E / R0, M0 = L / R0;
@@ -343,7 +351,9 @@ enum { SCREEN_POTENTIAL = 0, SCREEN_DENSITY = 1 };
 template <int N, int KIND>
 __global__ __launch_bounds__(RowCfg<N>::NT) void k_z_screen(float2 *buf, const float2 *__restrict__ dx, int HP,
                                                              size_t nrows, const float2 *__restrict__ tw, float p0,
-                                                             float p1) {
+                                                             float p1, size_t zs) {
+  buf = zbuf(buf, zs);
+  dx = zbuf(dx, zs);
   using C = RowCfg<N>;
   constexpr int ITER = C::RB * C::M / C::NT;
   static_assert(C::RB * C::M % C::NT == 0, "row block not a multiple of the block size");
@@ -438,8 +448,9 @@ int fused_x_launch_k(mgpu_ctx *ctx, const FusedXArgs &a) {
   int rc;
   auto kern = k_col_fused<N, T, FWD, MODE>;
   if ((rc = set_lds(ctx, kern, lds))) return rc;
-  hipLaunchKernelGGL(kern, dim3(HP / T, N), dim3(col_threads(N, T)), lds, ctx->stream, a.in, a.kout, a.dk, a.o0, a.o1,
-                     ls, os, ctx->fft.tw, a.ga, a.coupling, a.m2, ctx->winvf, ctx->pk_bin, nbins, a.pk);
+  hipLaunchKernelGGL(kern, dim3(HP / T, N, ctx->bz), dim3(col_threads(N, T)), lds, ctx->stream, a.in, a.kout, a.dk,
+                     a.o0, a.o1, ls, os, ctx->fft.tw, a.ga, a.coupling, a.m2, ctx->winvf, ctx->pk_bin, nbins, a.pk,
+                     ctx->zs);
   MGPU_HIP(ctx, hipGetLastError());
   return MGPU_OK;
 }
@@ -463,8 +474,8 @@ int dual_launch(mgpu_ctx *ctx, float2 *buf, float2 *oy) {
   const size_t lds = (size_t)N * T * sizeof(float2);
   int rc;
   if ((rc = set_lds(ctx, k_col_dual<N, T>, lds))) return rc;
-  hipLaunchKernelGGL((k_col_dual<N, T>), dim3(HP / T, N), dim3(col_threads(N, T)), lds, ctx->stream, buf, oy, ls, os,
-                     ctx->fft.tw);
+  hipLaunchKernelGGL((k_col_dual<N, T>), dim3(HP / T, N, ctx->bz), dim3(col_threads(N, T)), lds, ctx->stream, buf, oy,
+                     ls, os, ctx->fft.tw, ctx->zs);
   MGPU_HIP(ctx, hipGetLastError());
   return MGPU_OK;
 }
@@ -501,16 +512,16 @@ int screen_launch(mgpu_ctx *ctx, int kind, float2 *buf, const float2 *dx, float 
   using C = RowCfg<N>;
   const size_t nrows = (size_t)N * N;
   const size_t lds = (size_t)C::RB * C::LS * sizeof(float2);
-  dim3 grid(cdiv(nrows, C::RB)), block(C::NT);
+  dim3 grid(cdiv(nrows, C::RB), 1, ctx->bz), block(C::NT);
   int rc;
   if (kind == SCREEN_POTENTIAL) {
     if ((rc = set_lds(ctx, k_z_screen<N, SCREEN_POTENTIAL>, lds))) return rc;
     hipLaunchKernelGGL((k_z_screen<N, SCREEN_POTENTIAL>), grid, block, lds, ctx->stream, buf, dx, (int)ctx->fft.hp,
-                       nrows, ctx->fft.tw, p0, p1);
+                       nrows, ctx->fft.tw, p0, p1, ctx->zs);
   } else {
     if ((rc = set_lds(ctx, k_z_screen<N, SCREEN_DENSITY>, lds))) return rc;
     hipLaunchKernelGGL((k_z_screen<N, SCREEN_DENSITY>), grid, block, lds, ctx->stream, buf, dx, (int)ctx->fft.hp,
-                       nrows, ctx->fft.tw, p0, p1);
+                       nrows, ctx->fft.tw, p0, p1, ctx->zs);
   }
   MGPU_HIP(ctx, hipGetLastError());
   return MGPU_OK;

@@ -36,6 +36,24 @@ struct FFTPlan {
   float2 *tw = nullptr;  // W_N^m = exp(-2 pi i m / N), m < N
 };
 
+// One device allocation shared by the members of a batch (mgpu_batch_create):
+// member r's per-realization buffers lie at base + r * stride, each at the same
+// offset inside its share, so a kernel reaches realization blockIdx.z of a
+// batched launch by adding blockIdx.z * stride to realization 0's pointers.
+struct mgpu_arena {
+  char *base = nullptr;
+  size_t stride = 0;
+  int count = 0;
+  int refs = 0;                   // members not yet destroyed
+  hipStream_t stream = nullptr;   // the one stream of all members
+};
+
+// realization blockIdx.z of a batched launch (zs = 0: an unbatched context)
+template <class T>
+__device__ __forceinline__ T *zbuf(T *p, size_t zs) {
+  return p ? reinterpret_cast<T *>(reinterpret_cast<uintptr_t>(p) + (size_t)blockIdx.z * zs) : p;
+}
+
 struct mgpu_ctx {
   mgpu_params p;
   int N = 0, Np = 0, HP = 0;
@@ -111,6 +129,14 @@ struct mgpu_ctx {
   bool step_warm = false;           // one eager step has run (kernel attributes and launch shapes settled)
   hipGraphExec_t step_exec = nullptr;  // the per-step graph, updated in place each call
   unsigned *ic_scratch = nullptr;  // this device's seed-table scratch (allocation-free IC)
+  // batches: the shared allocation (null for a context of its own), the bump pointer
+  // while the context is created, and the launch shape of the step kernels: bz
+  // realizations at byte stride zs (bz > 1 only inside mgpu_batch_step)
+  mgpu_arena *arena = nullptr;
+  char *arena_cur = nullptr, *arena_end = nullptr;
+  size_t *arena_measure = nullptr;  // creation that only measures: the batch share's bytes are added here
+  unsigned bz = 1;
+  size_t zs = 0;
 
   // state
   bool have_deposit = false, have_density = false, have_forces = false, have_disp = false;

@@ -219,7 +219,9 @@ __global__ __launch_bounds__(kBlock) void k_gather(const float *__restrict__ pos
 
 // sums of Disp for the mean, in particle order (see block_sum3)
 __global__ __launch_bounds__(kBlock) void k_sum3(const float *__restrict__ disp, size_t np,
-                                                 double *__restrict__ partial) {
+                                                 double *__restrict__ partial, size_t zs) {
+  disp = zbuf(disp, zs);
+  partial = zbuf(partial, zs);
   const size_t base = (size_t)blockIdx.x * kChunk + threadIdx.x;
   double s[3] = {0, 0, 0};
 #pragma unroll
@@ -237,7 +239,10 @@ __global__ __launch_bounds__(kBlock) void k_sum3(const float *__restrict__ disp,
 }
 
 // fixed-order reduction of per-block partials [nb][3] -> out[3]
-__global__ __launch_bounds__(1024) void k_reduce3(const double *__restrict__ partial, int nb, double *__restrict__ out) {
+__global__ __launch_bounds__(1024) void k_reduce3(const double *__restrict__ partial, int nb, double *__restrict__ out,
+                                                  size_t zs) {
+  partial = zbuf(partial, zs);
+  out = zbuf(out, zs);
   __shared__ double red[3][1024];
   double s[3] = {0, 0, 0};
   for (int b = threadIdx.x; b < nb; b += blockDim.x)
@@ -337,8 +342,15 @@ __global__ __launch_bounds__(kBlock) void k_kick_drift(float *__restrict__ vel, 
                                                        double dda, double A, double dyyy, float box, int use_cola,
                                                        double sD0, double sD1, double sD2, double sV0, double sV1,
                                                        double sV2, const double *__restrict__ dsum,
-                                                       double *__restrict__ partial) {
+                                                       double *__restrict__ partial, size_t zs) {
 #pragma clang fp contract(off)  // same rounding sequence as the separate passes
+  vel = zbuf(vel, zs);
+  pos = zbuf(pos, zs);
+  disp = zbuf(disp, zs);
+  l0 = zbuf(l0, zs);
+  l1 = zbuf(l1, zs);
+  dsum = zbuf(dsum, zs);
+  partial = zbuf(partial, zs);
   const size_t base = (size_t)blockIdx.x * kChunk + threadIdx.x;
   double s[3] = {0, 0, 0};
   double sD[3] = {sD0, sD1, sD2}, sV[3] = {sV0, sV1, sV2};
@@ -380,7 +392,7 @@ __global__ __launch_bounds__(kBlock) void k_kick_drift(float *__restrict__ vel, 
 // for Disp, ctx->dsum + 3 for the velocities), and copied to out unless out is
 // nullptr (then nothing waits for the device)
 int pm_reduce3(mgpu_ctx *ctx, int nblocks, double out[3], double *dst) {
-  hipLaunchKernelGGL(k_reduce3, dim3(1), dim3(1024), 0, ctx->stream, ctx->partial, nblocks, dst);
+  hipLaunchKernelGGL(k_reduce3, dim3(1, 1, ctx->bz), dim3(1024), 0, ctx->stream, ctx->partial, nblocks, dst, ctx->zs);
   MGPU_HIP(ctx, hipGetLastError());
   if (!out) return MGPU_OK;
   MGPU_HIP(ctx, hipMemcpyAsync(out, dst, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -706,6 +718,7 @@ int pm_gather(mgpu_ctx *ctx, double sumD[3]) {
       if (rc) return rc;
     }
   } else {
+    if (ctx->bz > 1) return mgpu_fail(ctx, MGPU_E_STATE, "a batched gather needs the bins of the last deposit");
     KScope ks(ctx, MGPU_K_GATHER, bytes);
     hipLaunchKernelGGL(k_gather, dim3(cdiv(ctx->np, kBlock)), dim3(kBlock), 0, ctx->stream, ctx->pos, ctx->np, N,
                        2 * ctx->HP, (double)N / ctx->p.box, f0, f1, f2, ctx->disp);
@@ -713,7 +726,8 @@ int pm_gather(mgpu_ctx *ctx, double sumD[3]) {
   MGPU_HIP(ctx, hipGetLastError());
   {
     KScope ks(ctx, MGPU_K_MEAN, 12.0 * ctx->np);
-    hipLaunchKernelGGL(k_sum3, dim3(nb), dim3(kBlock), 0, ctx->stream, ctx->disp, ctx->np, ctx->partial);
+    hipLaunchKernelGGL(k_sum3, dim3(nb, 1, ctx->bz), dim3(kBlock), 0, ctx->stream, ctx->disp, ctx->np, ctx->partial,
+                       ctx->zs);
   }
   MGPU_HIP(ctx, hipGetLastError());
   double s[3];
@@ -781,11 +795,11 @@ int pm_kick_drift(mgpu_ctx *ctx, double dda, double A, double s1k, double s2k, c
   {
     // vel, pos read + written, Disp and the LPT vectors read
     KScope ks(ctx, MGPU_K_KICK, 84.0 * ctx->np);
-    hipLaunchKernelGGL(k_kick_drift, dim3(nb), dim3(kBlock), 0, ctx->stream, ctx->vel, ctx->pos,
+    hipLaunchKernelGGL(k_kick_drift, dim3(nb, 1, ctx->bz), dim3(kBlock), 0, ctx->stream, ctx->vel, ctx->pos,
                        (const float *)ctx->disp, (const float *)ctx->lpt[0], (const float *)ctx->lpt[1], ctx->np,
                        sdep ? 1 : 0, s1k, s2k, sdep ? 2 : 0, s1d, s2d, ctx->p.omega, dda, A, dyyy,
                        (float)ctx->p.box, ctx->p.use_cola, hD[0], hD[1], hD[2], hV[0], hV[1], hV[2],
-                       dev ? (const double *)ctx->dsum : nullptr, ctx->partial);
+                       dev ? (const double *)ctx->dsum : nullptr, ctx->partial, ctx->zs);
   }
   MGPU_HIP(ctx, hipGetLastError());
   double s[3];

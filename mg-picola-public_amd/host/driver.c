@@ -39,8 +39,19 @@ struct mgp_sim {
   double learn_theta;
   int nlog, caplog;
   double *log_a, *log_feat, *log_pk; /* log_pk: 3 nbins per entry (P, k mean, modes) */
+  /* a member of a batch (mgp_sim_create_batch): the record shared by the members and this member's index */
+  struct batch_rec *brec;
+  int batch_idx;
 };
 typedef struct mgp_sim sim;
+
+/* Shared by the members of a batch: their contexts in order, and the time-loop
+ * iteration in which member 0 last stepped all of them with mgpu_batch_step. */
+struct batch_rec {
+  mgpu_ctx **ctxs;
+  int count, refs;
+  long stepped_iter;
+};
 
 static double now_s(void) {
   struct timespec t;
@@ -330,7 +341,8 @@ static int get_displacements(sim *s, double A) {
 
 
 /* Set-up: growth tables and the GPU context (src/main.c:68-162). */
-mgp_sim *mgp_sim_create(mgp_params *p, int verbose, char *err, size_t errlen) {
+/* the host side of a run up to its GPU context: tables, P(k) binning and the context's parameters */
+static sim *sim_alloc(mgp_params *p, int verbose, mgpu_params *gp) {
   sim *s = calloc(1, sizeof(sim));
   s->p = p;
   s->verbose = verbose;
@@ -346,54 +358,56 @@ mgp_sim *mgp_sim_create(mgp_params *p, int verbose, char *err, size_t errlen) {
   }
   const double t_start = now_s();
   s->c = mgp_cosmo_create(p, s->err, s->errlen);
-  if (!s->c) goto fail;
+  if (!s->c) return s;
   if (s->verbose) printf("[mgpicola] growth tables ready (%.2f s)\n", now_s() - t_start);
-  mgpu_params gp;
-  memset(&gp, 0, sizeof gp);
-  gp.nmesh = p->Nmesh;
-  gp.nsample = p->Nsample;
-  gp.box = p->Box;
-  gp.omega = p->Omega;
+  memset(gp, 0, sizeof *gp);
+  gp->nmesh = p->Nmesh;
+  gp->nsample = p->Nsample;
+  gp->box = p->Box;
+  gp->omega = p->Omega;
   switch (p->model) {
     case MGP_MODEL_FOFR:
     case MGP_MODEL_FOFRNU:
-    case MGP_MODEL_FOFR_LCDM: gp.model = MGPU_FOFR; break;
-    case MGP_MODEL_DGP: gp.model = MGPU_DGP; break;
-    case MGP_MODEL_BRANSDICKE: gp.model = MGPU_BRANSDICKE; break;
-    default: gp.model = MGPU_MBETA;
+    case MGP_MODEL_FOFR_LCDM: gp->model = MGPU_FOFR; break;
+    case MGP_MODEL_DGP: gp->model = MGPU_DGP; break;
+    case MGP_MODEL_BRANSDICKE: gp->model = MGPU_BRANSDICKE; break;
+    default: gp->model = MGPU_MBETA;
   }
-  gp.mg_active = p->modified_gravity_active;
-  gp.include_screening = p->include_screening;
-  gp.filter = p->model == MGP_MODEL_DGP ? MGPU_FILTER_GAUSSIAN : MGPU_FILTER_NONE;
-  gp.scale_dependent = p->scale_dependent;
-  gp.use_cola = p->UseCOLA;
+  gp->mg_active = p->modified_gravity_active;
+  gp->include_screening = p->include_screening;
+  gp->filter = p->model == MGP_MODEL_DGP ? MGPU_FILTER_GAUSSIAN : MGPU_FILTER_NONE;
+  gp->scale_dependent = p->scale_dependent;
+  gp->use_cola = p->UseCOLA;
   s->nbins = p->pofk_nbins;
   s->bintype = p->pofk_bintype;
   s->subtract = p->pofk_subtract_shotnoise;
   s->kmin = p->pofk_kmin * p->Box / (2.0 * M_PI);
   s->kmax = p->pofk_kmax * p->Box / (2.0 * M_PI);
   adjust_pofk(p, &s->nbins, &s->bintype, &s->subtract, &s->kmin, &s->kmax);
-  gp.pofk_nbins = s->nbins;
-  gp.pofk_bintype = s->bintype;
-  gp.pofk_subtract_shotnoise = s->subtract;
-  gp.pofk_kmin = s->kmin;
-  gp.pofk_kmax = s->kmax;
-  if (mgpu_create(&gp, &s->g)) {
-    snprintf(s->err, s->errlen, "mgpu_create failed: %s", mgpu_last_error(NULL));
-    goto fail;
+  gp->pofk_nbins = s->nbins;
+  gp->pofk_bintype = s->bintype;
+  gp->pofk_subtract_shotnoise = s->subtract;
+  gp->pofk_kmin = s->kmin;
+  gp->pofk_kmax = s->kmax;
+  return s;
+}
+
+/* the column passes' tile width measured on this device, before any data */
+static int sim_autotune(sim *s) {
+  double ms[4];
+  if (mgpu_autotune_fft(s->g, ms)) {
+    snprintf(s->err, s->errlen, "FFT autotune failed: %s", mgpu_last_error(s->g));
+    return -1;
   }
-  {
-    /* the column passes' tile width measured on this device, before any data */
-    double ms[4];
-    if (mgpu_autotune_fft(s->g, ms)) {
-      snprintf(s->err, s->errlen, "FFT autotune failed: %s", mgpu_last_error(s->g));
-      goto fail;
-    }
-    int t0 = 0;
-    mgpu_get_fft_tile(s->g, 0, &t0);
-    if (verbose)
-      printf("[mgpicola] FFT tiles: %d columns (8: %.3f ms, 16: %.3f ms per y pass pair)\n", t0, ms[0], ms[1]);
-  }
+  int t0 = 0;
+  mgpu_get_fft_tile(s->g, 0, &t0);
+  if (s->verbose)
+    printf("[mgpicola] FFT tiles: %d columns (8: %.3f ms, 16: %.3f ms per y pass pair)\n", t0, ms[0], ms[1]);
+  return 0;
+}
+
+static void sim_buffers(sim *s) {
+  const mgp_params *p = s->p;
   s->pk = calloc(s->nbins, sizeof(double));
   s->pkk = calloc(s->nbins, sizeof(double));
   s->pkn = calloc(s->nbins, sizeof(double));
@@ -401,6 +415,18 @@ mgp_sim *mgp_sim_create(mgp_params *p, int verbose, char *err, size_t errlen) {
   const size_t n2len = mgpu_n2_table_len(p->Nmesh);
   s->g1 = malloc(sizeof(double) * (n2len > 8000 ? n2len : 8000));
   s->g2 = malloc(sizeof(double) * (n2len > 8000 ? n2len : 8000));
+}
+
+mgp_sim *mgp_sim_create(mgp_params *p, int verbose, char *err, size_t errlen) {
+  mgpu_params gp;
+  sim *s = sim_alloc(p, verbose, &gp);
+  if (!s->c) goto fail;
+  if (mgpu_create(&gp, &s->g)) {
+    snprintf(s->err, s->errlen, "mgpu_create failed: %s", mgpu_last_error(NULL));
+    goto fail;
+  }
+  if (sim_autotune(s)) goto fail;
+  sim_buffers(s);
   return s;
 fail:
   snprintf(err, errlen, "%s", s->err);
@@ -408,9 +434,70 @@ fail:
   return NULL;
 }
 
+int mgp_sim_create_batch(mgp_params **ps, int count, int verbose, mgp_sim **out, char *err, size_t errlen) {
+  if (count < 1) {
+    snprintf(err, errlen, "empty batch");
+    return -1;
+  }
+  for (int r = 0; r < count; r++) out[r] = NULL;
+  mgpu_params gp0, gp;
+  struct batch_rec *b = calloc(1, sizeof *b);
+  b->ctxs = calloc(count, sizeof(mgpu_ctx *));
+  b->count = count;
+  b->stepped_iter = -1;
+  int rc = 0;
+  for (int r = 0; r < count && !rc; r++) {
+    out[r] = sim_alloc(ps[r], verbose, r ? &gp : &gp0);
+    out[r]->brec = b;
+    out[r]->batch_idx = r;
+    b->refs++;
+    if (!out[r]->c) {
+      snprintf(err, errlen, "member %d: %s", r, out[r]->err);
+      rc = -1;
+    } else if (r && memcmp(&gp, &gp0, sizeof gp)) {
+      snprintf(err, errlen, "member %d: the members of a batch differ in mesh, particles, box, model or P(k) bins", r);
+      rc = -1;
+    }
+  }
+  if (!rc && mgpu_batch_create(&gp0, count, b->ctxs)) {
+    snprintf(err, errlen, "mgpu_batch_create failed: %s", mgpu_last_error(NULL));
+    rc = -1;
+  }
+  if (!rc) {
+    for (int r = 0; r < count; r++) out[r]->g = b->ctxs[r];
+    /* one width for all: the members' kernels are launched together */
+    int t0 = 0;
+    if (sim_autotune(out[0])) {
+      snprintf(err, errlen, "%s", out[0]->err);
+      rc = -1;
+    } else {
+      mgpu_get_fft_tile(out[0]->g, 0, &t0);
+      for (int r = 1; r < count && !rc; r++)
+        if (mgpu_set_fft_tile(out[r]->g, 0, t0)) {
+          snprintf(err, errlen, "member %d: %s", r, mgpu_last_error(out[r]->g));
+          rc = -1;
+        }
+    }
+  }
+  if (rc) {
+    for (int r = 0; r < count; r++) {
+      if (out[r] && !out[r]->g && b->ctxs[r]) mgpu_destroy(b->ctxs[r]);
+      mgp_sim_free(out[r]);
+      out[r] = NULL;
+    }
+    return -1;
+  }
+  for (int r = 0; r < count; r++) sim_buffers(out[r]);
+  return 0;
+}
+
 void mgp_sim_free(mgp_sim *s) {
   if (!s) return;
   if (s->g) mgpu_destroy(s->g);
+  if (s->brec && --s->brec->refs == 0) {
+    free(s->brec->ctxs);
+    free(s->brec);
+  }
   mgp_cosmo_free(s->c);
   free(s->pk);
   free(s->pkk);
@@ -514,14 +601,22 @@ int mgp_sim_advance(mgp_sim *s, int *counted) {
   }
   const int output_now = s->ts == 0 && s->i != s->NoutputStart;
   int stepped = 0;
-  if (!output_now && s->vmean_valid && p->fuse_kick_drift && device_step_ok(s)) {
+  if (s->brec && s->batch_idx > 0 && s->brec->stepped_iter == s->steps_done) {
+    /* member 0 stepped the whole batch in this iteration */
+    stepped = 1;
+    s->means_on_device = 1;
+  } else if (!output_now && s->vmean_valid && p->fuse_kick_drift && device_step_ok(s) &&
+             (!s->brec || s->batch_idx == 0)) {
     /* the whole step on the device: the means stay there */
     mgpu_mg mg;
     mgp_mg_coeffs(s->c, A, &mg);
     double c[7];
     kick_drift_coeffs(s, s->AI, s->AF, A, s->AFF, s->Di, s->Di2, c);
-    int rc = mgpu_step(s->g, p->modified_gravity_active ? &mg : NULL, c[0], c[6], c[1], c[2], c[3], c[4], c[5],
-                       p->device_step == 2);
+    const mgpu_mg *pm = p->modified_gravity_active ? &mg : NULL;
+    int rc = s->brec ? mgpu_batch_step(s->brec->ctxs, s->brec->count, pm, c[0], c[6], c[1], c[2], c[3], c[4], c[5],
+                                       p->device_step == 2)
+                     : mgpu_step(s->g, pm, c[0], c[6], c[1], c[2], c[3], c[4], c[5], p->device_step == 2);
+    if (s->brec) s->brec->stepped_iter = rc == 0 ? s->steps_done : -1;
     if (rc == 0) {
       stepped = 1;
       s->means_on_device = 1;

@@ -150,6 +150,12 @@ int mgp_run(mgp_params *p, int verbose, mgp_run_stats *stats, char *err, size_t 
  * conditions, then one time-loop iteration per advance call */
 typedef struct mgp_sim mgp_sim;
 mgp_sim *mgp_sim_create(mgp_params *p, int verbose, char *err, size_t errlen);
+/* count runs of one configuration (ps[r] may differ in Seed, the IC pairing flags
+ * and the output names only) whose GPU contexts form one mgpu batch; advanced
+ * in lockstep (each member's mgp_sim_advance in turn, member 0 first), the steps
+ * that go through mgpu_step advance all members with one mgpu_batch_step.  Each
+ * member's results are those of its solo run.  0 or -1 (err filled). */
+int mgp_sim_create_batch(mgp_params **ps, int count, int verbose, mgp_sim **out, char *err, size_t errlen);
 int mgp_sim_ic(mgp_sim *s);
 int mgp_sim_advance(mgp_sim *s, int *counted);
 int mgp_sim_finished(const mgp_sim *s);

@@ -30,7 +30,7 @@ MGPU_SYMBOLS = [
     "mgpu_lpt_fields", "mgpu_get_mesh", "mgpu_set_mesh", "mgpu_get_timers", "mgpu_enable_timers",
     "mgpu_kernel_name", "mgpu_enable_kernel_timing", "mgpu_kernel_timing", "mgpu_lpt_fields_slices",
     "mgpu_lpt_fields_slices2", "mgpu_ic_displacement_fields_noalloc", "mgpu_memory_plan", "mgpu_step",
-    "mgpu_get_means",
+    "mgpu_get_means", "mgpu_batch_create", "mgpu_batch_step",
 ]
 
 
@@ -173,9 +173,31 @@ class MGPU:
                              pofk_kmax)
         self.n, self.ns, self.np = nmesh, nsample, nsample ** 3
         self.ctx = C.c_void_p()
+        if getattr(self, "_no_create", False):
+            return
         rc = lib().mgpu_create(C.byref(self.p), C.byref(self.ctx))
         if rc:
             raise MGPUError(f"mgpu_create failed ({rc}): {lib().mgpu_last_error(None).decode()}")
+
+    @classmethod
+    def batch(cls, count, *args, **kw):
+        """count contexts of one configuration sharing one allocation and stream
+        (mgpu_batch_create); step them together with batch_step."""
+        members = []
+        for _ in range(count):
+            m = cls.__new__(cls)
+            m._no_create = True
+            cls.__init__(m, *args, **kw)
+            members.append(m)
+        L = lib()
+        arr = (C.c_void_p * count)()
+        L.mgpu_batch_create.argtypes = [C.POINTER(Params), C.c_int, C.c_void_p]
+        rc = L.mgpu_batch_create(C.byref(members[0].p), count, arr)
+        if rc:
+            raise MGPUError(f"mgpu_batch_create failed ({rc}): {L.mgpu_last_error(None).decode()}")
+        for m, h in zip(members, arr):
+            m.ctx = C.c_void_p(h)
+        return members
 
     def close(self):
         if self.ctx:
@@ -313,6 +335,18 @@ class MGPU:
         L.mgpu_step.argtypes = [C.c_void_p, C.c_void_p, d, d, d, d, d, d, d, C.c_int]
         self._ck(L.mgpu_step(self.ctx, None if mg is None else C.byref(mg), dda, A, s1k, s2k, dyyy, s1d, s2d,
                              1 if graph else 0), "step")
+
+    @staticmethod
+    def batch_step(members, dda, A, s1k, s2k, dyyy, s1d, s2d, mg=None, graph=False):
+        """mgpu_step for all members of one batch, one launch per kernel."""
+        L = lib()
+        d = C.c_double
+        arr = (C.c_void_p * len(members))(*[m.ctx.value for m in members])
+        L.mgpu_batch_step.argtypes = [C.c_void_p, C.c_int, C.c_void_p, d, d, d, d, d, d, d, C.c_int]
+        rc = L.mgpu_batch_step(arr, len(members), None if mg is None else C.byref(mg), dda, A, s1k, s2k, dyyy, s1d,
+                               s2d, 1 if graph else 0)
+        if rc:
+            raise MGPUError(f"batch_step failed ({rc}): {L.mgpu_last_error(members[0].ctx).decode()}")
 
     def get_means(self):
         """(mean Disp of the last gather, mean velocity of the last kick) from the device."""
@@ -597,33 +631,42 @@ def kernel_timing(g, reset=True):
     return out
 
 
+def _simlib():
+    L = hlib()
+    if not getattr(L, "_sim", False):
+        L.mgp_sim_create.argtypes = [C.POINTER(HostParams), C.c_int, C.c_char_p, C.c_size_t]
+        L.mgp_sim_create.restype = C.c_void_p
+        for n in ("mgp_sim_ic", "mgp_sim_finished"):
+            getattr(L, n).argtypes = [C.c_void_p]
+        L.mgp_sim_advance.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.mgp_sim_free.argtypes = [C.c_void_p]
+        L.mgp_sim_free.restype = None
+        L.mgp_sim_gpu.argtypes = [C.c_void_p]
+        L.mgp_sim_gpu.restype = C.c_void_p
+        L.mgp_sim_cosmo.argtypes = [C.c_void_p]
+        L.mgp_sim_cosmo.restype = C.c_void_p
+        L.mgp_sim_error.argtypes = [C.c_void_p]
+        L.mgp_sim_error.restype = C.c_char_p
+        L.mgp_sim_a.argtypes = [C.c_void_p]
+        L.mgp_sim_a.restype = C.c_double
+        L.mgp_sim_create_batch.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_char_p, C.c_size_t]
+        L._sim = True
+    return L
+
+
 class Sim:
     """The C host's simulation (libmgpicola mgp_sim_*): IC, then one time-loop iteration per advance()."""
 
-    def __init__(self, hp, verbose=False):
-        L = hlib()
-        if not getattr(L, "_sim", False):
-            L.mgp_sim_create.argtypes = [C.POINTER(HostParams), C.c_int, C.c_char_p, C.c_size_t]
-            L.mgp_sim_create.restype = C.c_void_p
-            for n in ("mgp_sim_ic", "mgp_sim_finished"):
-                getattr(L, n).argtypes = [C.c_void_p]
-            L.mgp_sim_advance.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
-            L.mgp_sim_free.argtypes = [C.c_void_p]
-            L.mgp_sim_free.restype = None
-            L.mgp_sim_gpu.argtypes = [C.c_void_p]
-            L.mgp_sim_gpu.restype = C.c_void_p
-            L.mgp_sim_cosmo.argtypes = [C.c_void_p]
-            L.mgp_sim_cosmo.restype = C.c_void_p
-            L.mgp_sim_error.argtypes = [C.c_void_p]
-            L.mgp_sim_error.restype = C.c_char_p
-            L.mgp_sim_a.argtypes = [C.c_void_p]
-            L.mgp_sim_a.restype = C.c_double
-            L._sim = True
+    def __init__(self, hp, verbose=False, _handle=None):
+        L = _simlib()
         self.hp = hp
-        err = C.create_string_buffer(4096)
-        self.h = L.mgp_sim_create(C.byref(hp), 1 if verbose else 0, err, 4096)
-        if not self.h:
-            raise MGPUError(err.value.decode())
+        if _handle is not None:
+            self.h = _handle
+        else:
+            err = C.create_string_buffer(4096)
+            self.h = L.mgp_sim_create(C.byref(hp), 1 if verbose else 0, err, 4096)
+            if not self.h:
+                raise MGPUError(err.value.decode())
         # an MGPU view sharing the context (not owning it)
         g = MGPU.__new__(MGPU)
         g.p = None
@@ -631,6 +674,19 @@ class Sim:
         g.ctx = C.c_void_p(L.mgp_sim_gpu(self.h))
         g.close = lambda: None
         self.gpu = g
+
+    @classmethod
+    def batch(cls, hps, verbose=False):
+        """Runs of one configuration (HostParams differing in seed, pairing and output names) whose contexts
+        form one mgpu batch (mgp_sim_create_batch); advance them in lockstep, member 0 first."""
+        L = _simlib()
+        n = len(hps)
+        arr = (C.c_void_p * n)(*[C.cast(C.pointer(hp), C.c_void_p).value for hp in hps])
+        out = (C.c_void_p * n)()
+        err = C.create_string_buffer(4096)
+        if L.mgp_sim_create_batch(arr, n, 1 if verbose else 0, out, err, 4096):
+            raise MGPUError(err.value.decode())
+        return [cls(hp, _handle=h) for hp, h in zip(hps, out)]
 
     def _ck(self, rc, what):
         if rc:

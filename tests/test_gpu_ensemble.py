@@ -114,6 +114,94 @@ def test_device_step_equals_separate_calls(model):
     assert not np.array_equal(out[0][0], pos0)
 
 
+def _member_start(g, mg):
+    sD, _ = g.get_displacements(mg=mg)
+    sV = g.kick(0.1, 0.5, 0.3, -0.2, sD)  # the velocity sums land on the device
+    g.drift(0.05, 0.01, 0.0, sV)
+
+
+@pytest.mark.parametrize("model,n,ns,count", [("lcdm", 48, 24, 3), ("dgp", 48, 24, 3), ("dgp", 96, 48, 4)])
+def test_batch_step_equals_solo_steps(model, n, ns, count):
+    """mgpu_batch_step (one launch per kernel over all members) gives every member the particles and
+    device-resident means of mgpu_step on that member alone, bit for bit, over five steps, eagerly and as a
+    per-step hipGraph; members start from different particles."""
+    from helpers import lattice_particles
+    box = float(n)
+    kw = dict(model=M.DGP, mg_active=1, include_screening=1, filter=1) if model == "dgp" else {}
+    mg = M.MG(a=0.6, coupling=0.4, mass2=0.0, phi_crit=0.0, dgp_fac=1.7, rsmooth=1.0, geff=1.0) if model == "dgp" \
+        else None
+    starts = []
+    for r in range(count):
+        pos0 = lattice_particles(ns, box, amp=1.0 + 0.2 * r, seed=11 + r)
+        vel0 = (np.random.default_rng(5 + r).normal(size=pos0.shape) * 2.0).astype(np.float32)
+        starts.append((pos0, vel0))
+    coeffs = [(0.1 + 0.01 * s, 0.5 + 0.05 * s, 0.3, -0.2, 0.07 + 0.01 * s, 0.02, -0.01) for s in range(5)]
+    solo = []
+    for pos0, vel0 in starts:
+        g = M.MGPU(n, ns, box, 0.3, **kw)
+        g.set_particles(pos0, vel0)
+        _member_start(g, mg)
+        for c in coeffs:
+            g.step(*c, mg=mg)
+        solo.append(g.get_particles() + g.get_means())
+        g.close()
+    for graph in (False, True):
+        members = M.MGPU.batch(count, n, ns, box, 0.3, **kw)
+        for g, (pos0, vel0) in zip(members, starts):
+            g.set_particles(pos0, vel0)
+            _member_start(g, mg)
+        for c in coeffs:
+            M.MGPU.batch_step(members, *c, mg=mg, graph=graph)
+        print(f"{model} n {n} count {count} graph {graph}", flush=True)
+        for r, g in enumerate(members):
+            p, v = g.get_particles()
+            md, mv = g.get_means()
+            np.testing.assert_array_equal(p, solo[r][0])
+            np.testing.assert_array_equal(v, solo[r][1])
+            np.testing.assert_array_equal(md, solo[r][2])
+            np.testing.assert_array_equal(mv, solo[r][3])
+        # a member is a whole context: the separate calls go on from the batch's state
+        sD, _ = members[1].get_displacements(mg=mg)
+        assert np.all(np.isfinite(sD))
+        for g in members:
+            g.close()
+    assert not np.array_equal(solo[0][0], solo[1][0])
+
+
+def test_batch_step_refuses_foreign_members():
+    a = M.MGPU.batch(2, 32, 16, 32.0, 0.3)
+    b = M.MGPU(32, 16, 32.0, 0.3)
+    for g in a + [b]:
+        g.set_particles(np.zeros((3, 16 ** 3), np.float32) + 1.0, np.zeros((3, 16 ** 3), np.float32))
+        sD, _ = g.get_displacements()
+        g.kick(0.1, 0.5, 0.0, 0.0, sD)
+    for bad in ([a[1], a[0]], [a[0], b], [a[0]]):
+        with pytest.raises(M.MGPUError):
+            M.MGPU.batch_step(bad, 0.1, 0.5, 0.0, 0.0, 0.05, 0.0, 0.0)
+    M.MGPU.batch_step(a, 0.1, 0.5, 0.0, 0.0, 0.05, 0.0, 0.0)
+    for g in a + [b]:
+        g.close()
+
+
+def test_batched_runs_equal_solo_runs(tmp_path):
+    """The ensemble as batches (ensemble.run_batch: one mgpu batch per size class, the members in lockstep, each
+    step between outputs one launch per kernel over all members, eagerly or as one hipGraph per step, the
+    batches one after the other or on two host threads) gives every realization the particles of its solo run
+    with the separate calls, bit for bit."""
+    sizes = (16, 16, 16, 16, 24, 24)
+    hps = [params(tmp_path, f"s{i}", ns, 200 + i // 2, i % 2, nsteps=6) for i, ns in enumerate(sizes)]
+    solo, _ = E.run_ensemble(hps, threads=1, keep_particles=True)
+    for graph, threads in ((False, 1), (True, 1), (False, 2)):
+        hps2 = [M.HostParams.from_buffer_copy(h) for h in hps]
+        bat, _ = E.run_batch(hps2, keep_particles=True, graph=graph, threads=threads)
+        print(f"graph {graph} threads {threads}: {len(bat)} realizations", flush=True)
+        for a, b in zip(solo, bat):
+            assert a["steps"] == b["steps"] == 7
+            np.testing.assert_array_equal(a["pos"], b["pos"])
+            np.testing.assert_array_equal(a["vel"], b["vel"])
+    assert not np.array_equal(solo[0]["pos"], solo[2]["pos"])
+
+
 def test_device_step_runs_through_the_host(tmp_path):
     """The C host with device_step = 1 (steps on the device) and 2 (each step one hipGraph) runs the
     ensemble's workload to the same particles as the separate calls, bit for bit."""
