@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/mgpu.h"
@@ -48,10 +49,14 @@ struct mgpu_arena {
   hipStream_t stream = nullptr;   // the one stream of all members
 };
 
-// realization blockIdx.z of a batched launch (zs = 0: an unbatched context)
+// realization blockIdx.z of a batched launch (zs = 0: an unbatched context).  Byte
+// arithmetic on the pointer itself (not through an integer), so the compiler still
+// sees a global-memory pointer and keeps global_load/global_store instead of flat
+// accesses; a null pointer (an absent optional buffer) stays null.
 template <class T>
 __device__ __forceinline__ T *zbuf(T *p, size_t zs) {
-  return p ? reinterpret_cast<T *>(reinterpret_cast<uintptr_t>(p) + (size_t)blockIdx.z * zs) : p;
+  using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+  return reinterpret_cast<T *>(reinterpret_cast<B *>(p) + (p ? (size_t)blockIdx.z * zs : (size_t)0));
 }
 
 struct mgpu_ctx {
