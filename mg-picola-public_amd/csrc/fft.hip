@@ -68,7 +68,7 @@ __global__ __launch_bounds__(col_threads(L, T)) void k_fft_col(float2 *__restric
     float2 *base = tile(i);
     auto ld = [&](int c, int j, int o) { return (base + (size_t)j * line_stride + c)[(size_t)o * line_stride]; };
     auto st = [&](int c, int j, int o, float2 v) { (base + (size_t)j * line_stride + c)[(size_t)o * line_stride] = v; };
-    fft_stages<L, T, NT, SIGN, 1, T, true, false, 1>(lds, tw, 1, tid, ld, st, cur);
+    fft_stages<L, T, NT, SIGN, 1, T, true, false, 1, false, true>(lds, tw, 1, tid, ld, st, cur);
     if (i + 1 < TPB) __syncthreads();  // the next tile's first exchange reuses the LDS
   }
 }

@@ -137,13 +137,15 @@ constexpr int elems_per_thread(int L) { return (L % 3 == 0) ? (L < 24 ? L : 24) 
 // in between exchange through LDS, element
 // (b, n) at lds[b*BS + n*ES].  BFAST maps consecutive lanes to consecutive b.
 // With IN_LDS the first stage's input lives in the same LDS buffer, so a
-// barrier separates those reads from the first exchange write.  With pre !=
-// nullptr the first stage's operands are already in registers, pre[q*R + r]
-// holding what ld(b, j, r*M) would return for butterfly q (see first_loads).
+// barrier separates those reads from the first exchange write.  With PRE the
+// first stage's operands are already in registers, pre[q*R + r] holding what
+// ld(b, j, r*M) would return for butterfly q (see first_loads); a compile-time
+// flag, not a null test on pre, so the register array is never addressed
+// through a pointer at run time (which would place it in scratch memory).
 // With SYNC_ST a barrier precedes the last stage's stores, so st may write the
 // same LDS buffer (every thread has read its last-stage operands by then).
 template <int L, int B, int NT, int SIGN, int BS, int ES, bool BFAST, bool IN_LDS, int Ns, bool SYNC_ST = false,
-          class LD, class ST>
+          bool PRE = false, class LD, class ST>
 __device__ __forceinline__ void fft_stages(float2 *lds, const float2 *__restrict__ tw, int twstride, int tid,
                                            const LD &ld, const ST &st, const float2 *pre = nullptr) {
   constexpr int E = L * B / NT;
@@ -182,7 +184,8 @@ __device__ __forceinline__ void fft_stages(float2 *lds, const float2 *__restrict
     for (int r = 0; r < R; ++r) {
       float2 x;
       if constexpr (FIRST) {
-        x = pre ? pre[q * R + r] : ld(b, j, r * M);
+        if constexpr (PRE) x = pre[q * R + r];
+        else x = ld(b, j, r * M);
       } else {
         x = lds[b * BS + (j + r * M) * ES];
         const int r1 = r % R1, r2 = r / R1;

@@ -114,11 +114,11 @@ __global__ __launch_bounds__(col_threads(L, T)) void k_col_green(const float2 *_
   }
   auto ld = [&](int, int, int) { return make_float2(0.f, 0.f); };
   auto st0 = [&](int c, int j, int o, float2 v) { out0[tile + (size_t)j * ls + c + (size_t)o * ls] = v; };
-  fft_stages<L, T, NT, +1, 1, T, true, false, 1>(lds, tw, 1, tid, ld, st0, p1);
+  fft_stages<L, T, NT, +1, 1, T, true, false, 1, false, true>(lds, tw, 1, tid, ld, st0, p1);
   if (MODE == GREEN_FORCE) {
     __syncthreads();
     auto st1 = [&](int c, int j, int o, float2 v) { out1[tile + (size_t)j * ls + c + (size_t)o * ls] = v; };
-    fft_stages<L, T, NT, +1, 1, T, true, false, 1>(lds, tw, 1, tid, ld, st1, p2);
+    fft_stages<L, T, NT, +1, 1, T, true, false, 1, false, true>(lds, tw, 1, tid, ld, st1, p2);
   }
 }
 
@@ -150,10 +150,10 @@ __global__ __launch_bounds__(col_threads(L, T)) void k_col_dual(float2 *buf, flo
   }
   auto ld = [&](int, int, int) { return make_float2(0.f, 0.f); };
   auto st0 = [&](int c, int j, int o, float2 v) { oy[tile + (size_t)j * ls + c + (size_t)o * ls] = v; };
-  fft_stages<L, T, NT, +1, 1, T, true, false, 1>(lds, tw, 1, tid, ld, st0, p1);
+  fft_stages<L, T, NT, +1, 1, T, true, false, 1, false, true>(lds, tw, 1, tid, ld, st0, p1);
   __syncthreads();
   auto st1 = [&](int c, int j, int o, float2 v) { buf[tile + (size_t)j * ls + c + (size_t)o * ls] = v; };
-  fft_stages<L, T, NT, +1, 1, T, true, false, 1>(lds, tw, 1, tid, ld, st1, p2);
+  fft_stages<L, T, NT, +1, 1, T, true, false, 1, false, true>(lds, tw, 1, tid, ld, st1, p2);
 }
 
 // x pass, forward, then phi = coupling n^2/(n^2 + m2) X (0 at n = 0): EffDensitykToPhiofk
@@ -291,7 +291,9 @@ __global__ __launch_bounds__(col_threads(L, T)) void k_col_fused(
     lds[c + kx * T] = v;
   };
   fft_stages<L, T, NT, -1, 1, T, true, false, 1, true>(lds, tw, 1, tid, ld, st);
-  // delta(k) to add, in the inverse's first-stage layout
+  // delta(k) to add, in the inverse's first-stage layout (requested after the
+  // forward's loads: the vector-memory counter is waited on in order, so loads
+  // issued earlier would delay the forward's first stage)
   float2 p1[E], p2[E];
   if (FWD != FWD_DENSITY) {
 #pragma unroll
@@ -325,11 +327,11 @@ __global__ __launch_bounds__(col_threads(L, T)) void k_col_fused(
   }
   auto ldz = [&](int, int, int) { return make_float2(0.f, 0.f); };
   auto st0 = [&](int c, int j, int o, float2 v) { out0[tile + (size_t)j * ls + c + (size_t)o * ls] = v; };
-  fft_stages<L, T, NT, +1, 1, T, true, false, 1>(lds, tw, 1, tid, ldz, st0, p1);
+  fft_stages<L, T, NT, +1, 1, T, true, false, 1, false, true>(lds, tw, 1, tid, ldz, st0, p1);
   if (MODE == GREEN_FORCE) {
     __syncthreads();
     auto st1 = [&](int c, int j, int o, float2 v) { out1[tile + (size_t)j * ls + c + (size_t)o * ls] = v; };
-    fft_stages<L, T, NT, +1, 1, T, true, false, 1>(lds, tw, 1, tid, ldz, st1, p2);
+    fft_stages<L, T, NT, +1, 1, T, true, false, 1, false, true>(lds, tw, 1, tid, ldz, st1, p2);
   }
   if (bin) {
     __syncthreads();
