@@ -197,7 +197,7 @@ bool fft_size_supported(int n) {
 // 16-column tiles fit one workgroup's LDS and thread limit
 bool fft_tile16_fits(const mgpu_ctx *ctx, int n) {
   const size_t need16 = (size_t)n * 16 * sizeof(float2);
-  return need16 <= ctx->lds_max && n * 16 / (n % 3 == 0 ? 24 : 16) <= 1024;
+  return need16 <= ctx->lds_max && n * 16 / elems_per_thread(n) <= 1024;
 }
 
 int fft_plan_init(mgpu_ctx *ctx, FFTPlan *plan, int n, int tcols) {

@@ -129,7 +129,12 @@ constexpr int pick_radix(int rem, int e) {
                                          : 0;
 }
 // elements held per thread for a length-L transform
-constexpr int elems_per_thread(int L) { return (L % 3 == 0) ? (L < 24 ? L : 24) : (L < 16 ? L : 16); }
+// (radix-3 lengths: 12 up to 768, so the z passes and the fused column passes of
+// the 192-768 meshes run two or more waves per SIMD; 24 from 1536 on, where 12
+// would need 1024-thread column blocks capped at 128 VGPRs)
+constexpr int elems_per_thread(int L) {
+  return (L % 3 == 0) ? (L < 12 ? L : L < 1536 ? 12 : 24) : (L < 16 ? L : 16);
+}
 
 // Stockham DIT on B independent transforms of length L, E = L*B/NT elements per
 // thread.  The first stage reads element n = j + r*M of transform b through

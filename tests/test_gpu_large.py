@@ -36,8 +36,10 @@ SIZES = [192, 256, 384, 512, 768, 1024, 1536]
 
 
 def tile_widths(n):
-    """16-column tiles where they fit a workgroup (160 KiB of LDS, 1024 threads)."""
-    fits16 = n * 16 * 8 <= 160 * 1024 and n * 16 // (24 if n % 3 == 0 else 16) <= 1024
+    """16-column tiles where they fit a workgroup (160 KiB of LDS, 1024 threads at 12 elements per thread for
+    the radix-3 sizes up to 768, 24 from 1536 on, 16 for the others)."""
+    ept = (12 if n < 1536 else 24) if n % 3 == 0 else 16
+    fits16 = n * 16 * 8 <= 160 * 1024 and n * 16 // ept <= 1024
     return (8, 16) if fits16 else (8,)
 
 
