@@ -213,6 +213,15 @@ int mgpu_get_means(mgpu_ctx *ctx, double sumD[3], double sumV[3]);
  * which member r's buffers lie at a fixed stride from member 0's.  Each member
  * runs the main loop of src/main.c:394-611 as a solo run would. */
 int mgpu_batch_create(const mgpu_params *p, int count, mgpu_ctx **members);
+/* mgpu_ic_displacement_fields for every member of a batch at once (members in
+ * the order of mgpu_batch_create; seeds[r] and inverted[r] per member, the
+ * sphere mode, the amplitude fixing and pk_n2 shared): each kernel of the IC is
+ * one launch over all members.  Each member's LPT slots are bit-identical to
+ * mgpu_ic_displacement_fields on it alone.  Not scale-dependent runs
+ * (MGPU_E_STATE).  src/2LPT.c:185-495 */
+int mgpu_batch_ic_displacement_fields(mgpu_ctx *const *members, int count, const unsigned long *seeds,
+                                      int sphere_mode, int amplitude_fixed, const int *inverted,
+                                      const double *pk_n2);
 /* mgpu_step for every member of a batch at once (members in the order of
  * mgpu_batch_create): each kernel of the step is one launch over all members
  * (grid z = member), so small meshes fill the device.  Every member must meet

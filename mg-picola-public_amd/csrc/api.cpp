@@ -822,18 +822,41 @@ int mgpu_batch_create(const mgpu_params *p, int count, mgpu_ctx **members) {
   return MGPU_OK;
 }
 
-int mgpu_batch_step(mgpu_ctx *const *members, int count, const mgpu_mg *mg, double dda, double A, double s1k,
-                    double s2k, double dyyy, double s1d, double s2d, int graph) {
+namespace {
+// all the contexts of one mgpu_batch_create, in order
+int batch_members(mgpu_ctx *const *members, int count) {
   if (!members || count < 1 || !members[0]) return mgpu_fail(nullptr, MGPU_E_INVALID, "bad batch arguments");
   mgpu_ctx *m0 = members[0];
   mgpu_arena *arena = m0->arena;
+  for (int r = 0; arena && arena->count == count && r < count; r++) {
+    const mgpu_ctx *m = members[r];
+    if (!m || m->arena != arena || m->pos != (float *)((char *)m0->pos + arena->stride * r)) arena = nullptr;
+  }
   if (!arena || arena->count != count)
     return mgpu_fail(m0, MGPU_E_INVALID, "the members must be all the contexts of one mgpu_batch_create, in order");
-  int rc;
+  return MGPU_OK;
+}
+}  // namespace
+
+int mgpu_batch_ic_displacement_fields(mgpu_ctx *const *members, int count, const unsigned long *seeds,
+                                      int sphere_mode, int amplitude_fixed, const int *inverted,
+                                      const double *pk_n2) {
+  int rc = batch_members(members, count);
+  if (rc) return rc;
+  if (!seeds || !inverted || !pk_n2) return mgpu_fail(members[0], MGPU_E_INVALID, "null argument");
+  for (int r = 0; r < count; r++)
+    if (members[r]->fft.tcols != members[0]->fft.tcols)
+      return mgpu_fail(members[0], MGPU_E_STATE, "the members' FFT tile widths differ");
+  return ic_displacement_fields_batch(members, count, seeds, sphere_mode, amplitude_fixed, inverted, pk_n2);
+}
+
+int mgpu_batch_step(mgpu_ctx *const *members, int count, const mgpu_mg *mg, double dda, double A, double s1k,
+                    double s2k, double dyyy, double s1d, double s2d, int graph) {
+  int rc = batch_members(members, count);
+  if (rc) return rc;
+  mgpu_ctx *m0 = members[0];
   for (int r = 0; r < count; r++) {
     mgpu_ctx *m = members[r];
-    if (!m || m->arena != arena || m->pos != (float *)((char *)m0->pos + arena->stride * r))
-      return mgpu_fail(m0, MGPU_E_INVALID, "the members must be all the contexts of one mgpu_batch_create, in order");
     if ((rc = step_check(m, mg))) return mgpu_fail(m0, rc, "member " + std::to_string(r) + ": " + m->err);
     if (m->fft.tcols != m0->fft.tcols) return mgpu_fail(m0, MGPU_E_STATE, "the members' FFT tile widths differ");
   }

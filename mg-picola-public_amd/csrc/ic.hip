@@ -34,7 +34,12 @@ unsigned *g_scratch_ptr[kMaxDevices] = {};
 // one thread per column (i, j) of the N x N grid
 __global__ __launch_bounds__(kBlock) void k_ic_delta(const unsigned *__restrict__ seedtable, int N, int Np, int HP,
                                                      double box, int sphere_mode, int amp_fixed, int inverted,
-                                                     const double *__restrict__ pk_n2, float2 *__restrict__ cd) {
+                                                     const double *__restrict__ pk_n2, float2 *__restrict__ cd,
+                                                     size_t zs, size_t seed_stride, const int *__restrict__ inv_z) {
+  // a batch: realization blockIdx.z's seed table and inversion flag
+  cd = zbuf(cd, zs);
+  seedtable += (size_t)blockIdx.z * seed_stride;
+  if (inv_z) inverted = inv_z[blockIdx.z];
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= N * N) return;
   const int i = col / N, j = col % N;
@@ -100,7 +105,10 @@ __global__ __launch_bounds__(kBlock) void k_lpt_spectrum(int mode, const float2 
                                                          const float2 *__restrict__ cd2, float2 *__restrict__ out,
                                                          int N, int HP, double kf, int a, int b,
                                                          const double *__restrict__ g1,
-                                                         const double *__restrict__ g2) {
+                                                         const double *__restrict__ g2, size_t zs) {
+  cd = zbuf(cd, zs);
+  cd2 = zbuf(cd2, zs);
+  out = zbuf(out, zs);
   size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (size_t)N * N * HP) return;
   const int k = (int)(e % HP);
@@ -152,14 +160,21 @@ __global__ __launch_bounds__(kBlock) void k_lpt_spectrum(int mode, const float2 
 // call), src -= d*d (off-diagonal calls).  src/2LPT.c:1282-1290
 // src may be d00 (each element is read before it is written)
 __global__ __launch_bounds__(kBlock) void k_source_diag(const float *d00, const float *__restrict__ d11,
-                                                        const float *__restrict__ d22, float *src, size_t n) {
+                                                        const float *__restrict__ d22, float *src, size_t n,
+                                                        size_t zs) {
+  d00 = zbuf(d00, zs);
+  d11 = zbuf(d11, zs);
+  d22 = zbuf(d22, zs);
+  src = zbuf(src, zs);
   size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   const double a = d00[e], b = d11[e], c = d22[e];
   src[e] = (float)(a * (b + c) + b * c);
 }
 __global__ __launch_bounds__(kBlock) void k_source_off(const float *__restrict__ d, float *__restrict__ src,
-                                                       size_t n) {
+                                                       size_t n, size_t zs) {
+  d = zbuf(d, zs);
+  src = zbuf(src, zs);
   size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   const double v = d[e];
@@ -169,7 +184,10 @@ __global__ __launch_bounds__(kBlock) void k_source_off(const float *__restrict__
 // trilinear lattice read-out, src/2LPT.c:1407-1476.  out[q] = scale * value
 __global__ __launch_bounds__(kBlock) void k_lattice_read(const float *__restrict__ m, int N, int Np, int HP2,
                                                          double scale, float *__restrict__ out,
-                                                         double *__restrict__ partial) {
+                                                         double *__restrict__ partial, size_t zs) {
+  m = zbuf(m, zs);
+  out = zbuf(out, zs);
+  partial = zbuf(partial, zs);
   __shared__ double red[kBlock];
   const size_t np = (size_t)Np * Np * Np;
   size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -203,14 +221,19 @@ __global__ __launch_bounds__(kBlock) void k_lattice_read(const float *__restrict
 }
 
 __global__ __launch_bounds__(kBlock) void k_sub_mean(float *__restrict__ v, size_t n, const double *__restrict__ sum,
-                                                     double inv_n) {
+                                                     double inv_n, size_t zs) {
+  v = zbuf(v, zs);
+  sum = zbuf(sum, zs);
   size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
   v[q] = (float)((double)v[q] - sum[0] * inv_n);
 }
 
 // sum partial[b*3] over blocks in a fixed order -> out[0]
-__global__ __launch_bounds__(1024) void k_reduce1(const double *__restrict__ partial, int nb, double *__restrict__ out) {
+__global__ __launch_bounds__(1024) void k_reduce1(const double *__restrict__ partial, int nb, double *__restrict__ out,
+                                                  size_t zs) {
+  partial = zbuf(partial, zs);
+  out = zbuf(out, zs);
   __shared__ double red[1024];
   double s = 0;
   for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partial[(size_t)b * 3];
@@ -554,8 +577,8 @@ int spectrum(mgpu_ctx *ctx, int mode, const float2 *cd, const float2 *cd2, float
   const double modes = (double)ctx->N * ctx->N * (ctx->N / 2 + 1);
   {
     KScope ks(ctx, MGPU_K_LPT, 8.0 * modes * (cd2 ? 3 : 2));
-    hipLaunchKernelGGL(k_lpt_spectrum, dim3(cdiv(ctx->slot_elems, kBlock)), dim3(kBlock), 0, ctx->stream, mode, cd,
-                       cd2, out, ctx->N, ctx->HP, kf, a, b, g1, g2);
+    hipLaunchKernelGGL(k_lpt_spectrum, dim3(cdiv(ctx->slot_elems, kBlock), 1, ctx->bz), dim3(kBlock), 0, ctx->stream,
+                       mode, cd, cd2, out, ctx->N, ctx->HP, kf, a, b, g1, g2, ctx->zs);
   }
   MGPU_HIP(ctx, hipGetLastError());
   return MGPU_OK;
@@ -568,13 +591,15 @@ int readout(mgpu_ctx *ctx, const float2 *mesh, double scale, float *dst) {
   {
     // compulsory: one mesh sample read and one float written per lattice point
     KScope ks(ctx, MGPU_K_READOUT, 8.0 * np);
-    hipLaunchKernelGGL(k_lattice_read, dim3(nb), dim3(kBlock), 0, ctx->stream, (const float *)mesh, ctx->N, ctx->Np,
-                       2 * ctx->HP, scale, dst, ctx->partial);
+    hipLaunchKernelGGL(k_lattice_read, dim3(nb, 1, ctx->bz), dim3(kBlock), 0, ctx->stream, (const float *)mesh, ctx->N,
+                       ctx->Np, 2 * ctx->HP, scale, dst, ctx->partial, ctx->zs);
   }
   MGPU_HIP(ctx, hipGetLastError());
-  hipLaunchKernelGGL(k_reduce1, dim3(1), dim3(1024), 0, ctx->stream, ctx->partial, (int)nb, ctx->dsum);
+  hipLaunchKernelGGL(k_reduce1, dim3(1, 1, ctx->bz), dim3(1024), 0, ctx->stream, ctx->partial, (int)nb, ctx->dsum,
+                     ctx->zs);
   MGPU_HIP(ctx, hipGetLastError());
-  hipLaunchKernelGGL(k_sub_mean, dim3(nb), dim3(kBlock), 0, ctx->stream, dst, np, ctx->dsum, 1.0 / (double)np);
+  hipLaunchKernelGGL(k_sub_mean, dim3(nb, 1, ctx->bz), dim3(kBlock), 0, ctx->stream, dst, np, ctx->dsum,
+                     1.0 / (double)np, ctx->zs);
   MGPU_HIP(ctx, hipGetLastError());
   return MGPU_OK;
 }
@@ -625,11 +650,20 @@ int ic_displacement_fields(mgpu_ctx *ctx, unsigned long seed, int sphere_mode, i
   float2 *cd = ctx->p.scale_dependent ? ctx->cdelta[0] : ctx->slot[0];
   MGPU_HIP(ctx, hipMemsetAsync(cd, 0, ctx->slot_elems * sizeof(float2), ctx->stream));
   hipLaunchKernelGGL(k_ic_delta, dim3(cdiv((size_t)N * N, kBlock)), dim3(kBlock), 0, ctx->stream, dseed, N, ctx->Np,
-                     ctx->HP, ctx->p.box, sphere_mode, amp_fixed, inverted, ctx->n2tab[0], cd);
+                     ctx->HP, ctx->p.box, sphere_mode, amp_fixed, inverted, ctx->n2tab[0], cd, (size_t)0, (size_t)0,
+                     (const int *)nullptr);
   MGPU_HIP(ctx, hipGetLastError());
   MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (!seed_dev) MGPU_HIP(ctx, hipFree(dseed));
+  return ic_lpt_from_delta(ctx, cd);
+}
 
+// From delta(k) in cd on: the 2LPT source and the ZA and 2LPT lattice
+// displacements (or, scale-dependent, the stored -source).  Every launch covers
+// ctx->bz realizations (a batch's member 0 with bz = count).
+int ic_lpt_from_delta(mgpu_ctx *ctx, float2 *cd) {
+  int rc;
+  const int N = ctx->N;
   // displacement gradient -> 2LPT source (real), built in slot 1 over the
   // psi_00 gradient (the diagonal gradients in slots 1-3)
   const size_t nr = 2 * ctx->slot_elems;
@@ -639,15 +673,15 @@ int ic_displacement_fields(mgpu_ctx *ctx, unsigned long seed, int sphere_mode, i
     if ((rc = spectrum(ctx, SPEC_GRAD, cd, nullptr, ctx->slot[1 + g], ga[g], ga[g]))) return rc;
     if ((rc = fft_c2r_3d(ctx, &ctx->fft, ctx->slot[1 + g]))) return rc;
   }
-  hipLaunchKernelGGL(k_source_diag, dim3(gr), dim3(kBlock), 0, ctx->stream, (const float *)ctx->slot[1],
-                     (const float *)ctx->slot[2], (const float *)ctx->slot[3], (float *)ctx->slot[1], nr);
+  hipLaunchKernelGGL(k_source_diag, dim3(gr, 1, ctx->bz), dim3(kBlock), 0, ctx->stream, (const float *)ctx->slot[1],
+                     (const float *)ctx->slot[2], (const float *)ctx->slot[3], (float *)ctx->slot[1], nr, ctx->zs);
   MGPU_HIP(ctx, hipGetLastError());
   static const int oa[3] = {0, 0, 1}, ob[3] = {1, 2, 2};
   for (int g = 0; g < 3; g++) {
     if ((rc = spectrum(ctx, SPEC_GRAD, cd, nullptr, ctx->slot[3], oa[g], ob[g]))) return rc;
     if ((rc = fft_c2r_3d(ctx, &ctx->fft, ctx->slot[3]))) return rc;
-    hipLaunchKernelGGL(k_source_off, dim3(gr), dim3(kBlock), 0, ctx->stream, (const float *)ctx->slot[3],
-                       (float *)ctx->slot[1], nr);
+    hipLaunchKernelGGL(k_source_off, dim3(gr, 1, ctx->bz), dim3(kBlock), 0, ctx->stream, (const float *)ctx->slot[3],
+                       (float *)ctx->slot[1], nr, ctx->zs);
     MGPU_HIP(ctx, hipGetLastError());
   }
   if ((rc = fft_r2c_3d(ctx, &ctx->fft, ctx->slot[1]))) return rc;
@@ -666,6 +700,49 @@ int ic_displacement_fields(mgpu_ctx *ctx, unsigned long seed, int sphere_mode, i
     if ((rc = fft_c2r_3d(ctx, &ctx->fft, ctx->slot[2]))) return rc;
     if ((rc = readout(ctx, ctx->slot[2], -3. / 7. * inv_n3, ctx->lpt[1] + (size_t)a * ctx->np))) return rc;
   }
+  MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MGPU_OK;
+}
+
+int ic_displacement_fields_batch(mgpu_ctx *const *m, int count, const unsigned long *seeds, int sphere_mode,
+                                 int amp_fixed, const int *inverted, const double *pk_n2) {
+  mgpu_ctx *ctx = m[0];
+  PhaseTimer pt(ctx, &ctx->t.ic);
+  if (ctx->p.scale_dependent)
+    return mgpu_fail(ctx, MGPU_E_STATE, "batched initial conditions: not for scale-dependent runs");
+  const int N = ctx->N;
+  const size_t nn = (size_t)N * N;
+  // all seed tables, then the inversion flags, in one upload
+  std::vector<unsigned> all(count * nn + count), st;
+  for (int r = 0; r < count; r++) {
+    seedtable_host(N, seeds[r], st);
+    std::copy(st.begin(), st.end(), all.begin() + r * nn);
+    all[count * nn + r] = inverted[r] ? 1u : 0u;
+  }
+  unsigned *dbuf = nullptr;
+  MGPU_HIP(ctx, hipMalloc(&dbuf, all.size() * sizeof(unsigned)));
+  struct Free {
+    unsigned *p;
+    ~Free() { (void)hipFree(p); }
+  } free_dbuf{dbuf};
+  MGPU_HIP(ctx, hipMemcpyAsync(dbuf, all.data(), all.size() * sizeof(unsigned), hipMemcpyHostToDevice, ctx->stream));
+  int rc;
+  for (int r = 0; r < count; r++) {
+    // every member holds the table (later calls on a member alone read its own)
+    if ((rc = upload_n2(m[r], 0, pk_n2))) return rc;
+    MGPU_HIP(ctx, hipMemsetAsync(m[r]->slot[0], 0, ctx->slot_elems * sizeof(float2), ctx->stream));
+  }
+  struct Restore {
+    mgpu_ctx *c;
+    ~Restore() { c->bz = 1; }
+  } restore{ctx};
+  ctx->bz = (unsigned)count;
+  float2 *cd = ctx->slot[0];
+  hipLaunchKernelGGL(k_ic_delta, dim3(cdiv(nn, kBlock), 1, count), dim3(kBlock), 0, ctx->stream, dbuf, N, ctx->Np,
+                     ctx->HP, ctx->p.box, sphere_mode, amp_fixed, 0, ctx->n2tab[0], cd, ctx->zs, nn,
+                     (const int *)(dbuf + count * nn));
+  MGPU_HIP(ctx, hipGetLastError());
+  if ((rc = ic_lpt_from_delta(ctx, cd))) return rc;
   MGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return MGPU_OK;
 }

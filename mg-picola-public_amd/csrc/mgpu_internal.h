@@ -280,6 +280,11 @@ int ic_displacement_fields(mgpu_ctx *ctx, unsigned long seed, int sphere_mode, i
                            const double *pk_n2, unsigned *seed_dev = nullptr);
 // looks up this device's seed-table scratch (ctx->ic_scratch); called by mgpu_create
 int ic_scratch_init(mgpu_ctx *ctx);
+// the IC from delta(k) in cd on (2LPT source, lattice displacements), over ctx->bz realizations
+int ic_lpt_from_delta(mgpu_ctx *ctx, float2 *cd);
+// ic_displacement_fields for every member of a batch (m[0..count-1] in order), one launch per kernel
+int ic_displacement_fields_batch(mgpu_ctx *const *m, int count, const unsigned long *seeds, int sphere_mode,
+                                 int amp_fixed, const int *inverted, const double *pk_n2);
 int ic_init_particles(mgpu_ctx *ctx, double Di, double Di2);
 int ic_lpt_fields(mgpu_ctx *ctx, int slot, const double *g1_n2, const double *g2_n2);
 int ic_lpt_fields_slices(mgpu_ctx *ctx, int slot, const mgpu_kslice *o1, const mgpu_kslice *o2);

@@ -49,8 +49,10 @@ typedef struct mgp_sim sim;
  * iteration in which member 0 last stepped all of them with mgpu_batch_step. */
 struct batch_rec {
   mgpu_ctx **ctxs;
+  struct mgp_sim **sims;
   int count, refs;
   long stepped_iter;
+  int ic_done; /* member 0 made every member's initial conditions with one batched IC */
 };
 
 static double now_s(void) {
@@ -443,11 +445,13 @@ int mgp_sim_create_batch(mgp_params **ps, int count, int verbose, mgp_sim **out,
   mgpu_params gp0, gp;
   struct batch_rec *b = calloc(1, sizeof *b);
   b->ctxs = calloc(count, sizeof(mgpu_ctx *));
+  b->sims = calloc(count, sizeof(struct mgp_sim *));
   b->count = count;
   b->stepped_iter = -1;
   int rc = 0;
   for (int r = 0; r < count && !rc; r++) {
     out[r] = sim_alloc(ps[r], verbose, r ? &gp : &gp0);
+    b->sims[r] = out[r];
     out[r]->brec = b;
     out[r]->batch_idx = r;
     b->refs++;
@@ -495,6 +499,7 @@ void mgp_sim_free(mgp_sim *s) {
   if (!s) return;
   if (s->g) mgpu_destroy(s->g);
   if (s->brec && --s->brec->refs == 0) {
+    free(s->brec->sims);
     free(s->brec->ctxs);
     free(s->brec);
   }
@@ -545,8 +550,38 @@ int mgp_sim_ic(mgp_sim *s) {
   s->AI = s->AF = s->AFF = s->A;
   double *pk = malloc(sizeof(double) * n2len);
   mgp_ic_pk_table(s->c, pk);
-  int rc = mgpu_ic_displacement_fields(s->g, (unsigned long)p->Seed, p->SphereMode,
+  int rc = 0;
+  struct batch_rec *b = s->brec;
+  if (b && b->ic_done && s->batch_idx > 0) {
+    /* made by member 0's batched IC */
+  } else if (b && s->batch_idx == 0 && !p->scale_dependent) {
+    /* one batched IC for all members when they share the sphere mode and the amplitude fixing */
+    int same = 1;
+    for (int r = 1; r < b->count; r++) {
+      const mgp_params *q = b->sims[r]->p;
+      same &= q->SphereMode == p->SphereMode &&
+              q->amplitude_fixed_initial_condition == p->amplitude_fixed_initial_condition;
+    }
+    if (same) {
+      unsigned long *seeds = malloc(sizeof(unsigned long) * b->count);
+      int *inv = malloc(sizeof(int) * b->count);
+      for (int r = 0; r < b->count; r++) {
+        seeds[r] = (unsigned long)b->sims[r]->p->Seed;
+        inv[r] = b->sims[r]->p->inverted_initial_condition;
+      }
+      rc = mgpu_batch_ic_displacement_fields(b->ctxs, b->count, seeds, p->SphereMode,
+                                             p->amplitude_fixed_initial_condition, inv, pk);
+      free(seeds);
+      free(inv);
+      b->ic_done = rc == 0;
+    } else {
+      rc = mgpu_ic_displacement_fields(s->g, (unsigned long)p->Seed, p->SphereMode,
                                        p->amplitude_fixed_initial_condition, p->inverted_initial_condition, pk);
+    }
+  } else {
+    rc = mgpu_ic_displacement_fields(s->g, (unsigned long)p->Seed, p->SphereMode,
+                                     p->amplitude_fixed_initial_condition, p->inverted_initial_condition, pk);
+  }
   free(pk);
   if (rc) {
     snprintf(s->err, s->errlen, "IC generation failed: %s", mgpu_last_error(s->g));

@@ -30,7 +30,7 @@ MGPU_SYMBOLS = [
     "mgpu_lpt_fields", "mgpu_get_mesh", "mgpu_set_mesh", "mgpu_get_timers", "mgpu_enable_timers",
     "mgpu_kernel_name", "mgpu_enable_kernel_timing", "mgpu_kernel_timing", "mgpu_lpt_fields_slices",
     "mgpu_lpt_fields_slices2", "mgpu_ic_displacement_fields_noalloc", "mgpu_memory_plan", "mgpu_step",
-    "mgpu_get_means", "mgpu_batch_create", "mgpu_batch_step",
+    "mgpu_get_means", "mgpu_batch_create", "mgpu_batch_step", "mgpu_batch_ic_displacement_fields",
 ]
 
 
@@ -388,6 +388,22 @@ class MGPU:
         assert pk.size == n2_table_len(self.n)
         self._ck(lib().mgpu_ic_displacement_fields(self.ctx, seed, sphere_mode, amplitude_fixed, inverted,
                                                    _ptr(pk)), "ic_displacement_fields")
+
+    @staticmethod
+    def batch_ic(members, seeds, pk_n2, sphere_mode=0, amplitude_fixed=0, inverted=None):
+        """mgpu_ic_displacement_fields for all members of one batch, one launch per kernel."""
+        L = lib()
+        n = len(members)
+        pk = np.ascontiguousarray(pk_n2, np.float64)
+        assert pk.size == n2_table_len(members[0].n)
+        arr = (C.c_void_p * n)(*[m.ctx.value for m in members])
+        sd = (C.c_ulong * n)(*seeds)
+        inv = (C.c_int * n)(*(inverted if inverted is not None else [0] * n))
+        L.mgpu_batch_ic_displacement_fields.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int,
+                                                        C.c_void_p, C.c_void_p]
+        rc = L.mgpu_batch_ic_displacement_fields(arr, n, sd, sphere_mode, amplitude_fixed, inv, _ptr(pk))
+        if rc:
+            raise MGPUError(f"batch_ic failed ({rc}): {L.mgpu_last_error(members[0].ctx).decode()}")
 
     def init_particles(self, Di, Di2):
         self._ck(lib().mgpu_init_particles(self.ctx, Di, Di2), "init_particles")

@@ -168,6 +168,35 @@ def test_batch_step_equals_solo_steps(model, n, ns, count):
     assert not np.array_equal(solo[0][0], solo[1][0])
 
 
+@pytest.mark.parametrize("n,ns,amp,sphere", [(32, 16, 1, 0), (48, 24, 0, 1)])
+def test_batch_ic_equals_solo_ic(n, ns, amp, sphere):
+    """mgpu_batch_ic_displacement_fields (one launch per kernel over the members, each with its own seed and
+    inversion) gives every member the ZA and 2LPT lattice displacements of mgpu_ic_displacement_fields on it
+    alone, bit for bit."""
+    box = 2.0 * ns
+    kf = 2 * np.pi / box
+    n2 = np.arange(M.n2_table_len(n), dtype=np.float64)
+    k = kf * np.sqrt(n2)
+    pk = np.where(n2 > 0, 3e3 * k / (1 + (k / 0.05) ** 2.5), 0.0)
+    seeds, inv = [777, 777, 901, 55], [0, 1, 0, 1]
+    kw = dict(model=M.DGP, mg_active=1, include_screening=1)
+    solo = []
+    for sd, iv in zip(seeds, inv):
+        g = M.MGPU(n, ns, box, 0.3, **kw)
+        g.ic_displacement_fields(sd, pk, sphere_mode=sphere, amplitude_fixed=amp, inverted=iv)
+        solo.append((g.get_lpt(0), g.get_lpt(1)))
+        g.close()
+    members = M.MGPU.batch(len(seeds), n, ns, box, 0.3, **kw)
+    M.MGPU.batch_ic(members, seeds, pk, sphere_mode=sphere, amplitude_fixed=amp, inverted=inv)
+    for r, g in enumerate(members):
+        np.testing.assert_array_equal(g.get_lpt(0), solo[r][0])
+        np.testing.assert_array_equal(g.get_lpt(1), solo[r][1])
+    for g in members:
+        g.close()
+    assert not np.array_equal(solo[0][0], solo[2][0])
+    np.testing.assert_array_equal(solo[1][0], -solo[0][0])
+
+
 def test_batch_step_refuses_foreign_members():
     a = M.MGPU.batch(2, 32, 16, 32.0, 0.3)
     b = M.MGPU(32, 16, 32.0, 0.3)
