@@ -90,15 +90,18 @@ __device__ __forceinline__ unsigned bin_key(const float *__restrict__ pos, const
 }
 
 // 1. bin key of every particle, 8 tile + face class (class bits: x, y, z local
-// coordinate = T - 1), and its rank inside the bin.  The lanes of a wave with
-// equal keys are grouped first (ballots only); then the group leaders add
-// their group sizes in one atomic instruction.  Each thread takes kBinU
-// particles (block chunks of kBinU kTB, coalesced per u) whose groups are all
-// formed before their atomics are issued back to back, so a wave waits for one
-// atomic round trip per kBinU particles.  Only the rank is stored: the scatter
-// forms the key again from the position.
+// coordinate = T - 1), and its rank inside the bin.  Each thread takes kBinU
+// particles (workgroup chunks of kBinU kTB, coalesced per u).  The workgroup's
+// keys go into a hash table in LDS, where each particle takes a local rank from
+// an LDS atomic; then one global atomic per distinct key of the workgroup adds
+// its count and returns the base of its ranks.  Only the rank is stored: the
+// scatter forms the key again from the position.
 constexpr int kBinU = 4;
 
+// LDS hash slots of a workgroup's keys (kBinU kTB particles: at most half full, so
+// every probe sequence ends at a free or matching slot)
+constexpr int kBinHBits = 11, kBinH = 1 << kBinHBits;
+static_assert(kBinH >= 2 * kBinU * kTB, "hash table too small for a workgroup's keys");
 template <bool RSD>
 __global__ __launch_bounds__(kTB) void k_dep_bin(const float *__restrict__ pos, const float4 *__restrict__ crd,
                                                  size_t np, TileGeom g, double scale, unsigned *__restrict__ count,
@@ -107,45 +110,50 @@ __global__ __launch_bounds__(kTB) void k_dep_bin(const float *__restrict__ pos, 
   crd = zbuf(crd, zs);
   count = zbuf(count, zs);
   rank = zbuf(rank, zs);
+  constexpr unsigned kEmpty = 0xffffffffu;  // bin keys are below 8 * tiles < 2^32 - 1
+  __shared__ unsigned hkey[kBinH], hcnt[kBinH];
+  for (int s = threadIdx.x; s < kBinH; s += kTB) {
+    hkey[s] = kEmpty;
+    hcnt[s] = 0u;
+  }
+  __syncthreads();
   const size_t i0 = (size_t)blockIdx.x * (kBinU * kTB) + threadIdx.x;
-  const int lane = threadIdx.x % warpSize;
-  unsigned t[kBinU], off_in[kBinU], gsize[kBinU], base[kBinU];
-  int leader_of[kBinU];
+  unsigned t[kBinU], loc[kBinU];
+  int slot[kBinU];
 #pragma unroll
   for (int u = 0; u < kBinU; u++) {
     const size_t i = i0 + (size_t)u * kTB;
     t[u] = i < np ? bin_key<RSD>(pos, crd, np, i, g, scale) : 0u;
+    slot[u] = -1;
+    loc[u] = 0u;
   }
+  // the workgroup's keys into the LDS table (linear probing), a local rank per key
 #pragma unroll
   for (int u = 0; u < kBinU; u++) {
-    const bool valid = i0 + (size_t)u * kTB < np;
-    bool pending = valid;
-    leader_of[u] = lane;
-    off_in[u] = 0;
-    gsize[u] = 0;
-    for (unsigned long long rem = __ballot(pending); rem;) {
-      const int leader = __ffsll((long long)rem) - 1;
-      const unsigned tl = __shfl(t[u], leader);
-      const unsigned long long same = __ballot(pending && t[u] == tl);
-      if (pending && t[u] == tl) {
-        leader_of[u] = leader;
-        off_in[u] = (unsigned)__popcll(same & ((1ull << lane) - 1ull));
-        pending = false;
+    if (i0 + (size_t)u * kTB < np) {
+      unsigned h = (t[u] * 2654435761u) >> (32 - kBinHBits);
+      for (int p = 0; p < kBinH; ++p) {
+        const unsigned prev = atomicCAS(&hkey[h], kEmpty, t[u]);
+        if (prev == kEmpty || prev == t[u]) {
+          slot[u] = (int)h;
+          break;
+        }
+        h = (h + 1) & (kBinH - 1);
       }
-      if (lane == leader) gsize[u] = (unsigned)__popcll(same);
-      rem &= ~same;
+      if (slot[u] >= 0) loc[u] = atomicAdd(&hcnt[slot[u]], 1u);
     }
   }
-#pragma unroll
-  for (int u = 0; u < kBinU; u++) {
-    base[u] = 0;
-    if (i0 + (size_t)u * kTB < np && lane == leader_of[u]) base[u] = atomicAdd(&count[t[u]], gsize[u]);
+  __syncthreads();
+  // one global atomic per distinct key of the workgroup: its base replaces the count
+  for (int s = threadIdx.x; s < kBinH; s += kTB) {
+    const unsigned k = hkey[s];
+    if (k != kEmpty) hcnt[s] = atomicAdd(&count[k], hcnt[s]);
   }
+  __syncthreads();
 #pragma unroll
   for (int u = 0; u < kBinU; u++) {
     const size_t i = i0 + (size_t)u * kTB;
-    const unsigned b = __shfl(base[u], leader_of[u]);
-    if (i < np) rank[i] = b + off_in[u];
+    if (i < np) rank[i] = slot[u] >= 0 ? hcnt[slot[u]] + loc[u] : atomicAdd(&count[t[u]], 1u);
   }
 }
 
